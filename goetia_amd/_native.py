@@ -1,0 +1,166 @@
+"""ctypes binding of libgoetia_amd.so (the C ABI in include/goetia_amd.h).
+
+The library is built in-tree by ``python -m goetia_amd.build``.  There is no
+fallback: if the library is missing, or a device entry point runs without a
+GPU, the call raises.
+"""
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libgoetia_amd.so')
+HEADER = os.path.join(os.path.dirname(HERE), 'include', 'goetia_amd.h')
+
+GT_OK = 0
+GT_ERR_INVALID_ARG = 1
+GT_ERR_SEQUENCE_LENGTH = 2
+GT_ERR_OUT_OF_MEMORY = 3
+GT_ERR_DEVICE = 4
+GT_ERR_UNSUPPORTED = 5
+GT_ERR_NO_DEVICE = 6
+GT_ERR_IO = 7
+GT_ERR_INTERNAL = 8
+
+BIT, BYTE, EXACT = 1, 2, 3
+MODE_RAW, MODE_SKIP_N = 0, 1
+
+
+class GoetiaException(Exception):
+    """goetia::GoetiaException (goetia.hh:140-153)."""
+
+
+class SequenceLengthException(GoetiaException):
+    """goetia::SequenceLengthException (exceptions.hh:23-28)."""
+
+
+class InvalidSequenceException(GoetiaException):
+    """goetia::InvalidSequenceException (raised by hash_base on short input)."""
+
+
+class UninitializedShifterException(GoetiaException):
+    """goetia::UninitializedShifterException (hashshifter.hh:133-145)."""
+
+
+class DeviceError(GoetiaException):
+    """The GPU path failed or no GPU is available."""
+
+
+class UnsupportedError(GoetiaException):
+    pass
+
+
+_ERRORS = {
+    GT_ERR_INVALID_ARG: ValueError,
+    GT_ERR_SEQUENCE_LENGTH: SequenceLengthException,
+    GT_ERR_OUT_OF_MEMORY: MemoryError,
+    GT_ERR_DEVICE: DeviceError,
+    GT_ERR_UNSUPPORTED: UnsupportedError,
+    GT_ERR_NO_DEVICE: DeviceError,
+    GT_ERR_IO: OSError,
+    GT_ERR_INTERNAL: GoetiaException,
+}
+
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+i64p = C.POINTER(C.c_int64)
+i16p = C.POINTER(C.c_int16)
+vp = C.c_void_p
+st_p = C.c_void_p
+u16 = C.c_uint16
+u32 = C.c_uint32
+u64 = C.c_uint64
+cint = C.c_int
+
+# name -> argument types (all return int status, except gt_last_error)
+SIGNATURES = {
+    'gt_abi_version': [],
+    'gt_device_init': [cint],
+    'gt_device_synchronize': [],
+    'gt_primes_near_x': [u32, u64, u64p, u32p],
+    'gt_storage_new': [cint, u64p, u32, u64, C.POINTER(vp)],
+    'gt_storage_free': [st_p],
+    'gt_storage_clone': [st_p, C.POINTER(vp)],
+    'gt_storage_reset': [st_p],
+    'gt_storage_kind': [st_p, C.POINTER(cint)],
+    'gt_storage_n_tables': [st_p, u32p],
+    'gt_storage_tablesizes': [st_p, u64p],
+    'gt_storage_table_bytes': [st_p, u32, u64p],
+    'gt_storage_counters': [st_p, u64p, u64p],
+    'gt_storage_set_counters': [st_p, u64, u64],
+    'gt_storage_read_table': [st_p, u32, vp],
+    'gt_storage_write_table': [st_p, u32, vp],
+    'gt_storage_update_from': [st_p, st_p],
+    'gt_storage_exact_keys': [st_p, vp, u64, u64p],
+    'gt_kmer_offsets_dev': [u16, vp, u64, vp, vp],
+    'gt_hash_reads_dev': [u16, cint, vp, vp, u64, u64, vp, vp, vp, vp],
+    'gt_hash_reads': [u16, cint, vp, vp, u64, vp, vp, vp],
+    'gt_shifter_hash_base': [u16, vp, u64p, u64p],
+    'gt_shifter_shift_right': [u16, u64p, u64p, C.c_uint8, C.c_uint8],
+    'gt_shifter_shift_left': [u16, u64p, u64p, C.c_uint8, C.c_uint8],
+    'gt_dbg_insert_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
+    'gt_dbg_insert': [st_p, u16, cint, cint, vp, vp, u64, vp, vp, u64p],
+    'gt_dbg_query_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
+    'gt_dbg_query': [st_p, u16, cint, cint, vp, vp, u64, vp, vp],
+    'gt_dbg_insert_and_query_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp],
+    'gt_dbg_insert_and_query': [st_p, u16, cint, cint, vp, vp, u64, vp],
+    'gt_dbg_degrees_dev': [st_p, u16, cint, cint, cint, vp, vp, u64, u64, vp, vp, vp, vp],
+    'gt_dbg_degrees': [st_p, u16, cint, cint, cint, vp, vp, u64, vp, vp, vp],
+    'gt_insert_hashes_dev': [st_p, vp, u64, vp, vp],
+    'gt_insert_hashes': [st_p, vp, u64, vp],
+    'gt_query_hashes_dev': [st_p, vp, u64, vp, vp],
+    'gt_query_hashes': [st_p, vp, u64, vp],
+    'gt_insert_and_query_hashes': [st_p, vp, u64, vp],
+    'gt_gen_genome': [u64, u64, vp],
+    'gt_gen_reads': [u64, vp, u64, u64, vp, u32, u32, C.c_double, C.c_double, vp, vp, u64p],
+}
+
+_lib = None
+
+
+def header_symbols():
+    """Names of the entry points declared in include/goetia_amd.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r'^(?:int|const char\*)\s+(gt_\w+)\(', text, re.M)))
+
+
+def load():
+    """Loads libgoetia_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f'{LIB_PATH} is missing: build it with `python -m goetia_amd.build` '
+                          '(goetia_amd has no CPU fallback)')
+    lib = C.CDLL(LIB_PATH)
+    lib.gt_last_error.restype = C.c_char_p
+    lib.gt_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what=''):
+    if rc != GT_OK:
+        msg = load().gt_last_error().decode(errors='replace')
+        exc = _ERRORS.get(rc, GoetiaException)
+        raise exc(f'{what}: {msg}' if what else msg)
+
+
+def call(name, *args):
+    rc = getattr(load(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def ptr(arr):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if arr is None:
+        return None
+    if hasattr(arr, 'data_ptr'):
+        return arr.data_ptr()
+    return arr.ctypes.data

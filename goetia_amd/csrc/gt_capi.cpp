@@ -1,0 +1,1184 @@
+// gt_capi.cpp -- C ABI of goetia_amd (declared in include/goetia_amd.h).
+//
+// Owns device storages (tables in HBM), per-storage scratch (claim map,
+// fresh masks, read layout), the sub-batch loop that keeps the first-touch
+// semantics of goetia's sequential inserts, and the host-pointer wrappers.
+#include "goetia_amd.h"
+
+#include "gt_hash.h"
+#include "gt_internal.h"
+#include "gt_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace gt;
+
+// ------------------------------------------------------------ errors
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define GT_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            return fail(GT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                                   \
+    } while (0)
+
+#define GT_TRY(call)             \
+    do {                         \
+        int rc_ = (call);        \
+        if (rc_ != GT_OK) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------ device
+
+struct DeviceState {
+    std::mutex mu;
+    bool ready = false;
+    int device = 0;
+    int cus = 0;
+    int grid = 0;
+};
+
+DeviceState g_dev;
+
+int ensure_device(int device = -1) {
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (g_dev.ready && (device < 0 || device == g_dev.device)) {
+        hipError_t e = hipSetDevice(g_dev.device);
+        if (e != hipSuccess) return fail(GT_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return GT_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        return fail(GT_ERR_NO_DEVICE, "goetia_amd needs an AMD GPU (gfx950); none is visible");
+    }
+    if (device < 0) device = 0;
+    if (device >= n) return fail(GT_ERR_INVALID_ARG, "device index out of range");
+    GT_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    GT_HIP(hipGetDeviceProperties(&prop, device));
+    g_dev.device = device;
+    g_dev.cus = prop.multiProcessorCount;
+    // Persistent tile grids: 8 workgroups of 4 waves per CU.
+    g_dev.grid = std::max(1, prop.multiProcessorCount * 8);
+    g_dev.ready = true;
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ device buffers
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // Grows (never shrinks); contents are not kept.
+    int ensure(size_t need) {
+        if (need <= bytes && p) return GT_OK;
+        release();
+        size_t sz = std::max<size_t>(need, 256);
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(sz) + " bytes failed: " +
+                                                  hipGetErrorString(e));
+        }
+        bytes = sz;
+        return GT_OK;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+uint32_t log2u(uint64_t p) {
+    uint32_t l = 0;
+    while ((1ULL << l) < p) ++l;
+    return l;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ storage
+
+struct gt_storage {
+    int kind = 0;
+    uint32_t n_tables = 0;
+    std::vector<uint64_t> sizes;       // bins per table
+    std::vector<uint64_t> nbytes;      // reference layout bytes per table
+    std::vector<uint64_t> table_off;   // offsets in d_tables (256-aligned)
+    DevBuf tables;
+    DevBuf counters;                   // kNCounters u64
+    // exact set
+    DevBuf ekeys, estamp, eones;
+    uint64_t ecap = 0;
+    uint64_t ecount_ub = 0;            // host upper bound on the set size
+    // scratch
+    DevBuf claim_keys, claim_vals;
+    uint64_t claim_cap = 0;
+    DevBuf fresh;
+    DevBuf kmer_off, tile_read, scan_blocks;
+    DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
+    uint32_t gen = 0;
+    std::recursive_mutex mu;
+
+    Tables tables_view() const {
+        Tables T{};
+        T.n = n_tables;
+        T.kind = kind;
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            T.p[i] = sizes[i];
+            T.m[i] = gt_mod_magic(sizes[i]);
+            T.base[i] = tables.as<uint8_t>() + table_off[i];
+        }
+        return T;
+    }
+    ExactSet exact_view() const {
+        ExactSet E{};
+        E.keys = ekeys.as<uint64_t>();
+        E.stamp = estamp.as<uint64_t>();
+        E.ones = eones.as<uint32_t>();
+        E.cap = ecap;
+        E.log2cap = log2u(ecap);
+        return E;
+    }
+    // Exact-set view usable by kernels of Bloom / count-min storages (never
+    // dereferenced there).
+    ExactSet exact_or_empty() const {
+        if (kind == kExact) return exact_view();
+        ExactSet E{};
+        E.cap = 1;
+        return E;
+    }
+};
+
+namespace {
+
+bool valid_storage(const gt_storage* st) { return st && st->kind >= kBit && st->kind <= kExact; }
+
+int alloc_exact(gt_storage* st, uint64_t cap, hipStream_t s) {
+    DevBuf keys, stamp;
+    GT_TRY(keys.ensure((cap + 1) * sizeof(uint64_t)));
+    GT_TRY(stamp.ensure((cap + 1) * sizeof(uint64_t)));
+    launch_fill_u64(keys.as<uint64_t>(), cap + 1, ~0ULL, s);
+    launch_fill_u64(stamp.as<uint64_t>(), cap + 1, 0, s);
+    GT_HIP(hipGetLastError());
+    if (st->ecap) {
+        ExactSet from = st->exact_view();
+        ExactSet to = from;
+        to.keys = keys.as<uint64_t>();
+        to.stamp = stamp.as<uint64_t>();
+        to.cap = cap;
+        to.log2cap = log2u(cap);
+        launch_exact_rehash(from, to, st->counters.as<uint64_t>(), s);
+        GT_HIP(hipGetLastError());
+        GT_HIP(hipStreamSynchronize(s));
+    }
+    std::swap(st->ekeys.p, keys.p);
+    std::swap(st->ekeys.bytes, keys.bytes);
+    std::swap(st->estamp.p, stamp.p);
+    std::swap(st->estamp.bytes, stamp.bytes);
+    st->ecap = cap;
+    return GT_OK;
+}
+
+int zero_tables(gt_storage* st, hipStream_t s) {
+    if (st->kind == kExact) {
+        launch_fill_u64(st->ekeys.as<uint64_t>(), st->ecap + 1, ~0ULL, s);
+        launch_fill_u64(st->estamp.as<uint64_t>(), st->ecap + 1, 0, s);
+        launch_fill_u64(st->eones.as<uint64_t>(), 1, 0, s);
+    } else {
+        launch_fill_u64(st->tables.as<uint64_t>(), st->tables.bytes / 8, 0, s);
+    }
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int read_counters(gt_storage* st, uint64_t* out, hipStream_t s) {
+    GT_HIP(hipMemcpyAsync(out, st->counters.p, kNCounters * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+int check_kernel_errors(gt_storage* st, hipStream_t s) {
+    uint64_t c[kNCounters];
+    GT_TRY(read_counters(st, c, s));
+    if (c[kErrors]) {
+        return fail(GT_ERR_INTERNAL, "device table probe limit reached (error bits " + std::to_string(c[kErrors]) +
+                                         "); results of this storage are incomplete");
+    }
+    return GT_OK;
+}
+
+// Prepares tile_read (and kmer_off when wanted) for one call.
+int prepare_reads(gt_storage* st, Reads& R, bool want_kmer_off, hipStream_t s) {
+    const int64_t n_tiles = (R.n_bases + kTile - 1) / kTile;
+    GT_TRY(st->tile_read.ensure((size_t)(n_tiles + 1) * sizeof(uint32_t)));
+    // tile_read entries are all written for well-formed offsets; zero first so
+    // malformed offsets stay in bounds.
+    launch_fill_u64(st->tile_read.as<uint64_t>(), (st->tile_read.bytes) / 8, 0, s);
+    launch_tile_read(R.off, R.n_reads, R.n_bases, st->tile_read.as<uint32_t>(), n_tiles, s);
+    R.tile_read = st->tile_read.as<uint32_t>();
+    R.kmer_off = nullptr;
+    if (want_kmer_off) {
+        GT_TRY(st->kmer_off.ensure((R.n_reads + 1) * sizeof(int64_t)));
+        GT_TRY(st->scan_blocks.ensure((scan_block_count(R.n_reads) + 1) * sizeof(int64_t)));
+        launch_kmer_offsets(R.off, R.n_reads, R.K, st->kmer_off.as<int64_t>(), st->scan_blocks.as<int64_t>(), s);
+        R.kmer_off = st->kmer_off.as<int64_t>();
+    }
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int check_read_args(uint16_t K, int mode, const void* seq, const int64_t* offsets, uint64_t n_reads,
+                    uint64_t n_bases) {
+    if (K < 1 || K >= kMaxK) return fail(GT_ERR_INVALID_ARG, "K must be in 1..255");
+    if (mode != kRaw && mode != kSkipN) return fail(GT_ERR_INVALID_ARG, "unknown input mode");
+    if (n_reads == 0) return GT_OK;
+    if (!offsets || (!seq && n_bases)) return fail(GT_ERR_INVALID_ARG, "null reads");
+    if (n_reads >= 0xFFFFFFFFULL) return fail(GT_ERR_INVALID_ARG, "too many reads in one call (max 2^32-2)");
+    if (n_bases >= (1ULL << 62)) return fail(GT_ERR_INVALID_ARG, "n_bases out of range");
+    return GT_OK;
+}
+
+Reads make_reads(const uint8_t* seq, const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint16_t K,
+                 int mode, int canonical) {
+    Reads R{};
+    R.seq = seq;
+    R.off = offsets;
+    R.n_reads = n_reads;
+    R.n_bases = (int64_t)n_bases;
+    R.b0 = 0;
+    R.b1 = (int64_t)n_bases;
+    R.K = K;
+    R.mode = mode;
+    R.canonical = canonical ? 1 : 0;
+    return R;
+}
+
+// Optional cap on the sub-batch size in start positions (environment variable
+// GOETIA_AMD_SUB_BATCH), used to exercise the multi-sub-batch path on small
+// inputs; 0 = no cap.
+uint64_t sub_batch_cap() {
+    const char* v = std::getenv("GOETIA_AMD_SUB_BATCH");
+    if (!v || !*v) return 0;
+    const unsigned long long x = std::strtoull(v, nullptr, 10);
+    return x ? std::max<uint64_t>((x + kTile - 1) / kTile * kTile, kTile) : 0;
+}
+
+// Largest sub-batch (in start positions, a multiple of kTile) whose claims fit
+// the claim map at load factor <= 1/2; grows the claim map and fresh buffer
+// as needed for this call.
+int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t s) {
+    const uint64_t kMaxClaimCap = 1ULL << 29;      // 8 GiB of claim slots
+    const uint64_t kMinClaimCap = 1ULL << 16;
+    const uint64_t per = std::max<uint32_t>(st->n_tables, 1);
+    const uint64_t cap = sub_batch_cap();
+    if (st->kind == kExact) {
+        uint64_t sb = std::min<uint64_t>(n_starts, 1ULL << 31);
+        sb = (sb + kTile - 1) / kTile * kTile;
+        if (cap) sb = std::min(sb, cap);
+        GT_TRY(st->fresh.ensure(sb));
+        *sub = sb;
+        return GT_OK;
+    }
+    uint64_t want = std::min(kMaxClaimCap, std::max(kMinClaimCap, next_pow2(2 * per * (n_starts + kTile))));
+    if (want > st->claim_cap) {
+        GT_TRY(st->claim_keys.ensure(want * sizeof(uint64_t)));
+        GT_TRY(st->claim_vals.ensure(want * sizeof(uint64_t)));
+        launch_fill_u64(st->claim_keys.as<uint64_t>(), want, 0, s);
+        launch_fill_u64(st->claim_vals.as<uint64_t>(), want, 0, s);
+        GT_HIP(hipGetLastError());
+        st->claim_cap = want;
+    }
+    uint64_t sb = st->claim_cap / (2 * per);
+    sb = std::min<uint64_t>(sb, 1ULL << 31) / kTile * kTile;
+    if (cap) sb = std::min(sb, cap);
+    if (sb == 0) return fail(GT_ERR_INTERNAL, "claim map too small");
+    GT_TRY(st->fresh.ensure(std::min<uint64_t>(sb, (n_starts + kTile - 1) / kTile * kTile)));
+    *sub = sb;
+    return GT_OK;
+}
+
+// Next batch generation; the 24-bit key field of the claim map never wraps
+// onto live entries because the map is cleared when it would.
+int next_gen(gt_storage* st, hipStream_t s) {
+    st->gen++;
+    if ((st->gen & 0xFFFFFFu) == 0) {
+        if (st->claim_cap) {
+            launch_fill_u64(st->claim_keys.as<uint64_t>(), st->claim_cap, 0, s);
+            launch_fill_u64(st->claim_vals.as<uint64_t>(), st->claim_cap, 0, s);
+        }
+        st->gen++;
+        GT_HIP(hipGetLastError());
+    }
+    if (st->gen == 0xFFFFFFFFu) return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
+    return GT_OK;
+}
+
+ClaimMap claim_view(const gt_storage* st) {
+    ClaimMap cm{};
+    cm.keys = st->claim_keys.as<uint64_t>();
+    cm.vals = st->claim_vals.as<uint64_t>();
+    cm.cap = st->claim_cap;
+    cm.log2cap = log2u(st->claim_cap);
+    cm.gen = st->gen;
+    return cm;
+}
+
+// Grows the exact set so that `more` new keys keep its load factor <= 1/2.
+int exact_reserve(gt_storage* st, uint64_t more, hipStream_t s) {
+    if (2 * (st->ecount_ub + more) <= st->ecap) return GT_OK;
+    uint64_t c[kNCounters];
+    GT_TRY(read_counters(st, c, s));
+    st->ecount_ub = c[kNUnique];
+    if (2 * (st->ecount_ub + more) <= st->ecap) return GT_OK;
+    return alloc_exact(st, next_pow2(2 * (st->ecount_ub + more)), s);
+}
+
+Launch make_launch(hipStream_t s) { return Launch{g_dev.grid, s}; }
+
+int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
+    if (R.n_reads == 0 || R.n_bases == 0) {
+        if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
+        return GT_OK;
+    }
+    GT_TRY(prepare_reads(st, R, new_flags != nullptr, s));
+    if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
+    uint64_t sub = 0;
+    GT_TRY(prepare_claims(st, (uint64_t)R.n_bases, &sub, s));
+    const Launch L = make_launch(s);
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    uint8_t* fresh = st->fresh.as<uint8_t>();
+    for (int64_t b0 = 0; b0 < R.n_bases; b0 += (int64_t)sub) {
+        Reads Rs = R;
+        Rs.b0 = b0;
+        Rs.b1 = std::min<int64_t>(R.n_bases, b0 + (int64_t)sub);
+        GT_TRY(next_gen(st, s));
+        if (st->kind == kExact) {
+            const uint64_t nstarts = (uint64_t)(Rs.b1 - Rs.b0);
+            if (2 * (st->ecount_ub + nstarts) > st->ecap) {
+                // Bound the growth by this sub-batch's fresh windows.
+                launch_fill_u64(ctr + kFresh, 1, 0, s);
+                launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+                GT_HIP(hipGetLastError());
+                uint64_t c[kNCounters];
+                GT_TRY(read_counters(st, c, s));
+                st->ecount_ub = c[kNUnique];
+                GT_TRY(exact_reserve(st, c[kFresh], s));
+                st->ecount_ub += c[kFresh];
+            } else {
+                launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+                st->ecount_ub += nstarts;
+            }
+            const ExactSet E = st->exact_view();
+            launch_exact_insert(L, Rs, E, fresh, st->gen, ctr);
+            launch_exact_resolve(L, Rs, E, fresh, st->gen, n_new, new_flags, ctr);
+        } else {
+            const Tables T = st->tables_view();
+            const ClaimMap cm = claim_view(st);
+            launch_claim(L, Rs, T, cm, fresh, ctr);
+            launch_apply(L, Rs, T, cm, fresh, n_new, new_flags, ctr);
+        }
+        GT_HIP(hipGetLastError());
+    }
+    return GT_OK;
+}
+
+int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* is_new, hipStream_t s) {
+    if (n == 0) return GT_OK;
+    const Launch L = make_launch(s);
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    uint64_t sub = 0;
+    GT_TRY(prepare_claims(st, n, &sub, s));
+    for (uint64_t i0 = 0; i0 < n; i0 += sub) {
+        const uint64_t m = std::min<uint64_t>(sub, n - i0);
+        GT_TRY(next_gen(st, s));
+        uint8_t* fresh = st->fresh.as<uint8_t>();
+        if (st->kind == kExact) {
+            GT_TRY(exact_reserve(st, m, s));
+            st->ecount_ub += m;
+            const ExactSet E = st->exact_view();
+            launch_hash_exact_lookup(L, h + i0, m, E, fresh, ctr);
+            launch_hash_exact_insert(L, h + i0, m, E, fresh, st->gen, ctr);
+            launch_hash_exact_resolve(L, h + i0, m, E, fresh, st->gen, is_new ? is_new + i0 : nullptr, ctr);
+        } else {
+            const Tables T = st->tables_view();
+            const ClaimMap cm = claim_view(st);
+            launch_hash_claim(L, h + i0, m, T, cm, fresh, ctr);
+            launch_hash_apply(L, h + i0, m, T, cm, fresh, is_new ? is_new + i0 : nullptr, ctr);
+        }
+        GT_HIP(hipGetLastError());
+    }
+    return GT_OK;
+}
+
+// Host staging: copies host arrays into the storage's staging buffers.
+template <class T>
+int stage_in(DevBuf& buf, const T* host, size_t n, hipStream_t s) {
+    GT_TRY(buf.ensure(std::max<size_t>(n * sizeof(T), 16)));
+    if (n) GT_HIP(hipMemcpyAsync(buf.p, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return GT_OK;
+}
+
+template <class T>
+int stage_out(T* host, const DevBuf& buf, size_t n, hipStream_t s) {
+    if (host && n) GT_HIP(hipMemcpyAsync(host, buf.p, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return GT_OK;
+}
+
+uint64_t packed_kmers(const int64_t* offsets, uint64_t n_reads, int K) {
+    uint64_t n = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        const int64_t len = offsets[r + 1] - offsets[r];
+        if (len >= K) n += (uint64_t)(len - K + 1);
+    }
+    return n;
+}
+
+int check_host_offsets(const int64_t* offsets, uint64_t n_reads) {
+    if (n_reads == 0) return GT_OK;
+    if (!offsets) return fail(GT_ERR_INVALID_ARG, "null offsets");
+    if (offsets[0] != 0) return fail(GT_ERR_INVALID_ARG, "offsets[0] must be 0");
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        if (offsets[r + 1] < offsets[r]) return fail(GT_ERR_INVALID_ARG, "offsets must be non-decreasing");
+    }
+    return GT_OK;
+}
+
+}  // namespace
+
+// ============================================================ C ABI
+
+extern "C" {
+
+const char* gt_last_error(void) { return g_err.c_str(); }
+
+int gt_abi_version(void) { return GT_ABI_VERSION; }
+
+int gt_device_init(int device) { return ensure_device(device); }
+
+int gt_device_synchronize(void) {
+    GT_TRY(ensure_device());
+    GT_HIP(hipDeviceSynchronize());
+    return GT_OK;
+}
+
+// storage.hh:146-190 (is_prime by trial division, odd candidates below x).
+int gt_primes_near_x(uint32_t n, uint64_t x, uint64_t* out, uint32_t* n_out) {
+    if (!out || !n_out) return fail(GT_ERR_INVALID_ARG, "null output");
+    auto is_prime = [](uint64_t v) {
+        if (v < 2) return false;
+        if (v == 2) return true;
+        if (v % 2 == 0) return false;
+        for (unsigned long long i = 3; i < std::sqrt((double)v) + 1; i += 2) {
+            if (v % i == 0) return false;
+        }
+        return true;
+    };
+    uint32_t found = 0;
+    if (x == 0) {
+        *n_out = 0;
+        return GT_OK;
+    }
+    if (x == 1) {
+        if (n) out[found++] = 1;
+        *n_out = found;
+        return GT_OK;
+    }
+    uint64_t i = x - 1;
+    if (i % 2 == 0) i--;
+    while (found != n) {
+        if (is_prime(i)) out[found++] = i;
+        if (i == 1) break;
+        i -= 2;
+    }
+    *n_out = found;
+    return GT_OK;
+}
+
+int gt_storage_new(int kind, const uint64_t* tablesizes, uint32_t n_tables, uint64_t capacity_hint,
+                   gt_storage** out) {
+    if (!out) return fail(GT_ERR_INVALID_ARG, "null output handle");
+    *out = nullptr;
+    if (kind < kBit || kind > kExact) return fail(GT_ERR_INVALID_ARG, "unknown storage kind");
+    if (kind != kExact) {
+        if (n_tables == 0 || n_tables > (uint32_t)kMaxTables)
+            return fail(GT_ERR_INVALID_ARG, "n_tables must be in 1..16");
+        if (!tablesizes) return fail(GT_ERR_INVALID_ARG, "null tablesizes");
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            if (tablesizes[i] == 0 || tablesizes[i] >= (1ULL << 36))
+                return fail(GT_ERR_INVALID_ARG, "table sizes must be in 1..2^36-1");
+        }
+    }
+    GT_TRY(ensure_device());
+    gt_storage* st = new (std::nothrow) gt_storage();
+    if (!st) return fail(GT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    st->kind = kind;
+    hipStream_t s = nullptr;
+    int rc = st->counters.ensure(kNCounters * sizeof(uint64_t));
+    if (rc == GT_OK) launch_fill_u64(st->counters.as<uint64_t>(), kNCounters, 0, s);
+    if (rc == GT_OK && kind != kExact) {
+        st->n_tables = n_tables;
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            st->sizes.push_back(tablesizes[i]);
+            const uint64_t nb = kind == kBit ? tablesizes[i] / 8 + 1 : tablesizes[i];
+            st->nbytes.push_back(nb);
+            st->table_off.push_back(total);
+            total += (nb + 255) / 256 * 256;
+        }
+        rc = st->tables.ensure(total);
+        if (rc == GT_OK) rc = zero_tables(st, s);
+    } else if (rc == GT_OK) {
+        rc = st->eones.ensure(256);
+        if (rc == GT_OK) launch_fill_u64(st->eones.as<uint64_t>(), 1, 0, s);
+        if (rc == GT_OK) rc = alloc_exact(st, next_pow2(std::max<uint64_t>(1024, 2 * capacity_hint)), s);
+    }
+    if (rc == GT_OK) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(GT_ERR_DEVICE, std::string("storage init: ") + hipGetErrorString(e));
+    }
+    if (rc != GT_OK) {
+        delete st;
+        return rc;
+    }
+    *out = st;
+    return GT_OK;
+}
+
+int gt_storage_free(gt_storage* st) {
+    if (!st) return GT_OK;
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    (void)hipDeviceSynchronize();
+    delete st;
+    return GT_OK;
+}
+
+int gt_storage_clone(const gt_storage* st, gt_storage** out) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    return gt_storage_new(st->kind, st->sizes.data(), st->n_tables, st->kind == kExact ? st->ecap / 2 : 0, out);
+}
+
+int gt_storage_reset(gt_storage* st) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(zero_tables(st, nullptr));
+    if (st->kind == kExact) {
+        launch_fill_u64(st->counters.as<uint64_t>() + kNUnique, 1, 0, nullptr);
+        st->ecount_ub = 0;
+    }
+    GT_HIP(hipStreamSynchronize(nullptr));
+    return GT_OK;
+}
+
+int gt_storage_kind(const gt_storage* st, int* kind) {
+    if (!valid_storage(st) || !kind) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    *kind = st->kind;
+    return GT_OK;
+}
+
+int gt_storage_n_tables(const gt_storage* st, uint32_t* n) {
+    if (!valid_storage(st) || !n) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    *n = st->n_tables;
+    return GT_OK;
+}
+
+int gt_storage_tablesizes(const gt_storage* st, uint64_t* out) {
+    if (!valid_storage(st) || (!out && st->n_tables)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    for (uint32_t i = 0; i < st->n_tables; ++i) out[i] = st->sizes[i];
+    return GT_OK;
+}
+
+int gt_storage_table_bytes(const gt_storage* st, uint32_t i, uint64_t* nbytes) {
+    if (!valid_storage(st) || !nbytes) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (i >= st->n_tables) return fail(GT_ERR_INVALID_ARG, "table index out of range");
+    *nbytes = st->nbytes[i];
+    return GT_OK;
+}
+
+int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    uint64_t c[kNCounters];
+    GT_HIP(hipDeviceSynchronize());
+    GT_TRY(read_counters(st, c, nullptr));
+    if (c[kErrors]) return fail(GT_ERR_INTERNAL, "device table probe limit was reached; counters are not exact");
+    if (n_unique) *n_unique = c[kNUnique];
+    if (n_occupied) *n_occupied = st->kind == kExact ? st->ecap : c[kNOccupied];
+    return GT_OK;
+}
+
+int gt_storage_set_counters(gt_storage* st, uint64_t n_unique, uint64_t n_occupied) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    uint64_t c[2] = {n_unique, n_occupied};
+    GT_HIP(hipMemcpy(st->counters.p, c, sizeof(c), hipMemcpyHostToDevice));
+    return GT_OK;
+}
+
+int gt_storage_read_table(gt_storage* st, uint32_t i, uint8_t* host_dst) {
+    if (!valid_storage(st) || !host_dst) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    if (st->kind == kExact) return fail(GT_ERR_UNSUPPORTED, "the exact set has no raw tables (get_raw_tables() is null)");
+    if (i >= st->n_tables) return fail(GT_ERR_INVALID_ARG, "table index out of range");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_HIP(hipDeviceSynchronize());
+    GT_HIP(hipMemcpy(host_dst, st->tables.as<uint8_t>() + st->table_off[i], st->nbytes[i], hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+int gt_storage_write_table(gt_storage* st, uint32_t i, const uint8_t* host_src) {
+    if (!valid_storage(st) || !host_src) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    if (st->kind == kExact) return fail(GT_ERR_UNSUPPORTED, "the exact set has no raw tables");
+    if (i >= st->n_tables) return fail(GT_ERR_INVALID_ARG, "table index out of range");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_HIP(hipDeviceSynchronize());
+    GT_HIP(hipMemcpy(st->tables.as<uint8_t>() + st->table_off[i], host_src, st->nbytes[i], hipMemcpyHostToDevice));
+    return GT_OK;
+}
+
+int gt_storage_update_from(gt_storage* st, const gt_storage* other) {
+    if (!valid_storage(st) || !valid_storage(other)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (st->kind != kBit || other->kind != kBit) return fail(GT_ERR_UNSUPPORTED, "update_from is defined for BitStorage");
+    if (st->sizes != other->sizes) return fail(GT_ERR_INVALID_ARG, "both nodegraphs must have same table sizes");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_HIP(hipDeviceSynchronize());
+    for (uint32_t i = 0; i < st->n_tables; ++i) {
+        const uint64_t words = (st->nbytes[i] + 3) / 4;
+        launch_bit_union(reinterpret_cast<uint32_t*>(st->tables.as<uint8_t>() + st->table_off[i]),
+                         reinterpret_cast<const uint32_t*>(other->tables.as<uint8_t>() + other->table_off[i]), words,
+                         i == 0, st->counters.as<uint64_t>(), nullptr);
+    }
+    GT_HIP(hipGetLastError());
+    GT_HIP(hipStreamSynchronize(nullptr));
+    return GT_OK;
+}
+
+int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint64_t* n) {
+    if (!valid_storage(st) || !n) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    if (st->kind != kExact) return fail(GT_ERR_UNSUPPORTED, "exact_keys is defined for the exact set");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_HIP(hipDeviceSynchronize());
+    DevBuf out, cnt;
+    GT_TRY(out.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
+    GT_TRY(cnt.ensure(sizeof(unsigned long long)));
+    launch_fill_u64(cnt.as<uint64_t>(), 1, 0, nullptr);
+    launch_exact_keys(st->exact_view(), out.as<uint64_t>(), cap, cnt.as<unsigned long long>(), nullptr);
+    GT_HIP(hipGetLastError());
+    uint64_t total = 0;
+    uint32_t ones = 0;
+    GT_HIP(hipMemcpy(&total, cnt.p, sizeof(total), hipMemcpyDeviceToHost));
+    GT_HIP(hipMemcpy(&ones, st->eones.p, sizeof(ones), hipMemcpyDeviceToHost));
+    const uint64_t take = std::min(total, cap);
+    if (take) GT_HIP(hipMemcpy(host_out, out.p, take * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (ones) {
+        if (total < cap) host_out[total] = ~0ULL;
+        total++;
+    }
+    *n = total;
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ hashing
+
+int gt_kmer_offsets_dev(uint16_t K, const int64_t* offsets, uint64_t n_reads, int64_t* kmer_offsets, void* stream) {
+    if (K < 1 || K >= kMaxK) return fail(GT_ERR_INVALID_ARG, "K must be in 1..255");
+    if (!kmer_offsets || (n_reads && !offsets)) return fail(GT_ERR_INVALID_ARG, "null pointer");
+    GT_TRY(ensure_device());
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf blocks;
+    GT_TRY(blocks.ensure((scan_block_count(n_reads) + 1) * sizeof(int64_t)));
+    launch_kmer_offsets(offsets, n_reads, K, kmer_offsets, blocks.as<int64_t>(), s);
+    GT_HIP(hipGetLastError());
+    GT_HIP(hipStreamSynchronize(s));   // `blocks` is released on return
+    return GT_OK;
+}
+
+int gt_hash_reads_dev(uint16_t K, int mode, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                      uint64_t n_bases, uint64_t* fw, uint64_t* rc, uint8_t* valid, void* stream) {
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return GT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    // A private scratch holder: hashing needs no storage.
+    static std::mutex mu;
+    static gt_storage* scratch = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!scratch) scratch = new gt_storage();
+    scratch->kind = kBit;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, 1);
+    GT_TRY(prepare_reads(scratch, R, true, s));
+    launch_hash(make_launch(s), R, fw, rc, valid);
+    GT_HIP(hipGetLastError());
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+int gt_hash_reads(uint16_t K, int mode, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads, uint64_t* fw,
+                  uint64_t* rc, uint8_t* valid) {
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    if (nk == 0) return GT_OK;
+    DevBuf dseq, doff, dfw, drc, dval;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    GT_TRY(dfw.ensure(nk * 8));
+    GT_TRY(drc.ensure(nk * 8));
+    GT_TRY(dval.ensure(nk));
+    GT_TRY(gt_hash_reads_dev(K, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads, n_bases, dfw.as<uint64_t>(),
+                             drc.as<uint64_t>(), dval.as<uint8_t>(), s));
+    GT_TRY(stage_out(fw, dfw, nk, s));
+    GT_TRY(stage_out(rc, drc, nk, s));
+    GT_TRY(stage_out(valid, dval, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ shifter
+
+int gt_shifter_hash_base(uint16_t K, const uint8_t* kmer, uint64_t* fw, uint64_t* rc) {
+    if (K < 1 || K >= kMaxK || !kmer || !fw || !rc) return fail(GT_ERR_INVALID_ARG, "invalid shifter arguments");
+    uint64_t f = 0, r = 0;
+    for (int i = 0; i < K; ++i) {
+        f = gt_rotl(f, 1) ^ GT_CHAR_HASH[kmer[i]];
+        r = gt_rotl(r, 1) ^ GT_CHAR_HASH[gt_complement(kmer[K - 1 - i])];
+    }
+    *fw = f;
+    *rc = r;
+    return GT_OK;
+}
+
+int gt_shifter_shift_right(uint16_t K, uint64_t* fw, uint64_t* rc, uint8_t out, uint8_t in) {
+    if (K < 1 || K >= kMaxK || !fw || !rc) return fail(GT_ERR_INVALID_ARG, "invalid shifter arguments");
+    const unsigned Kr = K & 63u;
+    *fw = gt_rotl(*fw, 1) ^ gt_rotl(GT_CHAR_HASH[out], Kr) ^ GT_CHAR_HASH[in];
+    *rc = gt_rotr(*rc ^ gt_rotl(GT_CHAR_HASH[gt_complement(in)], Kr) ^ GT_CHAR_HASH[gt_complement(out)], 1);
+    return GT_OK;
+}
+
+int gt_shifter_shift_left(uint16_t K, uint64_t* fw, uint64_t* rc, uint8_t in, uint8_t out) {
+    if (K < 1 || K >= kMaxK || !fw || !rc) return fail(GT_ERR_INVALID_ARG, "invalid shifter arguments");
+    const unsigned Kr = K & 63u;
+    *fw = gt_rotr(*fw ^ gt_rotl(GT_CHAR_HASH[in], Kr) ^ GT_CHAR_HASH[out], 1);
+    *rc = gt_rotl(*rc, 1) ^ gt_rotl(GT_CHAR_HASH[gt_complement(out)], Kr) ^ GT_CHAR_HASH[gt_complement(in)];
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ dBG batches
+
+int gt_dbg_insert_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                      uint64_t n_reads, uint64_t n_bases, uint64_t* n_new, uint8_t* new_flags, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    return dbg_insert_impl(st, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new, new_flags,
+                           (hipStream_t)stream);
+}
+
+int gt_dbg_insert(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                  uint64_t n_reads, uint64_t* n_new, uint8_t* new_flags, uint64_t* n_kmers) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = nullptr;
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
+    GT_TRY(stage_in(st->h_off, offsets, n_reads + 1, s));
+    GT_TRY(st->h_out1.ensure(std::max<uint64_t>(n_reads, 1) * 8));
+    GT_TRY(st->h_out2.ensure(std::max<uint64_t>(nk, 1)));
+    GT_TRY(dbg_insert_impl(st, make_reads(st->h_seq.as<uint8_t>(), st->h_off.as<int64_t>(), n_reads, n_bases, K, mode,
+                                          canonical),
+                           n_new ? st->h_out1.as<uint64_t>() : nullptr, new_flags ? st->h_out2.as<uint8_t>() : nullptr,
+                           s));
+    GT_TRY(stage_out(n_new, st->h_out1, n_reads, s));
+    GT_TRY(stage_out(new_flags, st->h_out2, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    GT_TRY(check_kernel_errors(st, s));
+    if (n_kmers) {
+        if (mode == kRaw) {
+            *n_kmers = nk;
+        } else {
+            uint64_t c = 0;
+            for (uint64_t r = 0; r < n_reads; ++r) {
+                int run = 0;
+                for (int64_t j = offsets[r]; j < offsets[r + 1]; ++j) {
+                    run = gt_is_acgt(seq[j]) ? run + 1 : 0;
+                    if (run >= K) c++;
+                }
+            }
+            *n_kmers = c;
+        }
+    }
+    return GT_OK;
+}
+
+int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                     uint64_t n_reads, uint64_t n_bases, int16_t* counts, uint64_t* hashes, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    GT_TRY(prepare_reads(st, R, true, s));
+    launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int gt_dbg_query(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                 uint64_t n_reads, int16_t* counts, uint64_t* hashes) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    if (nk == 0) return GT_OK;
+    hipStream_t s = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
+    GT_TRY(stage_in(st->h_off, offsets, n_reads + 1, s));
+    GT_TRY(st->h_out1.ensure(nk * sizeof(int16_t)));
+    GT_TRY(st->h_out3.ensure(nk * sizeof(uint64_t)));
+    GT_TRY(gt_dbg_query_dev(st, K, canonical, mode, st->h_seq.as<uint8_t>(), st->h_off.as<int64_t>(), n_reads, n_bases,
+                            counts ? st->h_out1.as<int16_t>() : nullptr, hashes ? st->h_out3.as<uint64_t>() : nullptr,
+                            s));
+    GT_TRY(stage_out(counts, st->h_out1, nk, s));
+    GT_TRY(stage_out(hashes, st->h_out3, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                                const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, int16_t* counts,
+                                void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    if (st->kind == kByte) {
+        // Count-min results depend on the exact order of every earlier insert
+        // of the same bins: hash the batch, then apply it in k-mer order.
+        GT_TRY(prepare_reads(st, R, true, s));
+        int64_t nk = 0;
+        GT_HIP(hipMemcpyAsync(&nk, st->kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        if (nk <= 0) return GT_OK;
+        DevBuf fw, rc, valid;
+        GT_TRY(fw.ensure((size_t)nk * 8));
+        GT_TRY(rc.ensure((size_t)nk * 8));
+        GT_TRY(valid.ensure((size_t)nk));
+        const Launch L = make_launch(s);
+        launch_hash(L, R, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>());
+        launch_seq_insert_query(fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), (uint64_t)nk, R.canonical,
+                                st->tables_view(), counts, nullptr, st->counters.as<uint64_t>(), s);
+        GT_HIP(hipGetLastError());
+        GT_HIP(hipStreamSynchronize(s));   // scratch is released on return
+        return GT_OK;
+    }
+    GT_TRY(dbg_insert_impl(st, R, nullptr, nullptr, s));
+    // Presence storages return 1 after every insert (bitstorage.cc:78-84,
+    // sparseppstorage.cc:21-25); a query after the batch gives exactly that
+    // (and -1 for skipped windows).
+    GT_TRY(prepare_reads(st, R, true, s));
+    launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, nullptr);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int gt_dbg_insert_and_query(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                            const int64_t* offsets, uint64_t n_reads, int16_t* counts) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    DevBuf dseq, doff, dout;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    GT_TRY(dout.ensure(std::max<uint64_t>(nk, 1) * sizeof(int16_t)));
+    GT_TRY(gt_dbg_insert_and_query_dev(st, K, canonical, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads,
+                                       n_bases, dout.as<int16_t>(), s));
+    GT_TRY(stage_out(counts, dout, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return check_kernel_errors(st, s);
+}
+
+int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                       const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, const uint8_t* only,
+                       uint8_t* in_deg, uint8_t* out_deg, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (!in_deg || !out_deg) return fail(GT_ERR_INVALID_ARG, "null degree outputs");
+    if (min_count < 1 || min_count > 255) return fail(GT_ERR_INVALID_ARG, "min_count must be in 1..255");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    GT_TRY(prepare_reads(st, R, true, s));
+    launch_degrees(make_launch(s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg, out_deg);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int gt_dbg_degrees(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                   const int64_t* offsets, uint64_t n_reads, const uint8_t* only, uint8_t* in_deg,
+                   uint8_t* out_deg) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    if (nk == 0) return GT_OK;
+    DevBuf dseq, doff, donly, din, dout;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    if (only) GT_TRY(stage_in(donly, only, nk, s));
+    GT_TRY(din.ensure(nk));
+    GT_TRY(dout.ensure(nk));
+    GT_TRY(gt_dbg_degrees_dev(st, K, canonical, mode, min_count, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads,
+                              n_bases, only ? donly.as<uint8_t>() : nullptr, din.as<uint8_t>(), dout.as<uint8_t>(), s));
+    GT_TRY(stage_out(in_deg, din, nk, s));
+    GT_TRY(stage_out(out_deg, dout, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ per-hash
+
+int gt_insert_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, uint8_t* is_new, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (n && !hashes) return fail(GT_ERR_INVALID_ARG, "null hashes");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    return hashes_insert_impl(st, hashes, n, is_new, (hipStream_t)stream);
+}
+
+int gt_insert_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, uint8_t* is_new) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (n && !hashes) return fail(GT_ERR_INVALID_ARG, "null hashes");
+    GT_TRY(ensure_device());
+    if (n == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(st->h_out3, hashes, n, s));
+    GT_TRY(st->h_out2.ensure(n));
+    GT_TRY(hashes_insert_impl(st, st->h_out3.as<uint64_t>(), n, is_new ? st->h_out2.as<uint8_t>() : nullptr, s));
+    GT_TRY(stage_out(is_new, st->h_out2, n, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return check_kernel_errors(st, s);
+}
+
+int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
+    GT_TRY(ensure_device());
+    if (n == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    launch_hash_query(make_launch(s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int gt_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
+    GT_TRY(ensure_device());
+    if (n == 0) return GT_OK;
+    DevBuf dh, dc;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dh, hashes, n, s));
+    GT_TRY(dc.ensure(n * sizeof(int16_t)));
+    GT_TRY(gt_query_hashes_dev(st, dh.as<uint64_t>(), n, dc.as<int16_t>(), s));
+    GT_TRY(stage_out(counts, dc, n, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+// Storage::insert_and_query per hash in index order.  Count-min results
+// depend on every earlier insert of the same bins, so they are applied by one
+// device lane in index order, which is exact.
+int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
+    GT_TRY(ensure_device());
+    if (n == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(st->h_out3, hashes, n, s));
+    GT_TRY(st->h_out2.ensure(n));
+    GT_TRY(st->h_out1.ensure(n * sizeof(int16_t)));
+    if (st->kind == kByte) {
+        launch_seq_insert_query(st->h_out3.as<uint64_t>(), nullptr, nullptr, n, 0, st->tables_view(),
+                                st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), s);
+        GT_HIP(hipGetLastError());
+        GT_TRY(stage_out(counts, st->h_out1, n, s));
+        GT_HIP(hipStreamSynchronize(s));
+    } else {
+        GT_TRY(hashes_insert_impl(st, st->h_out3.as<uint64_t>(), n, nullptr, s));
+        GT_HIP(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < n; ++i) counts[i] = 1;
+    }
+    return check_kernel_errors(st, s);
+}
+
+// ------------------------------------------------------------ workload
+
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+struct Xoshiro256ss {
+    uint64_t s[4];
+    explicit Xoshiro256ss(uint64_t seed) {
+        uint64_t x = seed;
+        for (auto& v : s) v = splitmix64(x);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t result = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return result;
+    }
+    // Uniform in [0, n) by multiply-shift (n >= 1).
+    uint64_t below(uint64_t n) { return (uint64_t)(((unsigned __int128)next() * n) >> 64); }
+};
+
+uint64_t rate_threshold(double p) {
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return ~0ULL;
+    return (uint64_t)std::ldexp(p, 64);
+}
+
+const char kBases[4] = {'A', 'C', 'G', 'T'};
+
+int base_index(uint8_t c) { return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : 3; }
+
+uint64_t read_seed(uint64_t seed, uint64_t r) { return seed * 0x9E3779B97F4A7C15ULL + (r + 1) * 0xD1B54A32D192ED03ULL; }
+
+}  // namespace
+
+int gt_gen_genome(uint64_t seed, uint64_t length, uint8_t* out) {
+    if (length && !out) return fail(GT_ERR_INVALID_ARG, "null output");
+    Xoshiro256ss rng(seed);
+    for (uint64_t i = 0; i < length; ++i) out[i] = (uint8_t)kBases[rng.next() >> 62];
+    return GT_OK;
+}
+
+int gt_gen_reads(uint64_t seed, const uint8_t* genome, uint64_t genome_len, uint64_t n_reads, const uint32_t* lengths,
+                 uint32_t len_min, uint32_t len_max, double sub_rate, double n_rate, uint8_t* seq, int64_t* offsets,
+                 uint64_t* n_bases) {
+    if (!genome || genome_len == 0 || !offsets || (n_reads && !seq))
+        return fail(GT_ERR_INVALID_ARG, "null generator arguments");
+    if (!lengths && len_min > len_max) return fail(GT_ERR_INVALID_ARG, "len_min > len_max");
+    // Pass 1: lengths (each read's generator is private, so this is cheap).
+    offsets[0] = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        uint64_t len;
+        Xoshiro256ss rng(read_seed(seed, r));
+        if (lengths) {
+            len = lengths[r];
+        } else {
+            len = len_min + rng.below((uint64_t)len_max - len_min + 1);
+        }
+        if (len > genome_len) len = genome_len;
+        offsets[r + 1] = offsets[r] + (int64_t)len;
+    }
+    const uint64_t sub_thr = rate_threshold(sub_rate), n_thr = rate_threshold(n_rate);
+    auto fill = [&](uint64_t r0, uint64_t r1) {
+        for (uint64_t r = r0; r < r1; ++r) {
+            Xoshiro256ss rng(read_seed(seed, r));
+            if (!lengths) (void)rng.next();   // the length draw of pass 1
+            const uint64_t len = (uint64_t)(offsets[r + 1] - offsets[r]);
+            const uint64_t start = rng.below(genome_len - len + 1);
+            const bool rev = rng.next() >> 63;
+            uint8_t* out = seq + offsets[r];
+            for (uint64_t j = 0; j < len; ++j) {
+                uint8_t b = rev ? (uint8_t)gt_complement(genome[start + len - 1 - j]) : genome[start + j];
+                if (rng.next() < sub_thr) b = (uint8_t)kBases[(base_index(b) + 1 + rng.below(3)) & 3];
+                if (n_thr && rng.next() < n_thr) b = 'N';
+                out[j] = b;
+            }
+        }
+    };
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n_reads < 4096) nt = 1;
+    std::vector<std::thread> th;
+    const uint64_t per = (n_reads + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t a = t * per, b = std::min(n_reads, a + per);
+        if (a < b) th.emplace_back(fill, a, b);
+    }
+    for (auto& t : th) t.join();
+    if (n_bases) *n_bases = (uint64_t)offsets[n_reads];
+    return GT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// gt_internal.h -- structures shared by the HIP kernels and the C-ABI layer
+// of goetia_amd.  Not part of the public interface (include/goetia_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gt {
+
+constexpr int kMaxTables = 16;
+
+// Work decomposition of the k-mer walkers: a tile is kTile consecutive k-mer
+// start positions of the concatenated read bytes; a workgroup of kBlock
+// threads owns one tile at a time and each thread rolls the hash over kPer
+// consecutive starts.
+constexpr int kBlock = 256;
+constexpr int kPer = 16;
+constexpr int kTile = kBlock * kPer;
+constexpr int kMaxK = 256;          // K <= 255 (goetia's K is a uint16_t)
+constexpr int kReadCache = 1024;    // read offsets cached in LDS per tile
+
+enum Kind : int { kBit = 1, kByte = 2, kExact = 3 };
+enum Mode : int { kRaw = 0, kSkipN = 1 };
+
+// Device counters of one storage (u64 each).
+enum Counter : int { kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kNCounters = 4 };
+
+// Error bits OR-ed into counters[kErrors] by kernels.
+enum ErrBit : uint64_t { kErrProbe = 1, kErrClaim = 2 };
+
+// Bloom / count-min tables: table i holds bins [0, p[i]) at base[i].
+struct Tables {
+    uint32_t n;
+    int kind;
+    uint64_t p[kMaxTables];
+    uint64_t m[kMaxTables];      // gt_mod_magic(p[i])
+    uint8_t* base[kMaxTables];   // 4-byte aligned, padded to a multiple of 4 bytes
+};
+
+// Exact k-mer set: open addressing with linear probing over cap = 2^log2cap
+// slots.  keys[cap] is the slot of the all-ones key (which is also the empty
+// marker); flag[0] says whether that key is present.  stamp[] carries the
+// first-touch stamp of keys created in the current batch.
+struct ExactSet {
+    uint64_t* keys;    // cap + 1 entries, empty = ~0
+    uint64_t* stamp;   // cap + 1 entries
+    uint32_t* ones;    // presence flag of the all-ones key
+    uint64_t cap;
+    uint32_t log2cap;
+};
+
+// First-touch claim map for Bloom / count-min batches: key word =
+// gen:24 | table:4 | bin:36, value word = gen:32 | ~ordinal:32 (atomicMax).
+struct ClaimMap {
+    uint64_t* keys;
+    uint64_t* vals;
+    uint64_t cap;
+    uint32_t log2cap;
+    uint32_t gen;
+};
+
+// One call's view of the reads.  Starts s in [b0, b1) are processed; the
+// ordinal of the k-mer at s is s - b0.
+struct Reads {
+    const uint8_t* seq;
+    const int64_t* off;        // n_reads + 1 entries, off[0] = 0
+    const int64_t* kmer_off;   // n_reads + 1 entries (may be null if unused)
+    const uint32_t* tile_read; // read holding the first base of each tile
+    uint64_t n_reads;
+    int64_t n_bases;
+    int64_t b0, b1;
+    int K;
+    int mode;
+    int canonical;
+};
+
+}  // namespace gt

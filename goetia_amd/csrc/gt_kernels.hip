@@ -1,0 +1,1024 @@
+// gt_kernels.hip -- gfx950 kernels of goetia_amd's streaming dBG hot path.
+//
+// All arithmetic is integer.  The k-mer walkers stage a tile of read bytes in
+// LDS with 16-byte loads, roll goetia's canonical cyclic hash
+// (rollinghashshifter.hh:180-219, cyclichash.h:85-108) over kPer consecutive
+// windows per thread, and hand every window to an operation functor:
+//
+//   Bloom / count-min insert  = claim pass + apply pass (first-touch exact)
+//   exact-set insert          = lookup pass + insert pass + resolve pass
+//   query, degrees            = one read-only pass
+//
+// "First touch": goetia inserts k-mers one at a time in read order, so a
+// k-mer is new iff, for some table, its bin was empty before it
+// (bitstorage.hh:195-219, bytestorage.cc:60-113).  A batch reproduces that
+// exactly: a bin that was empty at batch start is claimed by the batch event
+// with the smallest ordinal (start position), found with a 64-bit atomicMax on
+// gen:32 | ~ordinal:32 stamps, and that event is the one that counts as new.
+#include "gt_internal.h"
+#include "gt_hash.h"
+
+namespace gt {
+
+__constant__ uint64_t c_char_hash[256] = GT_CHAR_HASH_INIT;
+
+// Table loops run over the compile-time maximum with a uniform guard so that
+// every index into the by-value Tables argument is a constant (the argument
+// then stays in scalar registers instead of being copied to scratch).
+#define GT_FOR_TABLES(i, T) _Pragma("unroll") for (uint32_t i = 0; i < (uint32_t)kMaxTables; ++i) if (i < (T).n)
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+constexpr uint64_t kEmpty = ~0ULL;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+__device__ __forceinline__ uint64_t load_relaxed(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds a per-thread count to a device counter with one atomic per wave.
+__device__ __forceinline__ void add_counter(uint64_t* ctr, uint64_t v) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)ctr, (unsigned long long)v);
+}
+
+// ------------------------------------------------------------ claim map
+
+__device__ __forceinline__ uint64_t claim_key(uint32_t gen, uint32_t table, uint64_t bin) {
+    return ((uint64_t)(gen & 0xFFFFFFu) << 40) | ((uint64_t)table << 36) | bin;
+}
+
+__device__ __forceinline__ uint64_t claim_stamp(uint32_t gen, uint32_t ordinal) {
+    return ((uint64_t)gen << 32) | (uint64_t)(0xFFFFFFFFu - ordinal);
+}
+
+__device__ __forceinline__ uint64_t claim_slot(const ClaimMap& cm, uint64_t key) {
+    return ((key & ((1ULL << 40) - 1)) * kGolden) >> (64 - cm.log2cap);
+}
+
+__device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, uint64_t* errors) {
+    const uint64_t mask = cm.cap - 1;
+    const uint64_t gen = key >> 40;
+    uint64_t slot = claim_slot(cm, key);
+    for (uint64_t probes = 0; probes <= cm.cap; ++probes) {
+        uint64_t cur = load_relaxed(&cm.keys[slot]);
+        if (cur == key) {
+            atomicMax((unsigned long long*)&cm.vals[slot], (unsigned long long)stamp);
+            return;
+        }
+        if ((cur >> 40) != gen) {
+            // a slot left by an earlier generation is free
+            uint64_t prev = atomicCAS((unsigned long long*)&cm.keys[slot],
+                                      (unsigned long long)cur, (unsigned long long)key);
+            if (prev == cur || prev == key) {
+                atomicMax((unsigned long long*)&cm.vals[slot], (unsigned long long)stamp);
+                return;
+            }
+            if ((prev >> 40) != gen) continue;  // changed but still stale: look again
+        }
+        slot = (slot + 1) & mask;
+    }
+    atomicOr((unsigned long long*)errors, (unsigned long long)kErrClaim);
+}
+
+__device__ uint64_t claim_find(const ClaimMap& cm, uint64_t key) {
+    const uint64_t mask = cm.cap - 1;
+    const uint64_t gen = key >> 40;
+    uint64_t slot = claim_slot(cm, key);
+    for (uint64_t probes = 0; probes <= cm.cap; ++probes) {
+        uint64_t cur = cm.keys[slot];
+        if (cur == key) return cm.vals[slot];
+        if ((cur >> 40) != gen) return 0;
+        slot = (slot + 1) & mask;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ exact set
+
+__device__ __forceinline__ uint64_t exact_home(const ExactSet& es, uint64_t key) {
+    return mix64(key) >> (64 - es.log2cap);
+}
+
+// Slot of key or ~0 when absent (read-only tables).
+__device__ uint64_t exact_find(const ExactSet& es, uint64_t key) {
+    if (key == kEmpty) return (*es.ones) ? es.cap : kEmpty;
+    const uint64_t mask = es.cap - 1;
+    uint64_t slot = exact_home(es, key);
+    for (uint64_t probes = 0; probes < es.cap; ++probes) {
+        uint64_t cur = es.keys[slot];
+        if (cur == key) return slot;
+        if (cur == kEmpty) return kEmpty;
+        slot = (slot + 1) & mask;
+    }
+    return kEmpty;
+}
+
+// Inserts key (idempotent) and returns its slot.
+__device__ uint64_t exact_insert(const ExactSet& es, uint64_t key, uint64_t* errors) {
+    if (key == kEmpty) {
+        atomicOr(es.ones, 1u);
+        return es.cap;
+    }
+    const uint64_t mask = es.cap - 1;
+    uint64_t slot = exact_home(es, key);
+    for (uint64_t probes = 0; probes < es.cap; ++probes) {
+        uint64_t cur = load_relaxed(&es.keys[slot]);
+        if (cur == key) return slot;
+        if (cur == kEmpty) {
+            uint64_t prev = atomicCAS((unsigned long long*)&es.keys[slot],
+                                      (unsigned long long)kEmpty, (unsigned long long)key);
+            if (prev == kEmpty || prev == key) return slot;
+        }
+        slot = (slot + 1) & mask;
+    }
+    atomicOr((unsigned long long*)errors, (unsigned long long)kErrProbe);
+    return kEmpty;
+}
+
+// ------------------------------------------------------------ storage reads
+
+// Bloom: presence of bin b of table i (bit b%8 of byte b/8, which is bit b%32
+// of the little-endian 32-bit word b/32).
+__device__ __forceinline__ bool bit_get(const uint8_t* t, uint64_t b) {
+    const uint32_t w = reinterpret_cast<const uint32_t*>(t)[b >> 5];
+    return (w >> (b & 31)) & 1u;
+}
+
+// BitStorage::query (bitstorage.cc:87-100), ByteStorage::query
+// (bytestorage.cc:116-139: min over tables, bounded by 255), exact-set count
+// (sparseppstorage.cc:28-31).
+__device__ int16_t storage_query(const Tables& T, const ExactSet& E, uint64_t h) {
+    if (T.kind == kExact) return exact_find(E, h) != kEmpty ? 1 : 0;
+    if (T.kind == kBit) {
+        GT_FOR_TABLES(i, T) {
+            if (!bit_get(T.base[i], gt_fastmod(h, T.p[i], T.m[i]))) return 0;
+        }
+        return 1;
+    }
+    int16_t m = 255;
+    GT_FOR_TABLES(i, T) {
+        int16_t c = T.base[i][gt_fastmod(h, T.p[i], T.m[i])];
+        m = c < m ? c : m;
+    }
+    return m;
+}
+
+// ------------------------------------------------------------ tile walker
+
+struct TileShared {
+    uint64_t tf[256];                 // T[c]
+    uint64_t tr[256];                 // T[complement(c)]
+    int64_t roff[kReadCache + 1];
+    uint8_t bytes[kTile + kMaxK];
+    uint32_t r_lo, r_hi;
+    int cached;
+};
+
+__device__ __forceinline__ void load_char_tables(TileShared& sh) {
+    for (int i = threadIdx.x; i < 256; i += kBlock) {
+        sh.tf[i] = c_char_hash[i];
+        sh.tr[i] = c_char_hash[gt_complement((unsigned char)i)];
+    }
+}
+
+__device__ __forceinline__ int64_t tile_count(const Reads& R) {
+    return (R.n_bases + kTile - 1) / kTile;
+}
+
+// Visits every k-mer window whose start lies in [R.b0, R.b1).  For windows
+// inside a read, op(s, read, read_start, fw, rc, valid) is called in start
+// order per thread; valid is false for skip-N windows holding a non-ACGT byte.
+template <class Op>
+__device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
+    const int tid = threadIdx.x;
+    const int K = R.K;
+    const unsigned Kr = (unsigned)K & 63u;
+    const int64_t n_tiles = tile_count(R);
+    const int64_t tile_first = R.b0 / kTile;
+    const int64_t tile_end = (R.b1 + kTile - 1) / kTile;
+    const bool aligned16 = ((uintptr_t)R.seq & 15u) == 0;
+
+    load_char_tables(sh);
+    for (int64_t tile = tile_first + blockIdx.x; tile < tile_end; tile += gridDim.x) {
+        __syncthreads();
+        const int64_t t0 = tile * kTile;
+        const int span = kTile + K - 1;
+        const int64_t avail = R.n_bases - t0;
+        const int nload = (int)(avail < span ? avail : span);
+        if (aligned16) {
+            const int nvec = nload >> 4;
+            const uint4* src = reinterpret_cast<const uint4*>(R.seq + t0);
+            uint4* dst = reinterpret_cast<uint4*>(sh.bytes);
+            for (int i = tid; i < nvec; i += kBlock) dst[i] = src[i];
+            for (int i = (nvec << 4) + tid; i < span; i += kBlock) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
+        } else {
+            for (int i = tid; i < span; i += kBlock) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
+        }
+        if (tid == 0) {
+            uint32_t rl = R.tile_read[tile];
+            uint32_t rh = (tile + 1 < n_tiles) ? R.tile_read[tile + 1] : (uint32_t)(R.n_reads - 1);
+            if (rl >= R.n_reads) rl = (uint32_t)(R.n_reads - 1);
+            if (rh >= R.n_reads) rh = (uint32_t)(R.n_reads - 1);
+            if (rh < rl) rh = rl;
+            sh.r_lo = rl;
+            sh.r_hi = rh;
+            sh.cached = (rh - rl + 2) <= (kReadCache + 1);
+        }
+        __syncthreads();
+        const uint32_t rl = sh.r_lo, rh = sh.r_hi;
+        const bool cached = sh.cached;
+        if (cached) {
+            for (uint32_t i = tid; i < rh - rl + 2; i += kBlock) sh.roff[i] = R.off[rl + i];
+        }
+        __syncthreads();
+
+        op.begin_tile(t0);
+        const int64_t s0 = t0 + (int64_t)tid * kPer;
+        int64_t s_end = s0 + kPer;
+        if (s_end > R.b1) s_end = R.b1;
+        if (s0 >= s_end) continue;
+
+        auto OFF = [&](uint32_t r) -> int64_t { return cached ? sh.roff[r - rl] : R.off[r]; };
+        uint32_t lo = rl, hi = rh;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (OFF(mid) <= s0) lo = mid; else hi = mid - 1;
+        }
+        uint32_t r = lo;
+        int64_t rstart = OFF(r);
+        int64_t rend = OFF(r + 1);
+
+        const uint8_t* B = sh.bytes;
+        const int li = (int)(s0 - t0);
+        uint64_t fw = 0, rc = 0;
+        int bad = 0;
+        for (int k = 0; k < K; ++k) {
+            const uint8_t c = B[li + k];
+            fw = gt_rotl(fw, 1) ^ sh.tf[c];
+            bad += !gt_is_acgt(c);
+        }
+        for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
+
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t s = s0 + j;
+            if (s >= s_end) break;
+            if (j > 0) {
+                const uint8_t out = B[li + j - 1], in = B[li + j + K - 1];
+                // CyclicHash::update (fw) and reverse_update on complements (rc)
+                fw = gt_rotl(fw, 1) ^ gt_rotl(sh.tf[out], Kr) ^ sh.tf[in];
+                rc = gt_rotr(rc ^ gt_rotl(sh.tr[in], Kr) ^ sh.tr[out], 1);
+                bad += (int)!gt_is_acgt(in) - (int)!gt_is_acgt(out);
+            }
+            while (s >= rend && r < rh) {
+                ++r;
+                rstart = rend;
+                rend = OFF(r + 1);
+            }
+            if (s < rstart || s + K > rend) continue;
+            op(s, r, rstart, fw, rc, R.mode == kRaw || bad == 0);
+        }
+        op.flush();
+    }
+}
+
+__device__ __forceinline__ uint64_t kmer_value(uint64_t fw, uint64_t rc, int canonical) {
+    return canonical ? (fw < rc ? fw : rc) : fw;
+}
+
+// Per-read accumulation of new k-mers: one atomic per run of the same read.
+struct ReadAcc {
+    uint64_t* out;
+    uint32_t cur = 0xFFFFFFFFu;
+    uint64_t cnt = 0;
+    __device__ __forceinline__ void add(uint32_t r, uint64_t v) {
+        if (!out) return;
+        if (r != cur) {
+            flush();
+            cur = r;
+        }
+        cnt += v;
+    }
+    __device__ __forceinline__ void flush() {
+        if (out && cnt) atomicAdd((unsigned long long*)&out[cur], (unsigned long long)cnt);
+        cnt = 0;
+    }
+};
+
+// ------------------------------------------------------------ Bloom / CMS ops
+
+// Claim pass: reads the tables only.  For each table whose bin is empty at
+// batch start, records the bin in the claim map with this event's stamp and
+// sets bit i of the fresh mask (bit 7 stands for every table >= 7).
+struct ClaimOp {
+    Tables T;
+    ClaimMap cm;
+    uint8_t* fresh;
+    int64_t b0;
+    int canonical;
+    uint64_t* errors;
+    uint64_t n_fresh = 0;
+
+    __device__ void operator()(int64_t s, uint32_t, int64_t, uint64_t fw, uint64_t rc, bool valid) {
+        if (!valid) return;
+        const uint64_t h = kmer_value(fw, rc, canonical);
+        const uint32_t ord = (uint32_t)(s - b0);
+        uint8_t mask = 0;
+        GT_FOR_TABLES(i, T) {
+            const uint64_t b = gt_fastmod(h, T.p[i], T.m[i]);
+            const bool empty = T.kind == kBit ? !bit_get(T.base[i], b) : T.base[i][b] == 0;
+            if (empty) {
+                mask |= (uint8_t)(1u << (i < 7 ? i : 7));
+                claim_insert(cm, claim_key(cm.gen, i, b), claim_stamp(cm.gen, ord), errors);
+            }
+        }
+        fresh[s - b0] = mask;
+        n_fresh += mask != 0;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() {}
+};
+
+// Saturating byte increment (bytestorage.cc:88-93: +1 while below 255) with
+// a 32-bit CAS on the enclosing word; returns the count before the insert.
+__device__ __forceinline__ uint32_t byte_sat_inc(uint8_t* t, uint64_t b) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(t + (b & ~3ULL));
+    const uint32_t sh = (uint32_t)(b & 3) * 8u;
+    uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint32_t c = (old >> sh) & 0xFFu;
+        if (c == 255u) return c;
+        const uint32_t prev = atomicCAS(w, old, old + (1u << sh));
+        if (prev == old) return c;
+        old = prev;
+    }
+}
+
+// Apply pass: updates every table (atomicOr for Bloom, saturating CAS for
+// count-min), counts table-0 bins that go from empty to occupied, and
+// decides newness from the claim map.
+struct ApplyOp {
+    Tables T;
+    ClaimMap cm;
+    const uint8_t* fresh;
+    int64_t b0;
+    int canonical;
+    const int64_t* kmer_off;
+    uint8_t* new_flags;
+    ReadAcc acc;
+    uint64_t n_new = 0, n_occ = 0;
+
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        if (!valid) {
+            if (new_flags) new_flags[kmer_off[r] + (s - rstart)] = 0;
+            return;
+        }
+        const uint64_t h = kmer_value(fw, rc, canonical);
+        const uint8_t mask = fresh[s - b0];
+        const uint64_t stamp = claim_stamp(cm.gen, (uint32_t)(s - b0));
+        bool is_new = false;
+        GT_FOR_TABLES(i, T) {
+            const uint64_t b = gt_fastmod(h, T.p[i], T.m[i]);
+            if (T.kind == kBit) {
+                const uint32_t bit = 1u << (b & 31);
+                const uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(T.base[i]) + (b >> 5), bit);
+                if (i == 0 && !(old & bit)) n_occ++;
+            } else {
+                const uint32_t before = byte_sat_inc(T.base[i], b);
+                if (i == 0 && before == 0) n_occ++;
+            }
+            if (!is_new && (mask & (1u << (i < 7 ? i : 7)))) {
+                is_new = claim_find(cm, claim_key(cm.gen, i, b)) == stamp;
+            }
+        }
+        n_new += is_new;
+        acc.add(r, is_new);
+        if (new_flags) new_flags[kmer_off[r] + (s - rstart)] = is_new;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() { acc.flush(); }
+};
+
+// ------------------------------------------------------------ exact-set ops
+
+// Lookup pass: a window is fresh iff its value is absent at batch start.
+struct ExactLookupOp {
+    ExactSet E;
+    uint8_t* fresh;
+    int64_t b0;
+    int canonical;
+    uint64_t n_fresh = 0;
+    __device__ void operator()(int64_t s, uint32_t, int64_t, uint64_t fw, uint64_t rc, bool valid) {
+        if (!valid) return;
+        const uint8_t f = exact_find(E, kmer_value(fw, rc, canonical)) == kEmpty;
+        fresh[s - b0] = f;
+        n_fresh += f;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() {}
+};
+
+// Insert pass: fresh windows insert their value and stamp its slot.
+struct ExactInsertOp {
+    ExactSet E;
+    const uint8_t* fresh;
+    int64_t b0;
+    int canonical;
+    uint32_t gen;
+    uint64_t* errors;
+    __device__ void operator()(int64_t s, uint32_t, int64_t, uint64_t fw, uint64_t rc, bool valid) {
+        if (!valid || !fresh[s - b0]) return;
+        const uint64_t slot = exact_insert(E, kmer_value(fw, rc, canonical), errors);
+        if (slot != kEmpty) {
+            atomicMax((unsigned long long*)&E.stamp[slot],
+                      (unsigned long long)claim_stamp(gen, (uint32_t)(s - b0)));
+        }
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() {}
+};
+
+// Resolve pass: the fresh window holding the smallest stamp of its slot is
+// the one goetia's sequential insert reports as new (sparseppstorage.hh:92-97).
+struct ExactResolveOp {
+    ExactSet E;
+    const uint8_t* fresh;
+    int64_t b0;
+    int canonical;
+    uint32_t gen;
+    const int64_t* kmer_off;
+    uint8_t* new_flags;
+    ReadAcc acc;
+    uint64_t n_new = 0;
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        bool is_new = false;
+        if (valid && fresh[s - b0]) {
+            const uint64_t slot = exact_find(E, kmer_value(fw, rc, canonical));
+            is_new = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)(s - b0));
+        }
+        n_new += is_new;
+        acc.add(r, is_new);
+        if (new_flags) new_flags[kmer_off[r] + (s - rstart)] = is_new;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() { acc.flush(); }
+};
+
+// ------------------------------------------------------------ query / degrees
+
+// query_sequence (dbg.hh:364-376): per window count (or -1 when skipped) and
+// hash value.
+struct QueryOp {
+    Tables T;
+    ExactSet E;
+    int canonical;
+    const int64_t* kmer_off;
+    int16_t* counts;
+    uint64_t* hashes;
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        const int64_t k = kmer_off[r] + (s - rstart);
+        if (!valid) {
+            if (counts) counts[k] = -1;
+            if (hashes) hashes[k] = 0;
+            return;
+        }
+        const uint64_t h = kmer_value(fw, rc, canonical);
+        if (counts) counts[k] = storage_query(T, E, h);
+        if (hashes) hashes[k] = h;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() {}
+};
+
+// Hashing only: fw / rc / validity per window.
+struct HashOp {
+    const int64_t* kmer_off;
+    uint64_t* fw_out;
+    uint64_t* rc_out;
+    uint8_t* valid_out;
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        const int64_t k = kmer_off[r] + (s - rstart);
+        if (fw_out) fw_out[k] = fw;
+        if (rc_out) rc_out[k] = rc;
+        if (valid_out) valid_out[k] = valid;
+    }
+    __device__ void begin_tile(int64_t) {}
+    __device__ void flush() {}
+};
+
+// In/out degree of each window: the 8 one-symbol extensions in closed form
+// (equal to hashextender.hh:352-364,386-396 shift-and-undo), counted as present
+// when their query is >= min_count (unitig_walker.hh:361-371 uses != 0, which
+// is min_count 1).  Windows not selected by `only` get 0xFF.
+struct DegreeOp {
+    Tables T;
+    ExactSet E;
+    int canonical;
+    int K;
+    int min_count;
+    const int64_t* kmer_off;
+    const uint8_t* only;
+    uint8_t* in_deg;
+    uint8_t* out_deg;
+    const uint8_t* bytes;   // LDS tile
+    int64_t t0;
+    const uint64_t* tf;
+    const uint64_t* tr;
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        const int64_t k = kmer_off[r] + (s - rstart);
+        if (!valid || (only && !only[k])) {
+            in_deg[k] = 0xFF;
+            out_deg[k] = 0xFF;
+            return;
+        }
+        const unsigned Kr = (unsigned)K & 63u;
+        const uint8_t first = bytes[s - t0];
+        const uint8_t last = bytes[s - t0 + K - 1];
+        uint8_t nin = 0, nout = 0;
+        const uint64_t rf = gt_rotl(fw, 1) ^ gt_rotl(tf[first], Kr);
+        const uint64_t lr = gt_rotl(rc, 1) ^ gt_rotl(tr[last], Kr);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint8_t x = (uint8_t)"ACGT"[c];
+            // right extension: shift_right(first, x)
+            const uint64_t F1 = rf ^ tf[x];
+            const uint64_t R1 = gt_rotr(rc ^ gt_rotl(tr[x], Kr) ^ tr[first], 1);
+            nout += storage_query(T, E, kmer_value(F1, R1, canonical)) >= min_count;
+            // left extension: shift_left(x, last)
+            const uint64_t F2 = gt_rotr(fw ^ gt_rotl(tf[x], Kr) ^ tf[last], 1);
+            const uint64_t R2 = lr ^ tr[x];
+            nin += storage_query(T, E, kmer_value(F2, R2, canonical)) >= min_count;
+        }
+        in_deg[k] = nin;
+        out_deg[k] = nout;
+    }
+    __device__ void begin_tile(int64_t tile_base) { t0 = tile_base; }
+    __device__ void flush() {}
+};
+
+// ------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__(kBlock) void claim_kernel(Reads R, Tables T, ClaimMap cm, uint8_t* fresh,
+                                                     uint64_t* counters) {
+    __shared__ TileShared sh;
+    ClaimOp op{T, cm, fresh, R.b0, R.canonical, counters + kErrors};
+    walk_tiles(R, sh, op);
+    add_counter(counters + kFresh, op.n_fresh);
+}
+
+__global__ __launch_bounds__(kBlock) void apply_kernel(Reads R, Tables T, ClaimMap cm, const uint8_t* fresh,
+                                                     uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
+    __shared__ TileShared sh;
+    ApplyOp op{T, cm, fresh, R.b0, R.canonical, R.kmer_off, new_flags, ReadAcc{n_new}};
+    walk_tiles(R, sh, op);
+    add_counter(counters + kNUnique, op.n_new);
+    add_counter(counters + kNOccupied, op.n_occ);
+}
+
+__global__ __launch_bounds__(kBlock) void exact_lookup_kernel(Reads R, ExactSet E, uint8_t* fresh,
+                                                            uint64_t* counters) {
+    __shared__ TileShared sh;
+    ExactLookupOp op{E, fresh, R.b0, R.canonical};
+    walk_tiles(R, sh, op);
+    add_counter(counters + kFresh, op.n_fresh);
+}
+
+__global__ __launch_bounds__(kBlock) void exact_insert_kernel(Reads R, ExactSet E, const uint8_t* fresh,
+                                                            uint32_t gen, uint64_t* counters) {
+    __shared__ TileShared sh;
+    ExactInsertOp op{E, fresh, R.b0, R.canonical, gen, counters + kErrors};
+    walk_tiles(R, sh, op);
+}
+
+__global__ __launch_bounds__(kBlock) void exact_resolve_kernel(Reads R, ExactSet E, const uint8_t* fresh,
+                                                             uint32_t gen, uint64_t* n_new, uint8_t* new_flags,
+                                                             uint64_t* counters) {
+    __shared__ TileShared sh;
+    ExactResolveOp op{E, fresh, R.b0, R.canonical, gen, R.kmer_off, new_flags, ReadAcc{n_new}};
+    walk_tiles(R, sh, op);
+    add_counter(counters + kNUnique, op.n_new);
+}
+
+__global__ __launch_bounds__(kBlock) void query_kernel(Reads R, Tables T, ExactSet E, int16_t* counts,
+                                                     uint64_t* hashes) {
+    __shared__ TileShared sh;
+    QueryOp op{T, E, R.canonical, R.kmer_off, counts, hashes};
+    walk_tiles(R, sh, op);
+}
+
+__global__ __launch_bounds__(kBlock) void hash_kernel(Reads R, uint64_t* fw, uint64_t* rc, uint8_t* valid) {
+    __shared__ TileShared sh;
+    HashOp op{R.kmer_off, fw, rc, valid};
+    walk_tiles(R, sh, op);
+}
+
+__global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, ExactSet E, int min_count,
+                                                      const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
+    __shared__ TileShared sh;
+    DegreeOp op{T, E, R.canonical, R.K, min_count, R.kmer_off, only, in_deg, out_deg,
+                sh.bytes, 0, sh.tf, sh.tr};
+    walk_tiles(R, sh, op);
+}
+
+// Per-hash operations (dbg.hh:140-177 insert / query / insert_and_query on a
+// hash value), over an array in index order.
+__global__ __launch_bounds__(kBlock) void hash_claim_kernel(const uint64_t* h, uint64_t n, Tables T, ClaimMap cm,
+                                                          uint8_t* fresh, uint64_t* counters) {
+    uint64_t nf = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint8_t mask = 0;
+        GT_FOR_TABLES(t, T) {
+            const uint64_t b = gt_fastmod(h[i], T.p[t], T.m[t]);
+            const bool empty = T.kind == kBit ? !bit_get(T.base[t], b) : T.base[t][b] == 0;
+            if (empty) {
+                mask |= (uint8_t)(1u << (t < 7 ? t : 7));
+                claim_insert(cm, claim_key(cm.gen, t, b), claim_stamp(cm.gen, (uint32_t)i), counters + kErrors);
+            }
+        }
+        fresh[i] = mask;
+        nf += mask != 0;
+    }
+    add_counter(counters + kFresh, nf);
+}
+
+__global__ __launch_bounds__(kBlock) void hash_apply_kernel(const uint64_t* h, uint64_t n, Tables T, ClaimMap cm,
+                                                          const uint8_t* fresh, uint8_t* is_new,
+                                                          uint64_t* counters) {
+    uint64_t nn = 0, occ = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const uint8_t mask = fresh[i];
+        const uint64_t stamp = claim_stamp(cm.gen, (uint32_t)i);
+        bool nw = false;
+        GT_FOR_TABLES(t, T) {
+            const uint64_t b = gt_fastmod(h[i], T.p[t], T.m[t]);
+            if (T.kind == kBit) {
+                const uint32_t bit = 1u << (b & 31);
+                const uint32_t old = atomicOr(reinterpret_cast<uint32_t*>(T.base[t]) + (b >> 5), bit);
+                if (t == 0 && !(old & bit)) occ++;
+            } else {
+                if (byte_sat_inc(T.base[t], b) == 0 && t == 0) occ++;
+            }
+            if (!nw && (mask & (1u << (t < 7 ? t : 7)))) nw = claim_find(cm, claim_key(cm.gen, t, b)) == stamp;
+        }
+        nn += nw;
+        if (is_new) is_new[i] = nw;
+    }
+    add_counter(counters + kNUnique, nn);
+    add_counter(counters + kNOccupied, occ);
+}
+
+__global__ __launch_bounds__(kBlock) void hash_exact_lookup_kernel(const uint64_t* h, uint64_t n, ExactSet E,
+                                                                 uint8_t* fresh, uint64_t* counters) {
+    uint64_t nf = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const uint8_t f = exact_find(E, h[i]) == kEmpty;
+        fresh[i] = f;
+        nf += f;
+    }
+    add_counter(counters + kFresh, nf);
+}
+
+__global__ __launch_bounds__(kBlock) void hash_exact_insert_kernel(const uint64_t* h, uint64_t n, ExactSet E,
+                                                                 const uint8_t* fresh, uint32_t gen,
+                                                                 uint64_t* counters) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        if (!fresh[i]) continue;
+        const uint64_t slot = exact_insert(E, h[i], counters + kErrors);
+        if (slot != kEmpty) {
+            atomicMax((unsigned long long*)&E.stamp[slot], (unsigned long long)claim_stamp(gen, (uint32_t)i));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void hash_exact_resolve_kernel(const uint64_t* h, uint64_t n, ExactSet E,
+                                                                  const uint8_t* fresh, uint32_t gen,
+                                                                  uint8_t* is_new, uint64_t* counters) {
+    uint64_t nn = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        bool nw = false;
+        if (fresh[i]) {
+            const uint64_t slot = exact_find(E, h[i]);
+            nw = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)i);
+        }
+        nn += nw;
+        if (is_new) is_new[i] = nw;
+    }
+    add_counter(counters + kNUnique, nn);
+}
+
+__global__ __launch_bounds__(kBlock) void hash_query_kernel(const uint64_t* h, uint64_t n, Tables T, ExactSet E,
+                                                          int16_t* counts) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        counts[i] = storage_query(T, E, h[i]);
+    }
+}
+
+// Count-min insert_and_query in strict index order (bytestorage.cc:60-150):
+// one lane walks the k-mers, so every result sees exactly the inserts before
+// it.  Used for per-read calls, whose k-mer counts are small.  valid may be
+// null; invalid entries get -1 and no insert.
+__global__ void seq_insert_query_kernel(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n,
+                                        int canonical, Tables T, int16_t* counts, uint8_t* is_new,
+                                        uint64_t* counters) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t nn = 0, occ = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+        if (valid && !valid[k]) {
+            if (counts) counts[k] = -1;
+            if (is_new) is_new[k] = 0;
+            continue;
+        }
+        const uint64_t h = rc ? kmer_value(fw[k], rc[k], canonical) : fw[k];
+        bool nw = false;
+        int16_t m = 255;
+        GT_FOR_TABLES(i, T) {
+            uint8_t* cell = T.base[i] + gt_fastmod(h, T.p[i], T.m[i]);
+            const uint8_t c = *cell;
+            if (!nw && c == 0) {
+                nw = true;
+                if (i == 0) occ++;
+            }
+            const uint8_t after = c < 255 ? (uint8_t)(c + 1) : c;
+            *cell = after;
+            m = after < m ? after : m;
+        }
+        nn += nw;
+        if (counts) counts[k] = nw ? 1 : m;
+        if (is_new) is_new[k] = nw;
+    }
+    counters[kNUnique] += nn;
+    counters[kNOccupied] += occ;
+}
+
+// ------------------------------------------------------------ read layout
+
+// tile_read[t] = read holding base t*kTile (reads are [off[r], off[r+1])).
+__global__ void tile_read_kernel(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
+                                 int64_t n_tiles) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n_reads;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        int64_t a = off[r], b = off[r + 1];
+        a = a < 0 ? 0 : (a > n_bases ? n_bases : a);
+        b = b < a ? a : (b > n_bases ? n_bases : b);
+        for (int64_t t = (a + kTile - 1) / kTile; t * kTile < b && t < n_tiles; ++t) tile_read[t] = (uint32_t)r;
+    }
+}
+
+// Exclusive scan of per-read k-mer counts max(0, len - K + 1) in three steps:
+// per-block scan, scan of block sums, add-back.
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 4;
+constexpr int kScanChunk = kScanBlock * kScanPer;
+
+__device__ __forceinline__ int64_t read_kmers(const int64_t* off, uint64_t r, int K) {
+    const int64_t len = off[r + 1] - off[r];
+    return len >= K ? len - K + 1 : 0;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_local_kernel(const int64_t* off, uint64_t n_reads, int K,
+                                                              int64_t* kmer_off, int64_t* block_sums) {
+    __shared__ int64_t part[kScanBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanPer;
+    int64_t v[kScanPer];
+    int64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        v[j] = base + j < n_reads ? read_kmers(off, base + j, K) : 0;
+        sum += v[j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+        int64_t add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int64_t run = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        if (base + j < n_reads) kmer_off[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == kScanBlock - 1) block_sums[blockIdx.x] = part[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_blocks_kernel(int64_t* block_sums, uint64_t n_blocks,
+                                                               int64_t* total) {
+    __shared__ int64_t part[kScanBlock];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < n_blocks; base += kScanBlock) {
+        const uint64_t i = base + threadIdx.x;
+        const int64_t v = i < n_blocks ? block_sums[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kScanBlock; o <<= 1) {
+            int64_t add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n_blocks) block_sums[i] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) carry += part[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add_kernel(int64_t* kmer_off, uint64_t n_reads,
+                                                            const int64_t* block_sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    const int64_t add = block_sums[blockIdx.x];
+    for (int j = threadIdx.x; j < kScanChunk; j += kScanBlock) {
+        if (base + j < n_reads) kmer_off[base + j] += add;
+    }
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Bloom union (bitstorage.cc:103-137): me |= other, counting table-0 bits
+// that become set.
+__global__ void bit_union_kernel(uint32_t* me, const uint32_t* other, uint64_t n_words, int count_bits,
+                                 uint64_t* counters) {
+    uint64_t added = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = me[i], b = other[i];
+        const uint32_t c = a | b;
+        if (count_bits) added += __popc(c ^ a);
+        me[i] = c;
+    }
+    add_counter(counters + kNOccupied, added);
+}
+
+// Rehash of an exact set into a larger one (stamps are per batch and not kept).
+__global__ void exact_rehash_kernel(ExactSet from, ExactSet to, uint64_t* counters) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < from.cap;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = from.keys[i];
+        if (k != kEmpty) exact_insert(to, k, counters + kErrors);
+    }
+}
+
+// Compacts the keys of an exact set into out (count in *n).
+__global__ void exact_keys_kernel(ExactSet E, uint64_t* out, uint64_t cap_out, unsigned long long* n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E.cap;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = E.keys[i];
+        if (k != kEmpty) {
+            const unsigned long long j = atomicAdd(n, 1ULL);
+            if (j < cap_out) out[j] = k;
+        }
+    }
+}
+
+}  // namespace gt
+
+// ------------------------------------------------------------ launchers
+
+#include "gt_launch.h"
+
+namespace gt {
+
+static int tile_grid(const Launch& L, const Reads& R) {
+    const int64_t tiles = (R.b1 + kTile - 1) / kTile - R.b0 / kTile;
+    const int64_t g = tiles < L.grid ? tiles : L.grid;
+    return g > 0 ? (int)g : 1;
+}
+
+static int flat_grid(const Launch& L, uint64_t n) {
+    const uint64_t need = (n + kBlock - 1) / kBlock;
+    const uint64_t g = need < (uint64_t)L.grid * 4 ? need : (uint64_t)L.grid * 4;
+    return g > 0 ? (int)g : 1;
+}
+
+void launch_tile_read(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
+                      int64_t n_tiles, hipStream_t s) {
+    const uint64_t g = (n_reads + 255) / 256;
+    tile_read_kernel<<<(unsigned)(g < 65536 ? (g ? g : 1) : 65536), 256, 0, s>>>(off, n_reads, n_bases, tile_read,
+                                                                              n_tiles);
+}
+
+uint64_t scan_block_count(uint64_t n_reads) { return (n_reads + kScanChunk - 1) / kScanChunk; }
+
+void launch_kmer_offsets(const int64_t* off, uint64_t n_reads, int K, int64_t* kmer_off, int64_t* block_sums,
+                         hipStream_t s) {
+    const uint64_t nb = scan_block_count(n_reads);
+    if (nb) scan_local_kernel<<<(unsigned)nb, kScanBlock, 0, s>>>(off, n_reads, K, kmer_off, block_sums);
+    scan_blocks_kernel<<<1, kScanBlock, 0, s>>>(block_sums, nb, kmer_off + n_reads);
+    if (nb) scan_add_kernel<<<(unsigned)nb, kScanBlock, 0, s>>>(kmer_off, n_reads, block_sums);
+}
+
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+    if (!n) return;
+    const uint64_t g = (n + 255) / 256;
+    fill_u64_kernel<<<(unsigned)(g < 16384 ? g : 16384), 256, 0, s>>>(p, n, v);
+}
+
+void launch_claim(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, uint8_t* fresh,
+                  uint64_t* counters) {
+    claim_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, cm, fresh, counters);
+}
+
+void launch_apply(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, const uint8_t* fresh,
+                  uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
+    apply_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, cm, fresh, n_new, new_flags, counters);
+}
+
+void launch_exact_lookup(const Launch& L, const Reads& R, const ExactSet& E, uint8_t* fresh, uint64_t* counters) {
+    exact_lookup_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, counters);
+}
+
+void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+                         uint64_t* counters) {
+    exact_insert_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, gen, counters);
+}
+
+void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+                          uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
+    exact_resolve_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, gen, n_new, new_flags, counters);
+}
+
+void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,
+                  uint64_t* hashes) {
+    query_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, E, counts, hashes);
+}
+
+void launch_hash(const Launch& L, const Reads& R, uint64_t* fw, uint64_t* rc, uint8_t* valid) {
+    hash_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, fw, rc, valid);
+}
+
+void launch_degrees(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int min_count,
+                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
+    degree_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, E, min_count, only, in_deg, out_deg);
+}
+
+void launch_hash_claim(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
+                       uint8_t* fresh, uint64_t* counters) {
+    hash_claim_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, T, cm, fresh, counters);
+}
+
+void launch_hash_apply(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
+                       const uint8_t* fresh, uint8_t* is_new, uint64_t* counters) {
+    hash_apply_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, T, cm, fresh, is_new, counters);
+}
+
+void launch_hash_exact_lookup(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E, uint8_t* fresh,
+                              uint64_t* counters) {
+    hash_exact_lookup_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, counters);
+}
+
+void launch_hash_exact_insert(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
+                              const uint8_t* fresh, uint32_t gen, uint64_t* counters) {
+    hash_exact_insert_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, gen, counters);
+}
+
+void launch_hash_exact_resolve(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
+                               const uint8_t* fresh, uint32_t gen, uint8_t* is_new, uint64_t* counters) {
+    hash_exact_resolve_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, gen, is_new, counters);
+}
+
+void launch_hash_query(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ExactSet& E,
+                       int16_t* counts) {
+    hash_query_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, T, E, counts);
+}
+
+void launch_bit_union(uint32_t* me, const uint32_t* other, uint64_t n_words, int count_bits, uint64_t* counters,
+                      hipStream_t s) {
+    const uint64_t g = (n_words + 255) / 256;
+    bit_union_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(me, other, n_words, count_bits,
+                                                                              counters);
+}
+
+void launch_exact_rehash(const ExactSet& from, const ExactSet& to, uint64_t* counters, hipStream_t s) {
+    const uint64_t g = (from.cap + 255) / 256;
+    exact_rehash_kernel<<<(unsigned)(g < 16384 ? g : 16384), 256, 0, s>>>(from, to, counters);
+}
+
+void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n, int canonical,
+                             const Tables& T, int16_t* counts, uint8_t* is_new, uint64_t* counters, hipStream_t s) {
+    seq_insert_query_kernel<<<1, 64, 0, s>>>(fw, rc, valid, n, canonical, T, counts, is_new, counters);
+}
+
+void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s) {
+    const uint64_t g = (E.cap + 255) / 256;
+    exact_keys_kernel<<<(unsigned)(g < 16384 ? g : 16384), 256, 0, s>>>(E, out, cap_out, n);
+}
+
+}  // namespace gt

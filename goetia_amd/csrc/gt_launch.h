@@ -1,0 +1,55 @@
+// gt_launch.h -- host-side launchers of the kernels in gt_kernels.hip.
+#pragma once
+
+#include "gt_internal.h"
+
+namespace gt {
+
+struct Launch {
+    int grid;              // workgroups for tile walkers (persistent grid)
+    hipStream_t stream;
+};
+
+void launch_tile_read(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
+                      int64_t n_tiles, hipStream_t s);
+void launch_kmer_offsets(const int64_t* off, uint64_t n_reads, int K, int64_t* kmer_off,
+                         int64_t* block_sums, hipStream_t s);
+uint64_t scan_block_count(uint64_t n_reads);
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+
+void launch_claim(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, uint8_t* fresh,
+                  uint64_t* counters);
+void launch_apply(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, const uint8_t* fresh,
+                  uint64_t* n_new, uint8_t* new_flags, uint64_t* counters);
+void launch_exact_lookup(const Launch& L, const Reads& R, const ExactSet& E, uint8_t* fresh, uint64_t* counters);
+void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+                         uint64_t* counters);
+void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+                          uint64_t* n_new, uint8_t* new_flags, uint64_t* counters);
+void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,
+                  uint64_t* hashes);
+void launch_hash(const Launch& L, const Reads& R, uint64_t* fw, uint64_t* rc, uint8_t* valid);
+void launch_degrees(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int min_count,
+                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg);
+
+void launch_hash_claim(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
+                       uint8_t* fresh, uint64_t* counters);
+void launch_hash_apply(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
+                       const uint8_t* fresh, uint8_t* is_new, uint64_t* counters);
+void launch_hash_exact_lookup(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E, uint8_t* fresh,
+                              uint64_t* counters);
+void launch_hash_exact_insert(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
+                              const uint8_t* fresh, uint32_t gen, uint64_t* counters);
+void launch_hash_exact_resolve(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
+                               const uint8_t* fresh, uint32_t gen, uint8_t* is_new, uint64_t* counters);
+void launch_hash_query(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ExactSet& E,
+                       int16_t* counts);
+
+void launch_bit_union(uint32_t* me, const uint32_t* other, uint64_t n_words, int count_bits, uint64_t* counters,
+                      hipStream_t s);
+void launch_exact_rehash(const ExactSet& from, const ExactSet& to, uint64_t* counters, hipStream_t s);
+void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n, int canonical,
+                             const Tables& T, int16_t* counts, uint8_t* is_new, uint64_t* counters, hipStream_t s);
+void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s);
+
+}  // namespace gt

@@ -1,0 +1,237 @@
+/*
+ * goetia_amd.h -- C ABI of the MI355X-native streaming de Bruijn graph hot
+ * path (canonical rolling k-mer hashing feeding Bloom, count-min and exact
+ * k-mer set storages).
+ *
+ * Every entry point is extern "C" with plain pointers and sizes.  Each one
+ * names the goetia (v1.0) interface it replaces; paths are relative to the
+ * goetia checkout.  The C++ facade in include/goetia/ and the Python package
+ * goetia_amd bind to this header only.
+ *
+ * Conventions
+ *   - Return value: GT_OK (0) or a GT_ERR_* code; gt_last_error() gives a
+ *     thread-local message for the last failure on the calling thread.
+ *   - "_dev" entry points take DEVICE pointers and enqueue work on `stream`
+ *     (a hipStream_t, NULL = the null stream); they return once the work is
+ *     queued.  Calls on one storage must be ordered on one stream.
+ *   - Entry points without "_dev" take HOST pointers and are synchronous.
+ *   - Reads are concatenated bytes `seq` with `offsets[n_reads + 1]`
+ *     (offsets[0] = 0, non-decreasing, offsets[n_reads] = n_bases); read r is
+ *     seq[offsets[r] .. offsets[r+1]).
+ *   - Per-k-mer outputs are packed per read: read r owns max(0, len_r - K + 1)
+ *     consecutive entries in window order (dbg.hh:349-394 returns the same
+ *     count per call).
+ *   - Input policy `mode`: GT_MODE_RAW hashes every window of the raw bytes,
+ *     exactly as dBG::insert_sequence does (dbg.hh:296-305; goetia does no
+ *     validation there, tests/test_dbg.py:117-123).  GT_MODE_SKIP_N skips
+ *     windows holding a byte outside {A,C,G,T}, as inserting each A/C/G/T run
+ *     with insert_sequence in read order would; skipped windows report count -1,
+ *     hash 0 and degree 0xFF.
+ *   - K is 1..255 (HashShifter's K is a uint16_t, hashshifter.hh:95).
+ */
+#ifndef GOETIA_AMD_H
+#define GOETIA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_OK                  0
+#define GT_ERR_INVALID_ARG     1
+#define GT_ERR_SEQUENCE_LENGTH 2   /* SequenceLengthException, kmeriterator.hh:57-59 */
+#define GT_ERR_OUT_OF_MEMORY   3
+#define GT_ERR_DEVICE          4
+#define GT_ERR_UNSUPPORTED     5
+#define GT_ERR_NO_DEVICE       6
+#define GT_ERR_IO              7   /* GoetiaFileException, bitstorage.cc:186 */
+#define GT_ERR_INTERNAL        8
+
+#define GT_BIT_STORAGE   1         /* BitStorage, bitstorage.hh:93 */
+#define GT_BYTE_STORAGE  2         /* ByteStorage, bytestorage.hh:96 */
+#define GT_EXACT_STORAGE 3         /* SparseppSetStorage, sparseppstorage.hh:38 */
+
+#define GT_MODE_RAW    0
+#define GT_MODE_SKIP_N 1
+
+typedef struct gt_storage gt_storage;
+
+/* ------------------------------------------------------------ library */
+
+/* Thread-local message of the last failed call on this thread. */
+const char* gt_last_error(void);
+/* ABI version of this header (GT_ABI_VERSION). */
+#define GT_ABI_VERSION 1
+int gt_abi_version(void);
+/* Selects and initialises the GPU; fails with GT_ERR_NO_DEVICE without one.
+ * Called implicitly by every device entry point. */
+int gt_device_init(int device);
+/* Waits for all work queued by this library on the device. */
+int gt_device_synchronize(void);
+
+/* storage.hh:166-190 get_n_primes_near_x: up to n odd primes strictly below
+ * x, descending.  Host only. */
+int gt_primes_near_x(uint32_t n, uint64_t x, uint64_t* out, uint32_t* n_out);
+
+/* ------------------------------------------------------------ storages */
+
+/* BitStorage(tablesizes) bitstorage.hh:114-121, ByteStorage(tablesizes)
+ * bytestorage.hh:144-154, SparseppSetStorage() sparseppstorage.hh:53-56.
+ * For Bloom / count-min, tablesizes[n_tables] are the bin counts (the
+ * reference's (max_table, N) constructor is tablesizes =
+ * gt_primes_near_x(N, max_table)).  For the exact set, tablesizes is
+ * ignored and capacity_hint is the expected number of distinct k-mers (the
+ * set grows as needed; 0 = a small default).  Tables live in HBM. */
+int gt_storage_new(int kind, const uint64_t* tablesizes, uint32_t n_tables,
+                   uint64_t capacity_hint, gt_storage** out);
+/* ~BitStorage / ~ByteStorage */
+int gt_storage_free(gt_storage* st);
+/* clone() bitstorage.hh:136-138: a new EMPTY storage of the same sizes. */
+int gt_storage_clone(const gt_storage* st, gt_storage** out);
+/* reset() bitstorage.cc:141-148, bytestorage.hh:171-177,
+ * sparseppstorage.hh:60-62: zero the tables.  Like the reference, the
+ * n_unique / n_occupied counters of Bloom and count-min storages are left
+ * as they are; the exact set's size is its count and goes to 0. */
+int gt_storage_reset(gt_storage* st);
+int gt_storage_kind(const gt_storage* st, int* kind);
+/* n_tables() / get_tablesizes() bitstorage.hh:159-167 */
+int gt_storage_n_tables(const gt_storage* st, uint32_t* n_tables);
+int gt_storage_tablesizes(const gt_storage* st, uint64_t* out);
+/* Bytes of table i in the reference layout: p/8+1 (Bloom,
+ * bitstorage.hh:151), p (count-min, bytestorage.hh:131). */
+int gt_storage_table_bytes(const gt_storage* st, uint32_t i, uint64_t* nbytes);
+/* n_unique_kmers() / n_occupied(): bitstorage.hh:173-181,
+ * bytestorage.hh:199-209, sparseppstorage.hh:72-86 (for the exact set
+ * n_occupied is the slot count of the device table, its bucket_count). */
+int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied);
+/* Restores counters after load() (bitstorage.cc:196-307). */
+int gt_storage_set_counters(gt_storage* st, uint64_t n_unique, uint64_t n_occupied);
+/* get_raw_tables() bitstorage.hh:228-231: copies table i (reference layout)
+ * to host memory of gt_storage_table_bytes() bytes. */
+int gt_storage_read_table(gt_storage* st, uint32_t i, uint8_t* host_dst);
+/* Overwrites table i from host memory (the load path, bitstorage.cc:196-307). */
+int gt_storage_write_table(gt_storage* st, uint32_t i, const uint8_t* host_src);
+/* update_from() bitstorage.cc:103-137: Bloom union, occupied bins of table 0
+ * fixed by popcount.  Bloom only. */
+int gt_storage_update_from(gt_storage* st, const gt_storage* other);
+/* Exact set contents (unordered) into host memory: writes min(n, cap) keys
+ * and sets *n to the set size. */
+int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint64_t* n);
+
+/* ------------------------------------------------------------ hashing */
+
+/* Prefix-summed k-mer offsets: kmer_offsets[r] = sum over reads < r of
+ * max(0, len - K + 1); n_reads + 1 entries.  Device pointers. */
+int gt_kmer_offsets_dev(uint16_t K, const int64_t* offsets, uint64_t n_reads,
+                        int64_t* kmer_offsets, void* stream);
+
+/* KmerIterator<CanLemireShifter> over every read (kmeriterator.hh:109-123,
+ * rollinghashshifter.hh:180-219): forward and reverse-complement cyclic hash
+ * of every window.  Canonical = min(fw, rc) (canonical.hh:124-126);
+ * FwdLemireShifter's value is fw.  valid (optional) flags windows kept by
+ * the input policy.  Device pointers, packed per read. */
+int gt_hash_reads_dev(uint16_t K, int mode, const uint8_t* seq, const int64_t* offsets,
+                      uint64_t n_reads, uint64_t n_bases, uint64_t* fw, uint64_t* rc,
+                      uint8_t* valid, void* stream);
+int gt_hash_reads(uint16_t K, int mode, const uint8_t* seq, const int64_t* offsets,
+                  uint64_t n_reads, uint64_t* fw, uint64_t* rc, uint8_t* valid);
+
+/* Single-k-mer shifter state of HashShifter<LemireShifterPolicy<...>>, kept
+ * by the caller as (fw, rc) (host scalar code, as goetia's own per-call
+ * shifter API):  hash_base hashshifter.hh:147-170 /
+ * rollinghashshifter.hh:180-197 (kmer holds K bytes);  shift_right(out, in)
+ * rollinghashshifter.hh:200-208;  shift_left(in, out) :211-219.  The forward
+ * shifter uses fw only.  Sequence-level hashing runs on the GPU
+ * (gt_hash_reads*). */
+int gt_shifter_hash_base(uint16_t K, const uint8_t* kmer, uint64_t* fw, uint64_t* rc);
+int gt_shifter_shift_right(uint16_t K, uint64_t* fw, uint64_t* rc, uint8_t out, uint8_t in);
+int gt_shifter_shift_left(uint16_t K, uint64_t* fw, uint64_t* rc, uint8_t in, uint8_t out);
+
+/* ------------------------------------------------------------ dBG batches */
+
+/* dBG::insert_sequence(seq, n_new) (dbg.hh:307-318) over every read in order:
+ * tables end byte-identical to the sequential reference, n_new[r] is the
+ * number of k-mers of read r that were new when inserted in read order
+ * (bitstorage.hh:195-219, bytestorage.cc:60-113, sparseppstorage.hh:92-97),
+ * and new_flags (optional, per k-mer) marks them.  n_new (optional) has
+ * n_reads entries.  canonical selects CanLemireShifter (else Fwd). */
+int gt_dbg_insert_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                      const int64_t* offsets, uint64_t n_reads, uint64_t n_bases,
+                      uint64_t* n_new, uint8_t* new_flags, void* stream);
+int gt_dbg_insert(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                  const int64_t* offsets, uint64_t n_reads, uint64_t* n_new, uint8_t* new_flags,
+                  uint64_t* n_kmers);
+
+/* dBG::query_sequence(seq, counts, hashes) (dbg.hh:364-376) over every read:
+ * counts (count_t = int16, storage.hh:80) and hashes (optional) packed per
+ * read. */
+int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                     const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, int16_t* counts,
+                     uint64_t* hashes, void* stream);
+int gt_dbg_query(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                 const int64_t* offsets, uint64_t n_reads, int16_t* counts, uint64_t* hashes);
+
+/* dBG::insert_and_query_sequence (dbg.hh:327-340) over every read: the
+ * post-insert count of each k-mer as the sequential reference returns it
+ * (1 for Bloom and exact set, bitstorage.cc:78-84, sparseppstorage.cc:21-25;
+ * count-min: 1 if new else the post-insert minimum, bytestorage.cc:142-150). */
+int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int mode,
+                                const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                                uint64_t n_bases, int16_t* counts, void* stream);
+int gt_dbg_insert_and_query(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                            const int64_t* offsets, uint64_t n_reads, int16_t* counts);
+
+/* In/out degree of every window (UnitigWalker::in_degree / out_degree,
+ * unitig_walker.hh:322-330, counted as in count_nodes :361-371 with the
+ * presence test query >= min_count; the reference's != 0 is min_count 1).
+ * A window is a decision k-mer when in > 1 or out > 1 (is_decision_kmer
+ * :1016-1023, find_decision_kmers :1025-1045).  only (optional, per k-mer)
+ * restricts the work to flagged windows; others get 0xFF. */
+int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count,
+                       const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                       uint64_t n_bases, const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg,
+                       void* stream);
+int gt_dbg_degrees(gt_storage* st, uint16_t K, int canonical, int mode, int min_count,
+                   const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                   const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg);
+
+/* ------------------------------------------------------------ per-hash */
+
+/* dBG::insert(hash) / Storage::insert (dbg.hh:140-146, storage.hh:120)
+ * applied to hashes[0..n) in index order; is_new (optional) per hash. */
+int gt_insert_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, uint8_t* is_new,
+                         void* stream);
+int gt_insert_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, uint8_t* is_new);
+/* Storage::query (dbg.hh:171-177). */
+int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts,
+                        void* stream);
+int gt_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts);
+/* Storage::insert_and_query (dbg.hh:156-162) in index order. */
+int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n,
+                               int16_t* counts);
+
+/* ------------------------------------------------------------ workload */
+
+/* Seeded synthetic workload (host memory, deterministic for a seed; no
+ * counterpart in goetia, whose tests draw random sequences from an external
+ * fixture package).  gt_gen_genome: uniform A/C/G/T bases from xoshiro256**
+ * seeded through splitmix64.  gt_gen_reads: read r draws its own generator
+ * from (seed, r); its length is lengths[r] when lengths is not NULL, else
+ * uniform in [len_min, len_max]; it starts uniformly in the genome, is
+ * reverse complemented with probability 1/2, and each base is substituted
+ * by a different base with probability sub_rate, then replaced by 'N' with
+ * probability n_rate.  seq must hold the sum of the lengths (at most
+ * n_reads * len_max); offsets has n_reads + 1 entries; *n_bases receives the
+ * total.  Reads longer than the genome are clipped to it. */
+int gt_gen_genome(uint64_t seed, uint64_t length, uint8_t* out);
+int gt_gen_reads(uint64_t seed, const uint8_t* genome, uint64_t genome_len, uint64_t n_reads,
+                 const uint32_t* lengths, uint32_t len_min, uint32_t len_max, double sub_rate,
+                 double n_rate, uint8_t* seq, int64_t* offsets, uint64_t* n_bases);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOETIA_AMD_H */

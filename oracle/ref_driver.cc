@@ -1,0 +1,252 @@
+// ref_driver.cc -- extern "C" entry points over the compiled goetia reference
+// hot path, used only to cross-check the C restatement (oracle.c) and to write
+// the golden fixtures under tests/golden/.  Built by oracle/Makefile from the
+// reference sources where they lie; the output stays in oracle/_ref/.
+//
+// TEST INFRASTRUCTURE ONLY: nothing in the product loads this library.
+
+#include "goetia/dbg.hh"
+#include "goetia/storage/bitstorage.hh"
+#include "goetia/storage/bytestorage.hh"
+#include "goetia/storage/sparseppstorage.hh"
+#include "goetia/hashing/hashshifter.hh"
+#include "goetia/hashing/kmeriterator.hh"
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace goetia;
+
+namespace {
+
+template <class G>
+struct Holder {
+    std::shared_ptr<G> g;
+};
+
+struct Handle {
+    int kind;       // 1 bit, 2 byte, 3 sparsepp set
+    int canonical;
+    void* graph;    // Holder<dBG<...>>*
+};
+
+template <class S, class H>
+using Graph = dBG<S, H>;
+
+bool acgt(char c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+
+// Splits a read into the pieces the input policy hands to insert_sequence:
+// mode 0 = the whole read, mode 1 = every maximal A/C/G/T run of length >= K
+// (with its start position).
+std::vector<std::pair<size_t, std::string>> pieces(const char* s, size_t len, int K, int mode) {
+    std::vector<std::pair<size_t, std::string>> out;
+    if (mode == 0) {
+        if (len >= (size_t)K) out.emplace_back(0, std::string(s, len));
+        return out;
+    }
+    size_t i = 0;
+    while (i < len) {
+        while (i < len && !acgt(s[i])) ++i;
+        size_t j = i;
+        while (j < len && acgt(s[j])) ++j;
+        if (j - i >= (size_t)K) out.emplace_back(i, std::string(s + i, j - i));
+        i = j;
+    }
+    return out;
+}
+
+template <class G>
+G* get(Handle* h) { return static_cast<Holder<G>*>(h->graph)->g.get(); }
+
+template <class F>
+auto dispatch(Handle* h, F&& f) {
+    if (h->kind == 1) {
+        if (h->canonical) return f(get<Graph<BitStorage, CanLemireShifter>>(h));
+        return f(get<Graph<BitStorage, FwdLemireShifter>>(h));
+    } else if (h->kind == 2) {
+        if (h->canonical) return f(get<Graph<ByteStorage, CanLemireShifter>>(h));
+        return f(get<Graph<ByteStorage, FwdLemireShifter>>(h));
+    }
+    if (h->canonical) return f(get<Graph<SparseppSetStorage, CanLemireShifter>>(h));
+    return f(get<Graph<SparseppSetStorage, FwdLemireShifter>>(h));
+}
+
+template <class S, class H>
+void* make_graph(std::shared_ptr<S> store, int K) {
+    auto* holder = new Holder<Graph<S, H>>();
+    holder->g = Graph<S, H>::build(store, (uint16_t)K);
+    return holder;
+}
+
+}  // namespace
+
+extern "C" {
+
+// HashShifter::hash_base + KmerIterator over one read (kmeriterator.hh:109-123).
+int ref_hash_read(int K, const char* seq, size_t len, uint64_t* fw, uint64_t* rc) {
+    std::string s(seq, len);
+    if (len < (size_t)K) return 0;
+    KmerIterator<CanLemireShifter> it(s, (uint16_t)K);
+    size_t i = 0;
+    while (!it.done()) {
+        auto h = it.next();
+        fw[i] = h.fw_hash;
+        rc[i] = h.rc_hash;
+        ++i;
+    }
+    return (int)i;
+}
+
+uint64_t ref_fwd_hash(int K, const char* kmer) {
+    return FwdLemireShifter::hash(std::string(kmer, (size_t)K), (uint16_t)K).value();
+}
+
+uint32_t ref_primes(uint32_t n, uint64_t x, uint64_t* out) {
+    auto v = get_n_primes_near_x(n, x);
+    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+    return (uint32_t)v.size();
+}
+
+void* ref_new(int kind, int canonical, int K, uint64_t max_table, uint16_t n_tables) {
+    auto* h = new Handle{kind, canonical, nullptr};
+    if (kind == 1) {
+        auto s = BitStorage::build(max_table, n_tables);
+        h->graph = canonical ? make_graph<BitStorage, CanLemireShifter>(s, K)
+                             : make_graph<BitStorage, FwdLemireShifter>(s, K);
+    } else if (kind == 2) {
+        auto s = ByteStorage::build(max_table, n_tables);
+        h->graph = canonical ? make_graph<ByteStorage, CanLemireShifter>(s, K)
+                             : make_graph<ByteStorage, FwdLemireShifter>(s, K);
+    } else {
+        auto s = SparseppSetStorage::build();
+        h->graph = canonical ? make_graph<SparseppSetStorage, CanLemireShifter>(s, K)
+                             : make_graph<SparseppSetStorage, FwdLemireShifter>(s, K);
+    }
+    return h;
+}
+
+// dBG::insert_sequence(seq, n_new) per read piece, in read order.
+uint64_t ref_insert_reads(void* hv, int K, int mode, const char* seq, const int64_t* off,
+                          uint64_t n_reads, uint64_t* n_new) {
+    Handle* h = (Handle*)hv;
+    uint64_t total = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        uint64_t nn_read = 0;
+        for (auto& p : pieces(seq + off[r], (size_t)(off[r + 1] - off[r]), K, mode)) {
+            uint64_t nn = 0;
+            total += dispatch(h, [&](auto* g) { return g->insert_sequence(p.second, nn); });
+            nn_read += nn;
+        }
+        if (n_new) n_new[r] = nn_read;
+    }
+    return total;
+}
+
+// dBG::insert_and_query_sequence per read (mode 0 only).
+uint64_t ref_insert_and_query_reads(void* hv, int K, const char* seq, const int64_t* off,
+                                    uint64_t n_reads, int16_t* counts) {
+    Handle* h = (Handle*)hv;
+    uint64_t pos = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        size_t len = (size_t)(off[r + 1] - off[r]);
+        if (len < (size_t)K) continue;
+        std::string s(seq + off[r], len);
+        auto c = dispatch(h, [&](auto* g) { return g->insert_and_query_sequence(s); });
+        for (auto v : c) counts[pos++] = v;
+    }
+    return pos;
+}
+
+// dBG::query_sequence per read piece; counts packed len-K+1 per read with -1
+// for skipped windows.
+uint64_t ref_query_reads(void* hv, int K, int mode, const char* seq, const int64_t* off,
+                         uint64_t n_reads, int16_t* counts) {
+    Handle* h = (Handle*)hv;
+    uint64_t pos = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        size_t len = (size_t)(off[r + 1] - off[r]);
+        if (len < (size_t)K) continue;
+        size_t n = len - (size_t)K + 1;
+        for (size_t j = 0; j < n; ++j) counts[pos + j] = -1;
+        for (auto& p : pieces(seq + off[r], len, K, mode)) {
+            auto c = dispatch(h, [&](auto* g) { return g->query_sequence(p.second); });
+            for (size_t j = 0; j < c.size(); ++j) counts[pos + p.first + j] = c[j];
+        }
+        pos += n;
+    }
+    return pos;
+}
+
+int16_t ref_query_hash(void* hv, uint64_t v) {
+    Handle* h = (Handle*)hv;
+    return dispatch(h, [&](auto* g) -> int16_t {
+        using HT = typename std::remove_pointer_t<decltype(g)>::hash_type;
+        HT hh;
+        if constexpr (std::is_same_v<HT, Canonical<uint64_t>>) hh = HT(v, v);
+        else hh = HT(v);
+        return g->query(hh);
+    });
+}
+
+void ref_counters(void* hv, uint64_t* n_unique, uint64_t* n_occupied) {
+    Handle* h = (Handle*)hv;
+    *n_unique = dispatch(h, [&](auto* g) { return g->n_unique(); });
+    *n_occupied = dispatch(h, [&](auto* g) { return g->n_occupied(); });
+}
+
+// Copies table i of get_raw() (bitstorage.hh:228-231 / bytestorage.hh).
+void ref_table(void* hv, uint32_t i, uint64_t nbytes, uint8_t* out) {
+    Handle* h = (Handle*)hv;
+    uint8_t** t = dispatch(h, [&](auto* g) { return g->get_raw(); });
+    std::memcpy(out, t[i], nbytes);
+}
+
+// Left/right extension hashes and degrees of one k-mer via the walker
+// (set_cursor + in_degree/out_degree, pythonize_dbg.py:30-36).
+void ref_kmer_extensions(void* hv, const char* kmer, int K, uint64_t* left, uint64_t* right,
+                         uint8_t* in_deg, uint8_t* out_deg) {
+    Handle* h = (Handle*)hv;
+    dispatch(h, [&](auto* g) {
+        std::string s(kmer, (size_t)K);
+        g->set_cursor(s);
+        auto l = g->left_extensions();
+        auto r = g->right_extensions();
+        for (int c = 0; c < 4; ++c) {
+            left[c] = l[c].value();
+            right[c] = r[c].value();
+        }
+        g->set_cursor(s);
+        *in_deg = (uint8_t)g->in_degree();
+        *out_deg = (uint8_t)g->out_degree();
+        return 0;
+    });
+}
+
+// UnitigWalker::find_decision_kmers (unitig_walker.hh:1025-1045).
+uint64_t ref_find_decision_kmers(void* hv, const char* seq, size_t len, uint32_t* positions,
+                                 uint64_t* hashes, uint64_t cap) {
+    Handle* h = (Handle*)hv;
+    return dispatch(h, [&](auto* g) -> uint64_t {
+        using G = std::remove_pointer_t<decltype(g)>;
+        std::vector<uint32_t> pos;
+        std::vector<typename G::hash_type> hs;
+        std::vector<typename G::neighbor_pair_type> nb;
+        g->find_decision_kmers(std::string(seq, len), pos, hs, nb);
+        for (size_t i = 0; i < pos.size() && i < cap; ++i) {
+            positions[i] = pos[i];
+            hashes[i] = hs[i].value();
+        }
+        return pos.size();
+    });
+}
+
+void ref_free(void* hv) {
+    Handle* h = (Handle*)hv;
+    // graphs are leaked deliberately: the driver lives for one short process.
+    delete h;
+}
+
+}  // extern "C"
