@@ -1,0 +1,188 @@
+"""Benchmark of the streaming dBG hot path on one MI355X.
+
+Workload (the reference's count-min configuration at full size): a seeded
+50 Mbp uniform genome, 20,000,000 reads x 150 bp with 0.5% substitutions and
+0.01% N bases (k-mers spanning N are skipped), canonical k = 31, ByteStorage
+with 4 tables of the primes below 2^28 (1.07 GB of saturating 8-bit
+counters).  One step = reset the tables, insert every read (per-read n_new,
+exact first-touch semantics) and query_sequence every read (per-k-mer
+counts).  Inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line: k-mers/s = hashed k-mers / (insert + query time), the
+roofline of the dominant kernel (algorithmic bytes per k-mer over its
+HIP-event-timed duration), the plain chunk-at-a-time latency and the CPU
+oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic HBM bytes per k-mer: one 64-byte granule per table touched;
+# a read-modify-write counts the granule twice.
+BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=3)
+    p.add_argument('--warmup', type=int, default=1)
+    p.add_argument('--scale', type=float, default=1.0, help='fraction of the 20M reads (testing)')
+    p.add_argument('--chunk-reads', type=int, default=1_000_000)
+    p.add_argument('--cpu-sample-reads', type=int, default=500_000)
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-chunks', action='store_true')
+    return p.parse_args()
+
+
+def kernel_times(N, names):
+    import ctypes as C
+    out = {}
+    for n in names:
+        cnt, ms = C.c_uint64(), C.c_double()
+        N.call('gt_profile_get', n.encode(), C.byref(cnt), C.byref(ms))
+        out[n] = (cnt.value, ms.value)
+    return out
+
+
+def cpu_baseline(cfg, batch, sample_reads):
+    """The oracle (sequential C restatement, 1 thread) on the first
+    `sample_reads` reads: insert then query, k-mers/s."""
+    from oracle import pyoracle as po
+    n = min(sample_reads, batch.n_reads)
+    off = batch.offsets[:n + 1].copy()
+    seq = batch.seq[:off[-1]].copy()
+    st = po.Storage(po.BYTE, po.primes_near_x(cfg['n_tables'], cfg['max_table']))
+    t0 = time.perf_counter()
+    hashed, _ = st.insert_reads(seq, off, cfg['K'], True, po.MODE_SKIP_N)
+    st.query_reads(seq, off, cfg['K'], True, po.MODE_SKIP_N)
+    dt = time.perf_counter() - t0
+    return {'value': hashed / dt, 'unit': 'kmers/s', 'cores': 1, 'kind': 'port',
+            'sample': f'first {n} reads of the workload ({hashed} k-mers), insert + query_sequence, '
+                      f'oracle/oracle.c single-threaded, {dt:.1f} s'}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build targets one GPU; multi-GPU sharding is a design '
+                                   'described in DESIGN.md and is not run', 'n_gpus': args.gpus}))
+        sys.exit(2)
+    import torch
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+
+    cfg = dict(workload.CONFIGS['cms_20m'])
+    t0 = time.perf_counter()
+    _, batch = workload.make('cms_20m', args.scale)
+    log(f'generated {batch.n_reads} reads / {batch.n_bases} bases in {time.perf_counter() - t0:.1f}s')
+    K = cfg['K']
+    dev = torch.device('cuda:0')
+    reads = batch.to_device(dev)
+    nk_slots = int(batch.kmer_counts(K).sum())
+    n_new = torch.zeros(batch.n_reads, dtype=torch.int64, device=dev)
+    counts = torch.empty(nk_slots, dtype=torch.int16, device=dev)
+    store = lg.ByteStorage.build(cfg['max_table'], cfg['n_tables'])
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(store, K)
+    g.mode = N.MODE_SKIP_N
+    stream = torch.cuda.current_stream()
+
+    def step():
+        store.reset()
+        g.insert_device(reads, n_new=n_new, stream=stream)
+        g.query_device(reads, counts, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    n_kmers = int((counts >= 0).sum().item())
+    total_new = int(n_new.sum().item())
+    log(f'{n_kmers} k-mers per step, {total_new} new, n_unique={g.n_unique()}')
+
+    N.call('gt_profile_enable', 1)
+    N.call('gt_profile_reset')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    N.call('gt_profile_enable', 0)
+    kt = kernel_times(N, ['claim', 'apply', 'query'])
+    log('kernel ms/step: ' + ', '.join(f'{k} {v[1] / args.steps:.1f} ({v[0]} launches)' for k, v in kt.items()))
+
+    value = n_kmers * args.steps / elapsed
+    dom = max(kt, key=lambda k: kt[k][1])
+    dom_ms = kt[dom][1] / args.steps
+    achieved = n_kmers * BYTES_PER_KMER[dom] / (dom_ms / 1e3) / 1e9
+    roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
+                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': None,
+                'bytes_per_kmer': BYTES_PER_KMER[dom],
+                'kernel_ms_per_step': {k: round(v[1] / args.steps, 2) for k, v in kt.items()}}
+
+    chunk = None
+    if not args.no_chunks:
+        # Plain path, one chunk of reads at a time: insert + query of a chunk,
+        # synchronous (arrival = call, ready = results on the device).
+        lat = []
+        from goetia_amd.reads import DeviceReads
+        store.reset()
+        cr = args.chunk_reads
+        for r0 in range(0, batch.n_reads, cr):
+            r1 = min(batch.n_reads, r0 + cr)
+            b0, b1 = int(batch.offsets[r0]), int(batch.offsets[r1])
+            sub = DeviceReads(reads.seq[b0:b1], reads.offsets[r0:r1 + 1] - b0, b1 - b0)
+            nk = int(batch.kmer_counts(K)[r0:r1].sum())
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            g.insert_device(sub, n_new=n_new[r0:r1], stream=stream)
+            g.query_device(sub, counts[:nk], stream=stream)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - c0) * 1e3)
+        chunk = {'chunk_reads': cr, 'n_chunks': len(lat), 'p50_ms': round(float(np.percentile(lat, 50)), 2),
+                 'p99_ms': round(float(np.percentile(lat, 99)), 2)}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(cfg, batch, args.cpu_sample_reads)
+
+    out = {
+        'metric': 'canonical k-mers/s insert+query (1 GPU)',
+        'value': round(value, 1),
+        'unit': 'kmers/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(elapsed / args.steps * 1e3, 2),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'u64',
+        'data': 'synthetic (seeded generator, inputs resident in HBM)',
+        'config': {'workload': 'cms_20m: 50 Mbp genome, %d x 150 bp reads, 0.5%% subs, 0.01%% N (skip-N), '
+                               'K=31 canonical, ByteStorage(2^28, 4); step = reset + insert (n_new) + '
+                               'query_sequence' % batch.n_reads,
+                   'reads': batch.n_reads, 'kmers_per_step': n_kmers, 'K': K,
+                   'storage': 'ByteStorage', 'tablesizes': store.get_tablesizes()},
+        'roofline': roofline,
+        'chunk_latency': chunk,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

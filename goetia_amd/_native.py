@@ -112,6 +112,9 @@ SIGNATURES = {
     'gt_query_hashes_dev': [st_p, vp, u64, vp, vp],
     'gt_query_hashes': [st_p, vp, u64, vp],
     'gt_insert_and_query_hashes': [st_p, vp, u64, vp],
+    'gt_profile_enable': [cint],
+    'gt_profile_reset': [],
+    'gt_profile_get': [C.c_char_p, u64p, C.POINTER(C.c_double)],
     'gt_gen_genome': [u64, u64, vp],
     'gt_gen_reads': [u64, vp, u64, u64, vp, u32, u32, C.c_double, C.c_double, vp, vp, u64p],
 }

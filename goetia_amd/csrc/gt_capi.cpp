@@ -115,6 +115,66 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// ------------------------------------------------------------ kernel timing
+
+struct ProfRecord {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+struct ProfState {
+    std::mutex mu;
+    std::atomic<bool> on{false};
+    std::vector<ProfRecord> pending;
+    std::vector<std::pair<std::string, std::pair<uint64_t, double>>> totals;
+};
+
+ProfState g_prof;
+
+// Brackets the kernel launches of its scope with events on `s` while timing
+// is enabled.
+struct ProfScope {
+    const char* name;
+    hipStream_t s;
+    hipEvent_t start = nullptr;
+    ProfScope(const char* n, hipStream_t st) : name(n), s(st) {
+        if (!g_prof.on.load(std::memory_order_relaxed)) return;
+        if (hipEventCreate(&start) != hipSuccess || hipEventRecord(start, s) != hipSuccess) start = nullptr;
+    }
+    ~ProfScope() {
+        if (!start) return;
+        hipEvent_t stop = nullptr;
+        if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, s) != hipSuccess) {
+            (void)hipEventDestroy(start);
+            if (stop) (void)hipEventDestroy(stop);
+            return;
+        }
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        g_prof.pending.push_back({name, start, stop});
+    }
+};
+
+void prof_resolve() {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    for (auto& r : g_prof.pending) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(r.stop);
+        (void)hipEventElapsedTime(&ms, r.start, r.stop);
+        bool found = false;
+        for (auto& t : g_prof.totals) {
+            if (t.first == r.name) {
+                t.second.first += 1;
+                t.second.second += ms;
+                found = true;
+            }
+        }
+        if (!found) g_prof.totals.push_back({r.name, {1, (double)ms}});
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    g_prof.pending.clear();
+}
+
 uint64_t next_pow2(uint64_t x) {
     uint64_t p = 1;
     while (p < x) p <<= 1;
@@ -390,7 +450,10 @@ int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags
             if (2 * (st->ecount_ub + nstarts) > st->ecap) {
                 // Bound the growth by this sub-batch's fresh windows.
                 launch_fill_u64(ctr + kFresh, 1, 0, s);
-                launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+                {
+                    ProfScope ps("exact_lookup", s);
+                    launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+                }
                 GT_HIP(hipGetLastError());
                 uint64_t c[kNCounters];
                 GT_TRY(read_counters(st, c, s));
@@ -398,17 +461,30 @@ int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags
                 GT_TRY(exact_reserve(st, c[kFresh], s));
                 st->ecount_ub += c[kFresh];
             } else {
+                ProfScope ps("exact_lookup", s);
                 launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
                 st->ecount_ub += nstarts;
             }
             const ExactSet E = st->exact_view();
-            launch_exact_insert(L, Rs, E, fresh, st->gen, ctr);
-            launch_exact_resolve(L, Rs, E, fresh, st->gen, n_new, new_flags, ctr);
+            {
+                ProfScope ps("exact_insert", s);
+                launch_exact_insert(L, Rs, E, fresh, st->gen, ctr);
+            }
+            {
+                ProfScope ps("exact_resolve", s);
+                launch_exact_resolve(L, Rs, E, fresh, st->gen, n_new, new_flags, ctr);
+            }
         } else {
             const Tables T = st->tables_view();
             const ClaimMap cm = claim_view(st);
-            launch_claim(L, Rs, T, cm, fresh, ctr);
-            launch_apply(L, Rs, T, cm, fresh, n_new, new_flags, ctr);
+            {
+                ProfScope ps("claim", s);
+                launch_claim(L, Rs, T, cm, fresh, ctr);
+            }
+            {
+                ProfScope ps("apply", s);
+                launch_apply(L, Rs, T, cm, fresh, n_new, new_flags, ctr);
+            }
         }
         GT_HIP(hipGetLastError());
     }
@@ -745,7 +821,10 @@ int gt_hash_reads_dev(uint16_t K, int mode, const uint8_t* seq, const int64_t* o
     scratch->kind = kBit;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, 1);
     GT_TRY(prepare_reads(scratch, R, true, s));
-    launch_hash(make_launch(s), R, fw, rc, valid);
+    {
+        ProfScope ps("hash", s);
+        launch_hash(make_launch(s), R, fw, rc, valid);
+    }
     GT_HIP(hipGetLastError());
     GT_HIP(hipStreamSynchronize(s));
     return GT_OK;
@@ -867,7 +946,10 @@ int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const 
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(st, R, true, s));
-    launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
+    {
+        ProfScope ps("query", s);
+        launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
+    }
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -969,7 +1051,10 @@ int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int 
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(st, R, true, s));
-    launch_degrees(make_launch(s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg, out_deg);
+    {
+        ProfScope ps("degrees", s);
+        launch_degrees(make_launch(s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg, out_deg);
+    }
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -1076,6 +1161,35 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
         for (uint64_t i = 0; i < n; ++i) counts[i] = 1;
     }
     return check_kernel_errors(st, s);
+}
+
+// ------------------------------------------------------------ timing
+
+int gt_profile_enable(int on) {
+    g_prof.on.store(on != 0);
+    return GT_OK;
+}
+
+int gt_profile_reset(void) {
+    prof_resolve();
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.totals.clear();
+    return GT_OK;
+}
+
+int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms) {
+    if (!kernel || !launches || !total_ms) return fail(GT_ERR_INVALID_ARG, "null argument");
+    prof_resolve();
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    *launches = 0;
+    *total_ms = 0.0;
+    for (auto& t : g_prof.totals) {
+        if (t.first == kernel) {
+            *launches = t.second.first;
+            *total_ms = t.second.second;
+        }
+    }
+    return GT_OK;
 }
 
 // ------------------------------------------------------------ workload

@@ -212,6 +212,19 @@ int gt_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t*
 int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n,
                                int16_t* counts);
 
+/* ------------------------------------------------------------ timing */
+
+/* Kernel timing (no counterpart in goetia): while enabled, every kernel this
+ * library launches is bracketed by a pair of hipEvents recorded on the
+ * kernel's own stream; gt_profile_get synchronizes and returns, for one
+ * kernel name ("claim", "apply", "exact_lookup", "exact_insert",
+ * "exact_resolve", "query", "hash", "degrees", "seq_insert_query"), the number
+ * of launches and their summed duration in milliseconds since the last
+ * reset. */
+int gt_profile_enable(int on);
+int gt_profile_reset(void);
+int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
+
 /* ------------------------------------------------------------ workload */
 
 /* Seeded synthetic workload (host memory, deterministic for a seed; no
