@@ -30,6 +30,10 @@ HBM_PEAK_GBPS = 8000.0
 # Algorithmic HBM bytes per k-mer: one 64-byte granule per table touched;
 # a read-modify-write counts the granule twice.
 BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64}
+# HBM bytes per k-mer (read + write) measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE on this workload and scaled by a 1 GiB copy (the table and the
+# commands are in profiles/cms_20m_hbm_traffic.md).
+MEASURED_BYTES_PER_KMER = {'claim': 562 + 52, 'apply': 572 + 257, 'query': 513 + 32}
 
 
 def log(*a):
@@ -46,6 +50,8 @@ def parse():
     p.add_argument('--cpu-sample-reads', type=int, default=500_000)
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-chunks', action='store_true')
+    p.add_argument('--copy-calibration', action='store_true',
+                   help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
     return p.parse_args()
 
 
@@ -101,6 +107,12 @@ def main():
     g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(store, K)
     g.mode = N.MODE_SKIP_N
     stream = torch.cuda.current_stream()
+    if args.copy_calibration:
+        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         store.reset()
@@ -110,9 +122,6 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    n_kmers = int((counts >= 0).sum().item())
-    total_new = int(n_new.sum().item())
-    log(f'{n_kmers} k-mers per step, {total_new} new, n_unique={g.n_unique()}')
 
     N.call('gt_profile_enable', 1)
     N.call('gt_profile_reset')
@@ -123,6 +132,9 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     N.call('gt_profile_enable', 0)
+    n_kmers = int((counts >= 0).sum().item())
+    total_new = int(n_new.sum().item())
+    log(f'{n_kmers} k-mers per step, {total_new} new, n_unique={g.n_unique()}')
     kt = kernel_times(N, ['claim', 'apply', 'query'])
     log('kernel ms/step: ' + ', '.join(f'{k} {v[1] / args.steps:.1f} ({v[0]} launches)' for k, v in kt.items()))
 
@@ -130,8 +142,13 @@ def main():
     dom = max(kt, key=lambda k: kt[k][1])
     dom_ms = kt[dom][1] / args.steps
     achieved = n_kmers * BYTES_PER_KMER[dom] / (dom_ms / 1e3) / 1e9
+    launches = max(kt[dom][0] // args.steps, 1)
+    traffic = MEASURED_BYTES_PER_KMER.get(dom)
     roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
-                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': None,
+                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4),
+                'traffic': round(traffic * n_kmers / launches) if traffic else None,
+                'algorithmic_bytes_per_launch': round(BYTES_PER_KMER[dom] * n_kmers / launches),
+                'launches_per_step': launches,
                 'bytes_per_kmer': BYTES_PER_KMER[dom],
                 'kernel_ms_per_step': {k: round(v[1] / args.steps, 2) for k, v in kt.items()}}
 
