@@ -1,0 +1,101 @@
+// C++ facade check: goetia's dBG template API over libgoetia_amd (GPU).
+// Built by tests/test_cpp_facade.py with the host compiler against
+// include/ and goetia_amd/libgoetia_amd.so; prints "FACADE OK" on success.
+#include <cassert>
+#include <cstdio>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "goetia/dbg.hh"
+
+using namespace goetia;
+
+#define CHECK(c)                                                     \
+    do {                                                             \
+        if (!(c)) {                                                  \
+            std::fprintf(stderr, "check failed: %s (line %d)\n", #c, __LINE__); \
+            return 1;                                                \
+        }                                                            \
+    } while (0)
+
+template <class S>
+int run_presence(std::shared_ptr<S> store) {
+    auto g = dBG<S, CanLemireShifter>::build(store, 21);
+    const std::string s = "ACGTTGCAAGGCTTACCGATAGCATTACGGATCCAGT";
+    for (size_t i = 0; i + 21 <= s.size(); ++i) {
+        const std::string k = s.substr(i, 21);
+        CHECK(g->query(k) == 0);
+        CHECK(g->insert(k) == true);
+        CHECK(g->query(k) == 1);
+        CHECK(g->insert(k) == false);
+    }
+    return 0;
+}
+
+int main() {
+    // known-answer vector (goetia tests/test_hashing.py:15-18)
+    CanLemireShifter sh(27);
+    auto h = sh.hash_base("TCACCTGTGTTGTGCTACTTGCGGCGC");
+    CHECK(h.fw_hash == 13194817695400542713ULL);
+    CHECK(h.rc_hash == 4324216031038051805ULL);
+    CHECK(FwdLemireShifter::hash("TCACCTGTGTTGTGCTACTTGCGGCGC", 27).value() == 13194817695400542713ULL);
+
+    // KmerIterator (GPU) == per-k-mer host hashes
+    const std::string seq = "TCACCTGTGTTGTGCTACTTGCGGCGCAAGTCCATGATCGATCGGGTAC";
+    KmerIterator<CanLemireShifter> it(seq, 21);
+    for (size_t i = 0; !it.done(); ++i) CHECK(it.next() == sh.hash(seq.substr(i, 21)));
+
+    CHECK(run_presence(BitStorage::build(100000, 4)) == 0);
+    CHECK(run_presence(SparseppSetStorage::build()) == 0);
+
+    // counting storage: counts after repeated inserts, insert_and_query
+    auto cs = ByteStorage::build(100000, 4);
+    CHECK(cs->get_tablesizes()[0] == 99991);
+    auto g = dBG<ByteStorage, CanLemireShifter>::build(cs, 21);
+    for (int it2 = 0; it2 < 5; ++it2) {
+        auto c = g->query_sequence(seq);
+        for (auto v : c) CHECK(v == it2);
+        g->insert_sequence(seq);
+    }
+    std::vector<CanLemireShifter::hash_type> hashes;
+    std::vector<count_t> counts;
+    g->insert_sequence(seq, hashes, counts);
+    CHECK(hashes.size() == seq.size() - 20);
+    for (auto v : counts) CHECK(v == 6);
+
+    // per-read n_new and batched calls
+    auto g2 = dBG<BitStorage, CanLemireShifter>::build(BitStorage::build(100000, 4), 21);
+    uint64_t n_new = 0;
+    CHECK(g2->insert_sequence(seq, n_new) == seq.size() - 20);
+    CHECK(n_new == g2->n_unique());
+    ReadBatch batch({seq, seq.substr(5), "ACGT"});
+    auto nn = g2->insert_sequences(batch);
+    CHECK(nn.size() == 3 && nn[0] == 0 && nn[1] == 0 && nn[2] == 0);
+    auto q = g2->query_sequences(batch);
+    for (auto v : q) CHECK(v == 1);
+
+    // neighbours on a linear path (test_dbg.py:182-196)
+    auto g3 = dBG<SparseppSetStorage, CanLemireShifter>::build(SparseppSetStorage::build(), 21);
+    const std::string path = seq.substr(0, 23);
+    g3->insert_sequence(path);
+    auto nb = g3->neighbors(path.substr(1, 21));
+    CHECK(nb.first.size() == 1 && nb.second.size() == 1);
+    CHECK(!g3->is_decision_kmer(path.substr(1, 21)));
+
+    // errors
+    bool threw = false;
+    try {
+        g3->insert_sequence("ACGT");
+    } catch (const SequenceLengthException&) {
+        threw = true;
+    }
+    CHECK(threw);
+
+    // clone() is empty; get_raw() matches the device tables' layout
+    auto c2 = g->clone();
+    CHECK(c2->n_unique() == 0);
+    CHECK(g->get_raw() != nullptr);
+    std::printf("FACADE OK\n");
+    return 0;
+}
