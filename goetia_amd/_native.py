@@ -112,6 +112,13 @@ SIGNATURES = {
     'gt_query_hashes_dev': [st_p, vp, u64, vp, vp],
     'gt_query_hashes': [st_p, vp, u64, vp],
     'gt_insert_and_query_hashes': [st_p, vp, u64, vp],
+    'gt_tune_set': [cint, cint, cint],
+    'gt_tune_get': [cint, C.POINTER(cint), C.POINTER(cint)],
+    'gt_autotune': [st_p, u16, cint, cint, vp, vp, u64, u64, C.POINTER(cint), C.POINTER(cint), vp],
+    'gt_pipeline_new': [st_p, u16, cint, cint, u64, u64, C.POINTER(vp)],
+    'gt_pipeline_submit': [vp, vp, vp, u64, vp],
+    'gt_pipeline_flush': [vp],
+    'gt_pipeline_free': [vp],
     'gt_profile_enable': [cint],
     'gt_profile_reset': [],
     'gt_profile_get': [C.c_char_p, u64p, C.POINTER(C.c_double)],

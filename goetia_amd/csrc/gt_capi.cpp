@@ -139,6 +139,8 @@ struct ProfScope {
     hipEvent_t start = nullptr;
     ProfScope(const char* n, hipStream_t st) : name(n), s(st) {
         if (!g_prof.on.load(std::memory_order_relaxed)) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
         if (hipEventCreate(&start) != hipSuccess || hipEventRecord(start, s) != hipSuccess) start = nullptr;
     }
     ~ProfScope() {
@@ -209,7 +211,9 @@ struct gt_storage {
     DevBuf fresh;
     DevBuf kmer_off, tile_read, scan_blocks;
     DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
-    uint32_t gen = 0;
+    uint64_t gen_issued = 0;           // generation bumps enqueued so far (device kGen follows it)
+    uint64_t epoch = 0;                // bumped when device table pointers change (exact-set growth)
+    std::vector<struct gt_pipeline*> pipelines;   // attached pipelines (their claim maps are cleared on wrap)
     std::recursive_mutex mu;
 
     Tables tables_view() const {
@@ -242,6 +246,33 @@ struct gt_storage {
     }
 };
 
+// Batched insert engine (gt_pipeline_*): two slots of pinned + device
+// buffers; each slot replays its own captured hipGraph.
+struct gt_pipeline {
+    struct Slot {
+        uint8_t* h_seq = nullptr;
+        int64_t* h_off = nullptr;
+        int64_t* h_dyn = nullptr;
+        uint64_t* h_nnew = nullptr;
+        DevBuf d_seq, d_off, d_dyn, d_nnew;
+        hipEvent_t copied = nullptr, done = nullptr;
+        hipGraphExec_t exec = nullptr;
+        uint64_t epoch = 0;
+        uint64_t* user_out = nullptr;
+        uint64_t n_reads = 0;
+        bool busy = false;
+    };
+    gt_storage* st = nullptr;
+    uint16_t K = 0;
+    int canonical = 1, mode = 0;
+    uint64_t max_reads = 0, max_bases = 0;
+    hipStream_t compute = nullptr, copy = nullptr;
+    Slot slot[2];
+    int next = 0;
+    DevBuf claim_keys, claim_vals, fresh, tile_read;
+    uint64_t claim_cap = 0;
+};
+
 namespace {
 
 bool valid_storage(const gt_storage* st) { return st && st->kind >= kBit && st->kind <= kExact; }
@@ -269,6 +300,7 @@ int alloc_exact(gt_storage* st, uint64_t cap, hipStream_t s) {
     std::swap(st->estamp.p, stamp.p);
     std::swap(st->estamp.bytes, stamp.bytes);
     st->ecap = cap;
+    st->epoch++;
     return GT_OK;
 }
 
@@ -302,18 +334,19 @@ int check_kernel_errors(gt_storage* st, hipStream_t s) {
 
 // Prepares tile_read (and kmer_off when wanted) for one call.
 int prepare_reads(gt_storage* st, Reads& R, bool want_kmer_off, hipStream_t s) {
-    const int64_t n_tiles = (R.n_bases + kTile - 1) / kTile;
+    const int64_t n_tiles = (R.n_bases + kTileGrain - 1) / kTileGrain;
     GT_TRY(st->tile_read.ensure((size_t)(n_tiles + 1) * sizeof(uint32_t)));
     // tile_read entries are all written for well-formed offsets; zero first so
     // malformed offsets stay in bounds.
     launch_fill_u64(st->tile_read.as<uint64_t>(), (st->tile_read.bytes) / 8, 0, s);
-    launch_tile_read(R.off, R.n_reads, R.n_bases, st->tile_read.as<uint32_t>(), n_tiles, s);
+    launch_tile_read(R.off, R.n_reads, R.n_bases, st->tile_read.as<uint32_t>(), n_tiles, R.dyn, s);
     R.tile_read = st->tile_read.as<uint32_t>();
     R.kmer_off = nullptr;
     if (want_kmer_off) {
         GT_TRY(st->kmer_off.ensure((R.n_reads + 1) * sizeof(int64_t)));
         GT_TRY(st->scan_blocks.ensure((scan_block_count(R.n_reads) + 1) * sizeof(int64_t)));
-        launch_kmer_offsets(R.off, R.n_reads, R.K, st->kmer_off.as<int64_t>(), st->scan_blocks.as<int64_t>(), s);
+        launch_kmer_offsets(R.off, R.n_reads, R.K, st->kmer_off.as<int64_t>(), st->scan_blocks.as<int64_t>(), R.dyn,
+                            s);
         R.kmer_off = st->kmer_off.as<int64_t>();
     }
     GT_HIP(hipGetLastError());
@@ -353,10 +386,10 @@ uint64_t sub_batch_cap() {
     const char* v = std::getenv("GOETIA_AMD_SUB_BATCH");
     if (!v || !*v) return 0;
     const unsigned long long x = std::strtoull(v, nullptr, 10);
-    return x ? std::max<uint64_t>((x + kTile - 1) / kTile * kTile, kTile) : 0;
+    return x ? std::max<uint64_t>((x + kTileMax - 1) / kTileMax * kTileMax, kTileMax) : 0;
 }
 
-// Largest sub-batch (in start positions, a multiple of kTile) whose claims fit
+// Largest sub-batch (in start positions, a multiple of kTileMax) whose claims fit
 // the claim map at load factor <= 1/2; grows the claim map and fresh buffer
 // as needed for this call.
 int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t s) {
@@ -366,13 +399,13 @@ int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t
     const uint64_t cap = sub_batch_cap();
     if (st->kind == kExact) {
         uint64_t sb = std::min<uint64_t>(n_starts, 1ULL << 31);
-        sb = (sb + kTile - 1) / kTile * kTile;
+        sb = (sb + kTileMax - 1) / kTileMax * kTileMax;
         if (cap) sb = std::min(sb, cap);
         GT_TRY(st->fresh.ensure(sb));
         *sub = sb;
         return GT_OK;
     }
-    uint64_t want = std::min(kMaxClaimCap, std::max(kMinClaimCap, next_pow2(2 * per * (n_starts + kTile))));
+    uint64_t want = std::min(kMaxClaimCap, std::max(kMinClaimCap, next_pow2(2 * per * (n_starts + kTileMax))));
     if (want > st->claim_cap) {
         GT_TRY(st->claim_keys.ensure(want * sizeof(uint64_t)));
         GT_TRY(st->claim_vals.ensure(want * sizeof(uint64_t)));
@@ -382,27 +415,34 @@ int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t
         st->claim_cap = want;
     }
     uint64_t sb = st->claim_cap / (2 * per);
-    sb = std::min<uint64_t>(sb, 1ULL << 31) / kTile * kTile;
+    sb = std::min<uint64_t>(sb, 1ULL << 31) / kTileMax * kTileMax;
     if (cap) sb = std::min(sb, cap);
     if (sb == 0) return fail(GT_ERR_INTERNAL, "claim map too small");
-    GT_TRY(st->fresh.ensure(std::min<uint64_t>(sb, (n_starts + kTile - 1) / kTile * kTile)));
+    GT_TRY(st->fresh.ensure(std::min<uint64_t>(sb, (n_starts + kTileMax - 1) / kTileMax * kTileMax)));
     *sub = sb;
     return GT_OK;
 }
 
-// Next batch generation; the 24-bit key field of the claim map never wraps
-// onto live entries because the map is cleared when it would.
+// Enqueues the next first-touch generation.  Claim-map keys keep 28 bits of
+// it, so the map is cleared whenever that field would wrap, and generation
+// values that are 0 in those bits are skipped (cleared keys read as gen 0).
 int next_gen(gt_storage* st, hipStream_t s) {
-    st->gen++;
-    if ((st->gen & 0xFFFFFFu) == 0) {
+    if (st->gen_issued + 2 >= 0xFFFFFFFFULL) return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
+    st->gen_issued++;
+    launch_bump_gen(st->counters.as<uint64_t>(), s);
+    if ((st->gen_issued & kGenKeyMask) == 0) {
         if (st->claim_cap) {
             launch_fill_u64(st->claim_keys.as<uint64_t>(), st->claim_cap, 0, s);
             launch_fill_u64(st->claim_vals.as<uint64_t>(), st->claim_cap, 0, s);
         }
-        st->gen++;
-        GT_HIP(hipGetLastError());
+        for (auto* p : st->pipelines) {
+            launch_fill_u64(p->claim_keys.as<uint64_t>(), p->claim_cap, 0, s);
+            launch_fill_u64(p->claim_vals.as<uint64_t>(), p->claim_cap, 0, s);
+        }
+        st->gen_issued++;
+        launch_bump_gen(st->counters.as<uint64_t>(), s);
     }
-    if (st->gen == 0xFFFFFFFFu) return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
+    GT_HIP(hipGetLastError());
     return GT_OK;
 }
 
@@ -412,7 +452,6 @@ ClaimMap claim_view(const gt_storage* st) {
     cm.vals = st->claim_vals.as<uint64_t>();
     cm.cap = st->claim_cap;
     cm.log2cap = log2u(st->claim_cap);
-    cm.gen = st->gen;
     return cm;
 }
 
@@ -426,7 +465,78 @@ int exact_reserve(gt_storage* st, uint64_t more, hipStream_t s) {
     return alloc_exact(st, next_pow2(2 * (st->ecount_ub + more)), s);
 }
 
-Launch make_launch(hipStream_t s) { return Launch{g_dev.grid, s}; }
+// Tile shape per storage kind (index 0: hashing without a storage):
+// windows per thread and persistent workgroups per CU.
+struct Tune {
+    int per = 16;
+    int blocks_per_cu = 8;
+};
+
+std::mutex g_tune_mu;
+Tune g_tune[4];
+
+Launch make_launch(int kind, hipStream_t s) {
+    Tune t;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        t = g_tune[(kind >= 0 && kind < 4) ? kind : 0];
+    }
+    return Launch{std::max(1, g_dev.cus * t.blocks_per_cu), s, t.per};
+}
+
+Launch make_launch(hipStream_t s) { return make_launch(0, s); }
+Launch make_launch(const gt_storage* st, hipStream_t s) { return make_launch(st->kind, s); }
+
+// Device work of one first-touch sub-batch [Rs.b0, Rs.b1) with the given
+// claim map and fresh buffer.  For the exact set, `growth` allows a
+// synchronising capacity check (never while a graph is being captured).
+int enqueue_subbatch(gt_storage* st, const Launch& L, const Reads& Rs, const ClaimMap& cm, uint8_t* fresh,
+                     uint64_t* n_new, uint8_t* new_flags, hipStream_t s, bool growth) {
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    GT_TRY(next_gen(st, s));
+    if (st->kind == kExact) {
+        const uint64_t nstarts = (uint64_t)(Rs.b1 - Rs.b0);
+        if (growth && 2 * (st->ecount_ub + nstarts) > st->ecap) {
+            // Bound the growth by this sub-batch's fresh windows.
+            launch_fill_u64(ctr + kFresh, 1, 0, s);
+            {
+                ProfScope ps("exact_lookup", s);
+                launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+            }
+            GT_HIP(hipGetLastError());
+            uint64_t c[kNCounters];
+            GT_TRY(read_counters(st, c, s));
+            st->ecount_ub = c[kNUnique];
+            GT_TRY(exact_reserve(st, c[kFresh], s));
+            st->ecount_ub += c[kFresh];
+        } else {
+            ProfScope ps("exact_lookup", s);
+            launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
+            st->ecount_ub += nstarts;
+        }
+        const ExactSet E = st->exact_view();
+        {
+            ProfScope ps("exact_insert", s);
+            launch_exact_insert(L, Rs, E, fresh, ctr);
+        }
+        {
+            ProfScope ps("exact_resolve", s);
+            launch_exact_resolve(L, Rs, E, fresh, n_new, new_flags, ctr);
+        }
+    } else {
+        const Tables T = st->tables_view();
+        {
+            ProfScope ps("claim", s);
+            launch_claim(L, Rs, T, cm, fresh, ctr);
+        }
+        {
+            ProfScope ps("apply", s);
+            launch_apply(L, Rs, T, cm, fresh, n_new, new_flags, ctr);
+        }
+    }
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
 
 int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
     if (R.n_reads == 0 || R.n_bases == 0) {
@@ -437,63 +547,19 @@ int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags
     if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
     uint64_t sub = 0;
     GT_TRY(prepare_claims(st, (uint64_t)R.n_bases, &sub, s));
-    const Launch L = make_launch(s);
-    uint64_t* ctr = st->counters.as<uint64_t>();
-    uint8_t* fresh = st->fresh.as<uint8_t>();
+    const Launch L = make_launch(st, s);
     for (int64_t b0 = 0; b0 < R.n_bases; b0 += (int64_t)sub) {
         Reads Rs = R;
         Rs.b0 = b0;
         Rs.b1 = std::min<int64_t>(R.n_bases, b0 + (int64_t)sub);
-        GT_TRY(next_gen(st, s));
-        if (st->kind == kExact) {
-            const uint64_t nstarts = (uint64_t)(Rs.b1 - Rs.b0);
-            if (2 * (st->ecount_ub + nstarts) > st->ecap) {
-                // Bound the growth by this sub-batch's fresh windows.
-                launch_fill_u64(ctr + kFresh, 1, 0, s);
-                {
-                    ProfScope ps("exact_lookup", s);
-                    launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
-                }
-                GT_HIP(hipGetLastError());
-                uint64_t c[kNCounters];
-                GT_TRY(read_counters(st, c, s));
-                st->ecount_ub = c[kNUnique];
-                GT_TRY(exact_reserve(st, c[kFresh], s));
-                st->ecount_ub += c[kFresh];
-            } else {
-                ProfScope ps("exact_lookup", s);
-                launch_exact_lookup(L, Rs, st->exact_view(), fresh, ctr);
-                st->ecount_ub += nstarts;
-            }
-            const ExactSet E = st->exact_view();
-            {
-                ProfScope ps("exact_insert", s);
-                launch_exact_insert(L, Rs, E, fresh, st->gen, ctr);
-            }
-            {
-                ProfScope ps("exact_resolve", s);
-                launch_exact_resolve(L, Rs, E, fresh, st->gen, n_new, new_flags, ctr);
-            }
-        } else {
-            const Tables T = st->tables_view();
-            const ClaimMap cm = claim_view(st);
-            {
-                ProfScope ps("claim", s);
-                launch_claim(L, Rs, T, cm, fresh, ctr);
-            }
-            {
-                ProfScope ps("apply", s);
-                launch_apply(L, Rs, T, cm, fresh, n_new, new_flags, ctr);
-            }
-        }
-        GT_HIP(hipGetLastError());
+        GT_TRY(enqueue_subbatch(st, L, Rs, claim_view(st), st->fresh.as<uint8_t>(), n_new, new_flags, s, true));
     }
     return GT_OK;
 }
 
 int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* is_new, hipStream_t s) {
     if (n == 0) return GT_OK;
-    const Launch L = make_launch(s);
+    const Launch L = make_launch(st, s);
     uint64_t* ctr = st->counters.as<uint64_t>();
     uint64_t sub = 0;
     GT_TRY(prepare_claims(st, n, &sub, s));
@@ -506,8 +572,8 @@ int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* i
             st->ecount_ub += m;
             const ExactSet E = st->exact_view();
             launch_hash_exact_lookup(L, h + i0, m, E, fresh, ctr);
-            launch_hash_exact_insert(L, h + i0, m, E, fresh, st->gen, ctr);
-            launch_hash_exact_resolve(L, h + i0, m, E, fresh, st->gen, is_new ? is_new + i0 : nullptr, ctr);
+            launch_hash_exact_insert(L, h + i0, m, E, fresh, ctr);
+            launch_hash_exact_resolve(L, h + i0, m, E, fresh, is_new ? is_new + i0 : nullptr, ctr);
         } else {
             const Tables T = st->tables_view();
             const ClaimMap cm = claim_view(st);
@@ -613,8 +679,8 @@ int gt_storage_new(int kind, const uint64_t* tablesizes, uint32_t n_tables, uint
             return fail(GT_ERR_INVALID_ARG, "n_tables must be in 1..16");
         if (!tablesizes) return fail(GT_ERR_INVALID_ARG, "null tablesizes");
         for (uint32_t i = 0; i < n_tables; ++i) {
-            if (tablesizes[i] == 0 || tablesizes[i] >= (1ULL << 36))
-                return fail(GT_ERR_INVALID_ARG, "table sizes must be in 1..2^36-1");
+            if (tablesizes[i] == 0 || tablesizes[i] >= kMaxBins)
+                return fail(GT_ERR_INVALID_ARG, "table sizes must be in 1..2^32-1");
         }
     }
     GT_TRY(ensure_device());
@@ -657,6 +723,7 @@ int gt_storage_free(gt_storage* st) {
     if (!st) return GT_OK;
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     (void)hipDeviceSynchronize();
+    for (auto* p : st->pipelines) p->st = nullptr;   // their later calls fail cleanly
     delete st;
     return GT_OK;
 }
@@ -801,7 +868,7 @@ int gt_kmer_offsets_dev(uint16_t K, const int64_t* offsets, uint64_t n_reads, in
     hipStream_t s = (hipStream_t)stream;
     DevBuf blocks;
     GT_TRY(blocks.ensure((scan_block_count(n_reads) + 1) * sizeof(int64_t)));
-    launch_kmer_offsets(offsets, n_reads, K, kmer_offsets, blocks.as<int64_t>(), s);
+    launch_kmer_offsets(offsets, n_reads, K, kmer_offsets, blocks.as<int64_t>(), nullptr, s);
     GT_HIP(hipGetLastError());
     GT_HIP(hipStreamSynchronize(s));   // `blocks` is released on return
     return GT_OK;
@@ -948,7 +1015,7 @@ int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const 
     GT_TRY(prepare_reads(st, R, true, s));
     {
         ProfScope ps("query", s);
-        launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
+        launch_query(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
     }
     GT_HIP(hipGetLastError());
     return GT_OK;
@@ -1000,7 +1067,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         GT_TRY(fw.ensure((size_t)nk * 8));
         GT_TRY(rc.ensure((size_t)nk * 8));
         GT_TRY(valid.ensure((size_t)nk));
-        const Launch L = make_launch(s);
+        const Launch L = make_launch(st, s);
         launch_hash(L, R, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>());
         launch_seq_insert_query(fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), (uint64_t)nk, R.canonical,
                                 st->tables_view(), counts, nullptr, st->counters.as<uint64_t>(), s);
@@ -1013,7 +1080,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
     // sparseppstorage.cc:21-25); a query after the batch gives exactly that
     // (and -1 for skipped windows).
     GT_TRY(prepare_reads(st, R, true, s));
-    launch_query(make_launch(s), R, st->tables_view(), st->exact_or_empty(), counts, nullptr);
+    launch_query(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), counts, nullptr);
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -1053,7 +1120,8 @@ int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int 
     GT_TRY(prepare_reads(st, R, true, s));
     {
         ProfScope ps("degrees", s);
-        launch_degrees(make_launch(s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg, out_deg);
+        launch_degrees(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg,
+                       out_deg);
     }
     GT_HIP(hipGetLastError());
     return GT_OK;
@@ -1116,7 +1184,7 @@ int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int1
     if (n == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
     hipStream_t s = (hipStream_t)stream;
-    launch_hash_query(make_launch(s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
+    launch_hash_query(make_launch(st, s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -1161,6 +1229,318 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
         for (uint64_t i = 0; i < n; ++i) counts[i] = 1;
     }
     return check_kernel_errors(st, s);
+}
+
+// ------------------------------------------------------------ tuning
+
+int gt_tune_set(int kind, int per, int blocks_per_cu) {
+    if (kind < 0 || kind > 3) return fail(GT_ERR_INVALID_ARG, "kind must be 0..3");
+    if (per != 8 && per != 16 && per != 32) return fail(GT_ERR_INVALID_ARG, "per must be 8, 16 or 32");
+    if (blocks_per_cu < 1 || blocks_per_cu > 32) return fail(GT_ERR_INVALID_ARG, "blocks_per_cu must be 1..32");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[kind].per = per;
+    g_tune[kind].blocks_per_cu = blocks_per_cu;
+    return GT_OK;
+}
+
+int gt_tune_get(int kind, int* per, int* blocks_per_cu) {
+    if (kind < 0 || kind > 3 || !per || !blocks_per_cu) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    *per = g_tune[kind].per;
+    *blocks_per_cu = g_tune[kind].blocks_per_cu;
+    return GT_OK;
+}
+
+int gt_autotune(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                uint64_t n_reads, uint64_t n_bases, int* per_out, int* bpc_out, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return fail(GT_ERR_INVALID_ARG, "autotune needs a non-empty batch");
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    GT_TRY(prepare_reads(st, R, true, s));
+    uint64_t sub = 0;
+    GT_TRY(prepare_claims(st, n_bases, &sub, s));
+    R.b1 = std::min<int64_t>(R.n_bases, (int64_t)sub);
+    DevBuf counts;
+    GT_TRY(counts.ensure((size_t)std::max<uint64_t>(n_bases, 1) * sizeof(int16_t)));
+    hipEvent_t e0, e1;
+    GT_HIP(hipEventCreate(&e0));
+    GT_HIP(hipEventCreate(&e1));
+    const int pers[3] = {8, 16, 32};
+    const int bpcs[3] = {4, 8, 16};
+    float best = 1e30f;
+    Tune best_t;
+    Tune saved;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        saved = g_tune[st->kind];
+    }
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    for (int pi = 0; pi < 3; ++pi) {
+        for (int bi = 0; bi < 3; ++bi) {
+            {
+                std::lock_guard<std::mutex> g(g_tune_mu);
+                g_tune[st->kind].per = pers[pi];
+                g_tune[st->kind].blocks_per_cu = bpcs[bi];
+            }
+            const Launch L = make_launch(st, s);
+            float total = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                GT_HIP(hipEventRecord(e0, s));
+                // read-only passes: the claim (or lookup) pass writes only
+                // scratch; a later insert starts a new generation first.
+                if (st->kind == kExact) {
+                    launch_exact_lookup(L, R, st->exact_view(), st->fresh.as<uint8_t>(), ctr);
+                } else {
+                    launch_claim(L, R, st->tables_view(), claim_view(st), st->fresh.as<uint8_t>(), ctr);
+                }
+                launch_query(L, R, st->tables_view(), st->exact_or_empty(), counts.as<int16_t>(), nullptr);
+                GT_HIP(hipEventRecord(e1, s));
+                GT_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                GT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep) total += ms;   // the first repetition warms up
+            }
+            if (total < best) {
+                best = total;
+                best_t = Tune{pers[pi], bpcs[bi]};
+            }
+        }
+    }
+    (void)saved;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[st->kind] = best_t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (per_out) *per_out = best_t.per;
+    if (bpc_out) *bpc_out = best_t.blocks_per_cu;
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ pipeline
+
+namespace {
+
+void pipeline_release(gt_pipeline* p) {
+    for (auto& sl : p->slot) {
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.copied) (void)hipEventDestroy(sl.copied);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.h_seq) (void)hipHostFree(sl.h_seq);
+        if (sl.h_off) (void)hipHostFree(sl.h_off);
+        if (sl.h_dyn) (void)hipHostFree(sl.h_dyn);
+        if (sl.h_nnew) (void)hipHostFree(sl.h_nnew);
+        sl = gt_pipeline::Slot();
+    }
+    if (p->compute) (void)hipStreamDestroy(p->compute);
+    if (p->copy) (void)hipStreamDestroy(p->copy);
+    p->compute = p->copy = nullptr;
+}
+
+// Enqueues (while the compute stream is being captured) the device work of
+// one batch of slot `sl`.
+int pipeline_enqueue(gt_pipeline* p, gt_pipeline::Slot& sl) {
+    gt_storage* st = p->st;
+    hipStream_t s = p->compute;
+    Reads R = make_reads(sl.d_seq.as<uint8_t>(), sl.d_off.as<int64_t>(), p->max_reads, p->max_bases, p->K, p->mode,
+                         p->canonical);
+    R.dyn = sl.d_dyn.as<int64_t>();
+    const int64_t n_grains = ((int64_t)p->max_bases + kTileGrain - 1) / kTileGrain;
+    launch_fill_u64(p->tile_read.as<uint64_t>(), p->tile_read.bytes / 8, 0, s);
+    launch_tile_read(R.off, R.n_reads, R.n_bases, p->tile_read.as<uint32_t>(), n_grains, R.dyn, s);
+    R.tile_read = p->tile_read.as<uint32_t>();
+    launch_fill_u64(sl.d_nnew.as<uint64_t>(), p->max_reads, 0, s);
+    ClaimMap cm{};
+    cm.keys = p->claim_keys.as<uint64_t>();
+    cm.vals = p->claim_vals.as<uint64_t>();
+    cm.cap = p->claim_cap;
+    cm.log2cap = log2u(p->claim_cap);
+    return enqueue_subbatch(st, make_launch(st, s), R, cm, p->fresh.as<uint8_t>(), sl.d_nnew.as<uint64_t>(), nullptr,
+                            s, false);
+}
+
+int pipeline_capture(gt_pipeline* p, gt_pipeline::Slot& sl) {
+    if (sl.exec) {
+        (void)hipGraphExecDestroy(sl.exec);
+        sl.exec = nullptr;
+    }
+    const uint64_t gen_before = p->st->gen_issued, ub_before = p->st->ecount_ub;
+    GT_HIP(hipStreamBeginCapture(p->compute, hipStreamCaptureModeThreadLocal));
+    int rc = pipeline_enqueue(p, sl);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(p->compute, &graph);
+    // Capturing ran nothing: the generation bump and the set growth happen
+    // on replay and are counted there.
+    p->st->gen_issued = gen_before;
+    p->st->ecount_ub = ub_before;
+    if (rc != GT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(GT_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&sl.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(GT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    sl.epoch = p->st->epoch;
+    return GT_OK;
+}
+
+int pipeline_deliver(gt_pipeline::Slot& sl) {
+    if (!sl.busy) return GT_OK;
+    GT_HIP(hipEventSynchronize(sl.done));
+    if (sl.user_out) std::memcpy(sl.user_out, sl.h_nnew, sl.n_reads * sizeof(uint64_t));
+    sl.busy = false;
+    sl.user_out = nullptr;
+    return GT_OK;
+}
+
+}  // namespace
+
+int gt_pipeline_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_t max_reads, uint64_t max_bases,
+                    gt_pipeline** out) {
+    if (!out) return fail(GT_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (K < 1 || K >= kMaxK) return fail(GT_ERR_INVALID_ARG, "K must be in 1..255");
+    if (mode != kRaw && mode != kSkipN) return fail(GT_ERR_INVALID_ARG, "unknown input mode");
+    if (max_reads == 0 || max_bases == 0 || max_reads >= 0xFFFFFFFFULL)
+        return fail(GT_ERR_INVALID_ARG, "invalid batch bounds");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    const uint64_t per = std::max<uint32_t>(st->n_tables, 1);
+    const uint64_t cap = next_pow2(2 * per * (max_bases + kTileMax));
+    if (st->kind != kExact && cap > (1ULL << 29))
+        return fail(GT_ERR_INVALID_ARG, "max_bases exceeds one first-touch sub-batch");
+    if (max_bases >= (1ULL << 31)) return fail(GT_ERR_INVALID_ARG, "max_bases must be below 2^31");
+    gt_pipeline* p = new (std::nothrow) gt_pipeline();
+    if (!p) return fail(GT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    p->st = st;
+    p->K = K;
+    p->canonical = canonical ? 1 : 0;
+    p->mode = mode;
+    p->max_reads = max_reads;
+    p->max_bases = max_bases;
+    int rc = GT_OK;
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == GT_OK) rc = fail(GT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return rc == GT_OK;
+    };
+    hip_ok(hipStreamCreateWithFlags(&p->compute, hipStreamNonBlocking), "hipStreamCreate");
+    hip_ok(hipStreamCreateWithFlags(&p->copy, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& sl : p->slot) {
+        if (rc != GT_OK) break;
+        hip_ok(hipHostMalloc((void**)&sl.h_seq, max_bases), "hipHostMalloc");
+        hip_ok(hipHostMalloc((void**)&sl.h_off, (max_reads + 1) * sizeof(int64_t)), "hipHostMalloc");
+        hip_ok(hipHostMalloc((void**)&sl.h_dyn, 2 * sizeof(int64_t)), "hipHostMalloc");
+        hip_ok(hipHostMalloc((void**)&sl.h_nnew, max_reads * sizeof(uint64_t)), "hipHostMalloc");
+        hip_ok(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming), "hipEventCreate");
+        hip_ok(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+        if (rc == GT_OK) rc = sl.d_seq.ensure(max_bases);
+        if (rc == GT_OK) rc = sl.d_off.ensure((max_reads + 1) * sizeof(int64_t));
+        if (rc == GT_OK) rc = sl.d_dyn.ensure(2 * sizeof(int64_t));
+        if (rc == GT_OK) rc = sl.d_nnew.ensure(max_reads * sizeof(uint64_t));
+    }
+    if (rc == GT_OK) rc = p->tile_read.ensure(((max_bases + kTileGrain - 1) / kTileGrain + 1) * sizeof(uint32_t));
+    if (rc == GT_OK) rc = p->fresh.ensure((max_bases + kTileMax - 1) / kTileMax * kTileMax);
+    if (rc == GT_OK && st->kind != kExact) {
+        p->claim_cap = cap;
+        rc = p->claim_keys.ensure(cap * sizeof(uint64_t));
+        if (rc == GT_OK) rc = p->claim_vals.ensure(cap * sizeof(uint64_t));
+        if (rc == GT_OK) {
+            launch_fill_u64(p->claim_keys.as<uint64_t>(), cap, 0, p->compute);
+            launch_fill_u64(p->claim_vals.as<uint64_t>(), cap, 0, p->compute);
+        }
+    } else if (rc == GT_OK) {
+        p->claim_cap = 2;   // unused by the exact set; a valid view
+        rc = p->claim_keys.ensure(16);
+        if (rc == GT_OK) rc = p->claim_vals.ensure(16);
+    }
+    if (rc == GT_OK) hip_ok(hipStreamSynchronize(p->compute), "hipStreamSynchronize");
+    if (rc != GT_OK) {
+        pipeline_release(p);
+        delete p;
+        return rc;
+    }
+    st->pipelines.push_back(p);
+    *out = p;
+    return GT_OK;
+}
+
+int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads, uint64_t* n_new) {
+    if (!p || !p->st) return fail(GT_ERR_INVALID_ARG, "invalid pipeline");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    if (n_reads > p->max_reads || n_bases > p->max_bases) return fail(GT_ERR_INVALID_ARG, "batch exceeds the pipeline bounds");
+    if (n_bases && !seq) return fail(GT_ERR_INVALID_ARG, "null reads");
+    GT_TRY(ensure_device());
+    gt_storage* st = p->st;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    gt_pipeline::Slot& sl = p->slot[p->next];
+    p->next ^= 1;
+    GT_TRY(pipeline_deliver(sl));
+    if (n_reads == 0) return GT_OK;
+    if (st->kind == kExact && 2 * (st->ecount_ub + n_bases) > st->ecap) {
+        // Grow the set between batches: finish in-flight work, read the exact
+        // size, rehash; graphs holding the old table are captured again.
+        GT_TRY(pipeline_deliver(p->slot[0]));
+        GT_TRY(pipeline_deliver(p->slot[1]));
+        GT_HIP(hipStreamSynchronize(p->compute));
+        uint64_t c[kNCounters];
+        GT_TRY(read_counters(st, c, p->compute));
+        st->ecount_ub = c[kNUnique];
+        GT_TRY(exact_reserve(st, n_bases, p->compute));
+    }
+    std::memcpy(sl.h_seq, seq, n_bases);
+    std::memcpy(sl.h_off, offsets, (n_reads + 1) * sizeof(int64_t));
+    sl.h_dyn[0] = (int64_t)n_reads;
+    sl.h_dyn[1] = (int64_t)n_bases;
+    GT_HIP(hipMemcpyAsync(sl.d_seq.p, sl.h_seq, n_bases, hipMemcpyHostToDevice, p->copy));
+    GT_HIP(hipMemcpyAsync(sl.d_off.p, sl.h_off, (n_reads + 1) * sizeof(int64_t), hipMemcpyHostToDevice, p->copy));
+    GT_HIP(hipMemcpyAsync(sl.d_dyn.p, sl.h_dyn, 2 * sizeof(int64_t), hipMemcpyHostToDevice, p->copy));
+    GT_HIP(hipEventRecord(sl.copied, p->copy));
+    GT_HIP(hipStreamWaitEvent(p->compute, sl.copied, 0));
+    // The replay bumps the generation once; skip the values that would wrap
+    // the claim-map key field (next_gen clears the maps there).
+    if (((st->gen_issued + 1) & kGenKeyMask) == 0) GT_TRY(next_gen(st, p->compute));
+    if (!sl.exec || sl.epoch != st->epoch) GT_TRY(pipeline_capture(p, sl));
+    st->gen_issued++;
+    if (st->kind == kExact) st->ecount_ub += n_bases;
+    GT_HIP(hipGraphLaunch(sl.exec, p->compute));
+    GT_HIP(hipMemcpyAsync(sl.h_nnew, sl.d_nnew.p, n_reads * sizeof(uint64_t), hipMemcpyDeviceToHost, p->compute));
+    GT_HIP(hipEventRecord(sl.done, p->compute));
+    sl.busy = true;
+    sl.user_out = n_new;
+    sl.n_reads = n_reads;
+    return GT_OK;
+}
+
+int gt_pipeline_flush(gt_pipeline* p) {
+    if (!p || !p->st) return fail(GT_ERR_INVALID_ARG, "invalid pipeline");
+    std::lock_guard<std::recursive_mutex> lk(p->st->mu);
+    // deliver in submission order
+    GT_TRY(pipeline_deliver(p->slot[p->next]));
+    GT_TRY(pipeline_deliver(p->slot[p->next ^ 1]));
+    GT_HIP(hipStreamSynchronize(p->compute));
+    return check_kernel_errors(p->st, p->compute);
+}
+
+int gt_pipeline_free(gt_pipeline* p) {
+    if (!p) return GT_OK;
+    if (p->st) {
+        std::lock_guard<std::recursive_mutex> lk(p->st->mu);
+        (void)pipeline_deliver(p->slot[0]);
+        (void)pipeline_deliver(p->slot[1]);
+        auto& v = p->st->pipelines;
+        v.erase(std::remove(v.begin(), v.end(), p), v.end());
+    }
+    pipeline_release(p);
+    delete p;
+    return GT_OK;
 }
 
 // ------------------------------------------------------------ timing

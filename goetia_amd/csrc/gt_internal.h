@@ -9,21 +9,27 @@ namespace gt {
 
 constexpr int kMaxTables = 16;
 
-// Work decomposition of the k-mer walkers: a tile is kTile consecutive k-mer
-// start positions of the concatenated read bytes; a workgroup of kBlock
-// threads owns one tile at a time and each thread rolls the hash over kPer
-// consecutive starts.
+// Work decomposition of the k-mer walkers: a tile is kBlock * PER consecutive
+// k-mer start positions of the concatenated read bytes; a workgroup of kBlock
+// threads owns one tile at a time and each thread rolls the hash over PER
+// consecutive starts.  PER is 8, 16 or 32 (picked per storage kind by the
+// autotuner); tiles are multiples of kTileGrain, and sub-batch boundaries are
+// multiples of kTileMax.
 constexpr int kBlock = 256;
-constexpr int kPer = 16;
-constexpr int kTile = kBlock * kPer;
+constexpr int kPerMin = 8;
+constexpr int kPerMax = 32;
+constexpr int kTileGrain = kBlock * kPerMin;
+constexpr int kTileMax = kBlock * kPerMax;
 constexpr int kMaxK = 256;          // K <= 255 (goetia's K is a uint16_t)
 constexpr int kReadCache = 1024;    // read offsets cached in LDS per tile
 
 enum Kind : int { kBit = 1, kByte = 2, kExact = 3 };
 enum Mode : int { kRaw = 0, kSkipN = 1 };
 
-// Device counters of one storage (u64 each).
-enum Counter : int { kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kNCounters = 4 };
+// Device counters of one storage (u64 each).  kGen is the first-touch
+// generation: bumped on the device once per sub-batch, so replays of a
+// captured batch never reuse a generation.
+enum Counter : int { kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kGen = 4, kNCounters = 5 };
 
 // Error bits OR-ed into counters[kErrors] by kernels.
 enum ErrBit : uint64_t { kErrProbe = 1, kErrClaim = 2 };
@@ -50,22 +56,27 @@ struct ExactSet {
 };
 
 // First-touch claim map for Bloom / count-min batches: key word =
-// gen:24 | table:4 | bin:36, value word = gen:32 | ~ordinal:32 (atomicMax).
+// gen:28 | table:4 | bin:32, value word = gen:32 | ~ordinal:32 (atomicMax).
 struct ClaimMap {
     uint64_t* keys;
     uint64_t* vals;
     uint64_t cap;
     uint32_t log2cap;
-    uint32_t gen;
 };
 
+constexpr uint64_t kMaxBins = 1ULL << 32;       // bins per table
+constexpr uint64_t kGenKeyMask = (1ULL << 28) - 1;
+
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the
-// ordinal of the k-mer at s is s - b0.
+// ordinal of the k-mer at s is s - b0.  With `dyn` set, n_reads and n_bases
+// are upper bounds and the actual sizes are read from device memory, so one
+// captured launch sequence serves batches of any size up to the bounds.
 struct Reads {
     const uint8_t* seq;
     const int64_t* off;        // n_reads + 1 entries, off[0] = 0
     const int64_t* kmer_off;   // n_reads + 1 entries (may be null if unused)
     const uint32_t* tile_read; // read holding the first base of each tile
+    const int64_t* dyn;        // optional device {n_reads, n_bases}: sizes read at run time
     uint64_t n_reads;
     int64_t n_bases;
     int64_t b0, b1;

@@ -2,7 +2,7 @@
 //
 // All arithmetic is integer.  The k-mer walkers stage a tile of read bytes in
 // LDS with 16-byte loads, roll goetia's canonical cyclic hash
-// (rollinghashshifter.hh:180-219, cyclichash.h:85-108) over kPer consecutive
+// (rollinghashshifter.hh:180-219, cyclichash.h:85-108) over PER consecutive
 // windows per thread, and hand every window to an operation functor:
 //
 //   Bloom / count-min insert  = claim pass + apply pass (first-touch exact)
@@ -55,7 +55,7 @@ __device__ __forceinline__ void add_counter(uint64_t* ctr, uint64_t v) {
 // ------------------------------------------------------------ claim map
 
 __device__ __forceinline__ uint64_t claim_key(uint32_t gen, uint32_t table, uint64_t bin) {
-    return ((uint64_t)(gen & 0xFFFFFFu) << 40) | ((uint64_t)table << 36) | bin;
+    return ((uint64_t)(gen & kGenKeyMask) << 36) | ((uint64_t)table << 32) | bin;
 }
 
 __device__ __forceinline__ uint64_t claim_stamp(uint32_t gen, uint32_t ordinal) {
@@ -63,12 +63,17 @@ __device__ __forceinline__ uint64_t claim_stamp(uint32_t gen, uint32_t ordinal) 
 }
 
 __device__ __forceinline__ uint64_t claim_slot(const ClaimMap& cm, uint64_t key) {
-    return ((key & ((1ULL << 40) - 1)) * kGolden) >> (64 - cm.log2cap);
+    return ((key & ((1ULL << 36) - 1)) * kGolden) >> (64 - cm.log2cap);
+}
+
+// The generation of the running sub-batch (uniform across the grid).
+__device__ __forceinline__ uint32_t current_gen(const uint64_t* counters) {
+    return (uint32_t)__hip_atomic_load(counters + kGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, uint64_t* errors) {
     const uint64_t mask = cm.cap - 1;
-    const uint64_t gen = key >> 40;
+    const uint64_t gen = key >> 36;
     uint64_t slot = claim_slot(cm, key);
     for (uint64_t probes = 0; probes <= cm.cap; ++probes) {
         uint64_t cur = load_relaxed(&cm.keys[slot]);
@@ -76,7 +81,7 @@ __device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, u
             atomicMax((unsigned long long*)&cm.vals[slot], (unsigned long long)stamp);
             return;
         }
-        if ((cur >> 40) != gen) {
+        if ((cur >> 36) != gen) {
             // a slot left by an earlier generation is free
             uint64_t prev = atomicCAS((unsigned long long*)&cm.keys[slot],
                                       (unsigned long long)cur, (unsigned long long)key);
@@ -84,7 +89,7 @@ __device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, u
                 atomicMax((unsigned long long*)&cm.vals[slot], (unsigned long long)stamp);
                 return;
             }
-            if ((prev >> 40) != gen) continue;  // changed but still stale: look again
+            if ((prev >> 36) != gen) continue;  // changed but still stale: look again
         }
         slot = (slot + 1) & mask;
     }
@@ -93,12 +98,12 @@ __device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, u
 
 __device__ uint64_t claim_find(const ClaimMap& cm, uint64_t key) {
     const uint64_t mask = cm.cap - 1;
-    const uint64_t gen = key >> 40;
+    const uint64_t gen = key >> 36;
     uint64_t slot = claim_slot(cm, key);
     for (uint64_t probes = 0; probes <= cm.cap; ++probes) {
         uint64_t cur = cm.keys[slot];
         if (cur == key) return cm.vals[slot];
-        if ((cur >> 40) != gen) return 0;
+        if ((cur >> 36) != gen) return 0;
         slot = (slot + 1) & mask;
     }
     return 0;
@@ -180,7 +185,7 @@ struct TileShared {
     uint64_t tf[256];                 // T[c]
     uint64_t tr[256];                 // T[complement(c)]
     int64_t roff[kReadCache + 1];
-    uint8_t bytes[kTile + kMaxK];
+    uint8_t bytes[kTileMax + kMaxK];
     uint32_t r_lo, r_hi;
     int cached;
 };
@@ -192,21 +197,34 @@ __device__ __forceinline__ void load_char_tables(TileShared& sh) {
     }
 }
 
-__device__ __forceinline__ int64_t tile_count(const Reads& R) {
-    return (R.n_bases + kTile - 1) / kTile;
+// Actual batch sizes: the bounds in R, or the run-time sizes in R.dyn.
+__device__ __forceinline__ void batch_sizes(const Reads& R, uint64_t& n_reads, int64_t& n_bases) {
+    n_reads = R.n_reads;
+    n_bases = R.n_bases;
+    if (R.dyn) {
+        const uint64_t dr = (uint64_t)R.dyn[0];
+        const int64_t db = R.dyn[1];
+        n_reads = dr < n_reads ? dr : n_reads;
+        n_bases = db < n_bases ? db : n_bases;
+    }
 }
 
 // Visits every k-mer window whose start lies in [R.b0, R.b1).  For windows
 // inside a read, op(s, read, read_start, fw, rc, valid) is called in start
 // order per thread; valid is false for skip-N windows holding a non-ACGT byte.
-template <class Op>
+template <int PER, class Op>
 __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
+    constexpr int kTile = kBlock * PER;
     const int tid = threadIdx.x;
     const int K = R.K;
     const unsigned Kr = (unsigned)K & 63u;
-    const int64_t n_tiles = tile_count(R);
+    uint64_t n_reads;
+    int64_t n_bases;
+    batch_sizes(R, n_reads, n_bases);
+    const int64_t b1 = R.b1 < n_bases ? R.b1 : n_bases;
+    const int64_t n_grains = (n_bases + kTileGrain - 1) / kTileGrain;
     const int64_t tile_first = R.b0 / kTile;
-    const int64_t tile_end = (R.b1 + kTile - 1) / kTile;
+    const int64_t tile_end = n_reads ? (b1 + kTile - 1) / kTile : 0;
     const bool aligned16 = ((uintptr_t)R.seq & 15u) == 0;
 
     load_char_tables(sh);
@@ -214,7 +232,7 @@ __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
         __syncthreads();
         const int64_t t0 = tile * kTile;
         const int span = kTile + K - 1;
-        const int64_t avail = R.n_bases - t0;
+        const int64_t avail = n_bases - t0;
         const int nload = (int)(avail < span ? avail : span);
         if (aligned16) {
             const int nvec = nload >> 4;
@@ -226,10 +244,11 @@ __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
             for (int i = tid; i < span; i += kBlock) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
         }
         if (tid == 0) {
-            uint32_t rl = R.tile_read[tile];
-            uint32_t rh = (tile + 1 < n_tiles) ? R.tile_read[tile + 1] : (uint32_t)(R.n_reads - 1);
-            if (rl >= R.n_reads) rl = (uint32_t)(R.n_reads - 1);
-            if (rh >= R.n_reads) rh = (uint32_t)(R.n_reads - 1);
+            const int64_t g_end = (t0 + kTile) / kTileGrain;
+            uint32_t rl = R.tile_read[t0 / kTileGrain];
+            uint32_t rh = g_end < n_grains ? R.tile_read[g_end] : (uint32_t)(n_reads - 1);
+            if (rl >= n_reads) rl = (uint32_t)(n_reads - 1);
+            if (rh >= n_reads) rh = (uint32_t)(n_reads - 1);
             if (rh < rl) rh = rl;
             sh.r_lo = rl;
             sh.r_hi = rh;
@@ -244,9 +263,9 @@ __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
         __syncthreads();
 
         op.begin_tile(t0);
-        const int64_t s0 = t0 + (int64_t)tid * kPer;
-        int64_t s_end = s0 + kPer;
-        if (s_end > R.b1) s_end = R.b1;
+        const int64_t s0 = t0 + (int64_t)tid * PER;
+        int64_t s_end = s0 + PER;
+        if (s_end > b1) s_end = b1;
         if (s0 >= s_end) continue;
 
         auto OFF = [&](uint32_t r) -> int64_t { return cached ? sh.roff[r - rl] : R.off[r]; };
@@ -270,7 +289,7 @@ __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
         }
         for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
 
-        for (int j = 0; j < kPer; ++j) {
+        for (int j = 0; j < PER; ++j) {
             const int64_t s = s0 + j;
             if (s >= s_end) break;
             if (j > 0) {
@@ -323,6 +342,7 @@ struct ReadAcc {
 struct ClaimOp {
     Tables T;
     ClaimMap cm;
+    uint32_t gen;
     uint8_t* fresh;
     int64_t b0;
     int canonical;
@@ -339,7 +359,7 @@ struct ClaimOp {
             const bool empty = T.kind == kBit ? !bit_get(T.base[i], b) : T.base[i][b] == 0;
             if (empty) {
                 mask |= (uint8_t)(1u << (i < 7 ? i : 7));
-                claim_insert(cm, claim_key(cm.gen, i, b), claim_stamp(cm.gen, ord), errors);
+                claim_insert(cm, claim_key(gen, i, b), claim_stamp(gen, ord), errors);
             }
         }
         fresh[s - b0] = mask;
@@ -370,6 +390,7 @@ __device__ __forceinline__ uint32_t byte_sat_inc(uint8_t* t, uint64_t b) {
 struct ApplyOp {
     Tables T;
     ClaimMap cm;
+    uint32_t gen;
     const uint8_t* fresh;
     int64_t b0;
     int canonical;
@@ -385,7 +406,7 @@ struct ApplyOp {
         }
         const uint64_t h = kmer_value(fw, rc, canonical);
         const uint8_t mask = fresh[s - b0];
-        const uint64_t stamp = claim_stamp(cm.gen, (uint32_t)(s - b0));
+        const uint64_t stamp = claim_stamp(gen, (uint32_t)(s - b0));
         bool is_new = false;
         GT_FOR_TABLES(i, T) {
             const uint64_t b = gt_fastmod(h, T.p[i], T.m[i]);
@@ -398,7 +419,7 @@ struct ApplyOp {
                 if (i == 0 && before == 0) n_occ++;
             }
             if (!is_new && (mask & (1u << (i < 7 ? i : 7)))) {
-                is_new = claim_find(cm, claim_key(cm.gen, i, b)) == stamp;
+                is_new = claim_find(cm, claim_key(gen, i, b)) == stamp;
             }
         }
         n_new += is_new;
@@ -568,66 +589,74 @@ struct DegreeOp {
 
 // ------------------------------------------------------------ kernels
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void claim_kernel(Reads R, Tables T, ClaimMap cm, uint8_t* fresh,
                                                      uint64_t* counters) {
     __shared__ TileShared sh;
-    ClaimOp op{T, cm, fresh, R.b0, R.canonical, counters + kErrors};
-    walk_tiles(R, sh, op);
+    ClaimOp op{T, cm, current_gen(counters), fresh, R.b0, R.canonical, counters + kErrors};
+    walk_tiles<PER>(R, sh, op);
     add_counter(counters + kFresh, op.n_fresh);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void apply_kernel(Reads R, Tables T, ClaimMap cm, const uint8_t* fresh,
                                                      uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
     __shared__ TileShared sh;
-    ApplyOp op{T, cm, fresh, R.b0, R.canonical, R.kmer_off, new_flags, ReadAcc{n_new}};
-    walk_tiles(R, sh, op);
+    ApplyOp op{T, cm, current_gen(counters), fresh, R.b0, R.canonical, R.kmer_off, new_flags, ReadAcc{n_new}};
+    walk_tiles<PER>(R, sh, op);
     add_counter(counters + kNUnique, op.n_new);
     add_counter(counters + kNOccupied, op.n_occ);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void exact_lookup_kernel(Reads R, ExactSet E, uint8_t* fresh,
                                                             uint64_t* counters) {
     __shared__ TileShared sh;
     ExactLookupOp op{E, fresh, R.b0, R.canonical};
-    walk_tiles(R, sh, op);
+    walk_tiles<PER>(R, sh, op);
     add_counter(counters + kFresh, op.n_fresh);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void exact_insert_kernel(Reads R, ExactSet E, const uint8_t* fresh,
-                                                            uint32_t gen, uint64_t* counters) {
+                                                            uint64_t* counters) {
     __shared__ TileShared sh;
-    ExactInsertOp op{E, fresh, R.b0, R.canonical, gen, counters + kErrors};
-    walk_tiles(R, sh, op);
+    ExactInsertOp op{E, fresh, R.b0, R.canonical, current_gen(counters), counters + kErrors};
+    walk_tiles<PER>(R, sh, op);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void exact_resolve_kernel(Reads R, ExactSet E, const uint8_t* fresh,
-                                                             uint32_t gen, uint64_t* n_new, uint8_t* new_flags,
+                                                             uint64_t* n_new, uint8_t* new_flags,
                                                              uint64_t* counters) {
     __shared__ TileShared sh;
-    ExactResolveOp op{E, fresh, R.b0, R.canonical, gen, R.kmer_off, new_flags, ReadAcc{n_new}};
-    walk_tiles(R, sh, op);
+    ExactResolveOp op{E, fresh, R.b0, R.canonical, current_gen(counters), R.kmer_off, new_flags, ReadAcc{n_new}};
+    walk_tiles<PER>(R, sh, op);
     add_counter(counters + kNUnique, op.n_new);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void query_kernel(Reads R, Tables T, ExactSet E, int16_t* counts,
                                                      uint64_t* hashes) {
     __shared__ TileShared sh;
     QueryOp op{T, E, R.canonical, R.kmer_off, counts, hashes};
-    walk_tiles(R, sh, op);
+    walk_tiles<PER>(R, sh, op);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void hash_kernel(Reads R, uint64_t* fw, uint64_t* rc, uint8_t* valid) {
     __shared__ TileShared sh;
     HashOp op{R.kmer_off, fw, rc, valid};
-    walk_tiles(R, sh, op);
+    walk_tiles<PER>(R, sh, op);
 }
 
+template <int PER>
 __global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, ExactSet E, int min_count,
                                                       const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
     __shared__ TileShared sh;
     DegreeOp op{T, E, R.canonical, R.K, min_count, R.kmer_off, only, in_deg, out_deg,
                 sh.bytes, 0, sh.tf, sh.tr};
-    walk_tiles(R, sh, op);
+    walk_tiles<PER>(R, sh, op);
 }
 
 // Per-hash operations (dbg.hh:140-177 insert / query / insert_and_query on a
@@ -635,6 +664,7 @@ __global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, Exact
 __global__ __launch_bounds__(kBlock) void hash_claim_kernel(const uint64_t* h, uint64_t n, Tables T, ClaimMap cm,
                                                           uint8_t* fresh, uint64_t* counters) {
     uint64_t nf = 0;
+    const uint32_t gen = current_gen(counters);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint8_t mask = 0;
         GT_FOR_TABLES(t, T) {
@@ -642,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void hash_claim_kernel(const uint64_t* h, u
             const bool empty = T.kind == kBit ? !bit_get(T.base[t], b) : T.base[t][b] == 0;
             if (empty) {
                 mask |= (uint8_t)(1u << (t < 7 ? t : 7));
-                claim_insert(cm, claim_key(cm.gen, t, b), claim_stamp(cm.gen, (uint32_t)i), counters + kErrors);
+                claim_insert(cm, claim_key(gen, t, b), claim_stamp(gen, (uint32_t)i), counters + kErrors);
             }
         }
         fresh[i] = mask;
@@ -655,9 +685,10 @@ __global__ __launch_bounds__(kBlock) void hash_apply_kernel(const uint64_t* h, u
                                                           const uint8_t* fresh, uint8_t* is_new,
                                                           uint64_t* counters) {
     uint64_t nn = 0, occ = 0;
+    const uint32_t gen = current_gen(counters);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         const uint8_t mask = fresh[i];
-        const uint64_t stamp = claim_stamp(cm.gen, (uint32_t)i);
+        const uint64_t stamp = claim_stamp(gen, (uint32_t)i);
         bool nw = false;
         GT_FOR_TABLES(t, T) {
             const uint64_t b = gt_fastmod(h[i], T.p[t], T.m[t]);
@@ -668,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void hash_apply_kernel(const uint64_t* h, u
             } else {
                 if (byte_sat_inc(T.base[t], b) == 0 && t == 0) occ++;
             }
-            if (!nw && (mask & (1u << (t < 7 ? t : 7)))) nw = claim_find(cm, claim_key(cm.gen, t, b)) == stamp;
+            if (!nw && (mask & (1u << (t < 7 ? t : 7)))) nw = claim_find(cm, claim_key(gen, t, b)) == stamp;
         }
         nn += nw;
         if (is_new) is_new[i] = nw;
@@ -689,8 +720,8 @@ __global__ __launch_bounds__(kBlock) void hash_exact_lookup_kernel(const uint64_
 }
 
 __global__ __launch_bounds__(kBlock) void hash_exact_insert_kernel(const uint64_t* h, uint64_t n, ExactSet E,
-                                                                 const uint8_t* fresh, uint32_t gen,
-                                                                 uint64_t* counters) {
+                                                                 const uint8_t* fresh, uint64_t* counters) {
+    const uint32_t gen = current_gen(counters);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         if (!fresh[i]) continue;
         const uint64_t slot = exact_insert(E, h[i], counters + kErrors);
@@ -701,9 +732,10 @@ __global__ __launch_bounds__(kBlock) void hash_exact_insert_kernel(const uint64_
 }
 
 __global__ __launch_bounds__(kBlock) void hash_exact_resolve_kernel(const uint64_t* h, uint64_t n, ExactSet E,
-                                                                  const uint8_t* fresh, uint32_t gen,
-                                                                  uint8_t* is_new, uint64_t* counters) {
+                                                                  const uint8_t* fresh, uint8_t* is_new,
+                                                                  uint64_t* counters) {
     uint64_t nn = 0;
+    const uint32_t gen = current_gen(counters);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         bool nw = false;
         if (fresh[i]) {
@@ -762,15 +794,23 @@ __global__ void seq_insert_query_kernel(const uint64_t* fw, const uint64_t* rc, 
 
 // ------------------------------------------------------------ read layout
 
-// tile_read[t] = read holding base t*kTile (reads are [off[r], off[r+1])).
+// tile_read[t] = read holding base t*kTileGrain (reads are [off[r], off[r+1])).
+// With dyn set, the sizes are read from device memory (bounded by the args).
 __global__ void tile_read_kernel(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
-                                 int64_t n_tiles) {
+                                 int64_t n_tiles, const int64_t* dyn) {
+    if (dyn) {
+        n_reads = (uint64_t)dyn[0] < n_reads ? (uint64_t)dyn[0] : n_reads;
+        n_bases = dyn[1] < n_bases ? dyn[1] : n_bases;
+        const int64_t t = (n_bases + kTileGrain - 1) / kTileGrain;
+        n_tiles = t < n_tiles ? t : n_tiles;
+    }
     for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n_reads;
          r += (uint64_t)gridDim.x * blockDim.x) {
         int64_t a = off[r], b = off[r + 1];
         a = a < 0 ? 0 : (a > n_bases ? n_bases : a);
         b = b < a ? a : (b > n_bases ? n_bases : b);
-        for (int64_t t = (a + kTile - 1) / kTile; t * kTile < b && t < n_tiles; ++t) tile_read[t] = (uint32_t)r;
+        for (int64_t t = (a + kTileGrain - 1) / kTileGrain; t * kTileGrain < b && t < n_tiles; ++t)
+            tile_read[t] = (uint32_t)r;
     }
 }
 
@@ -780,20 +820,23 @@ constexpr int kScanBlock = 256;
 constexpr int kScanPer = 4;
 constexpr int kScanChunk = kScanBlock * kScanPer;
 
-__device__ __forceinline__ int64_t read_kmers(const int64_t* off, uint64_t r, int K) {
+__device__ __forceinline__ int64_t read_kmers(const int64_t* off, uint64_t r, int K, uint64_t live) {
+    if (r >= live) return 0;
     const int64_t len = off[r + 1] - off[r];
     return len >= K ? len - K + 1 : 0;
 }
 
 __global__ __launch_bounds__(kScanBlock) void scan_local_kernel(const int64_t* off, uint64_t n_reads, int K,
-                                                              int64_t* kmer_off, int64_t* block_sums) {
+                                                              int64_t* kmer_off, int64_t* block_sums,
+                                                              const int64_t* dyn) {
+    const uint64_t live = dyn && (uint64_t)dyn[0] < n_reads ? (uint64_t)dyn[0] : n_reads;
     __shared__ int64_t part[kScanBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanPer;
     int64_t v[kScanPer];
     int64_t sum = 0;
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
-        v[j] = base + j < n_reads ? read_kmers(off, base + j, K) : 0;
+        v[j] = base + j < n_reads ? read_kmers(off, base + j, K, live) : 0;
         sum += v[j];
     }
     part[threadIdx.x] = sum;
@@ -847,6 +890,12 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(int64_t* kmer_off,
     }
 }
 
+// Starts a new first-touch generation (one lane; stream-ordered before the
+// sub-batch that uses it).
+__global__ void bump_gen_kernel(uint64_t* counters) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kGen] += 1;
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -897,7 +946,8 @@ __global__ void exact_keys_kernel(ExactSet E, uint64_t* out, uint64_t cap_out, u
 namespace gt {
 
 static int tile_grid(const Launch& L, const Reads& R) {
-    const int64_t tiles = (R.b1 + kTile - 1) / kTile - R.b0 / kTile;
+    const int64_t tile = (int64_t)kBlock * L.per;
+    const int64_t tiles = (R.b1 + tile - 1) / tile - R.b0 / tile;
     const int64_t g = tiles < L.grid ? tiles : L.grid;
     return g > 0 ? (int)g : 1;
 }
@@ -908,19 +958,29 @@ static int flat_grid(const Launch& L, uint64_t n) {
     return g > 0 ? (int)g : 1;
 }
 
+// Instantiates a tile kernel for the launch's PER.
+#define GT_TILE_LAUNCH(L, kernel, ...)                                                        \
+    do {                                                                                      \
+        switch ((L).per) {                                                                    \
+            case 8: kernel<8><<<tile_grid(L, R), kBlock, 0, (L).stream>>>(__VA_ARGS__); break;   \
+            case 32: kernel<32><<<tile_grid(L, R), kBlock, 0, (L).stream>>>(__VA_ARGS__); break; \
+            default: kernel<16><<<tile_grid(L, R), kBlock, 0, (L).stream>>>(__VA_ARGS__); break; \
+        }                                                                                     \
+    } while (0)
+
 void launch_tile_read(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
-                      int64_t n_tiles, hipStream_t s) {
+                      int64_t n_tiles, const int64_t* dyn, hipStream_t s) {
     const uint64_t g = (n_reads + 255) / 256;
     tile_read_kernel<<<(unsigned)(g < 65536 ? (g ? g : 1) : 65536), 256, 0, s>>>(off, n_reads, n_bases, tile_read,
-                                                                              n_tiles);
+                                                                              n_tiles, dyn);
 }
 
 uint64_t scan_block_count(uint64_t n_reads) { return (n_reads + kScanChunk - 1) / kScanChunk; }
 
 void launch_kmer_offsets(const int64_t* off, uint64_t n_reads, int K, int64_t* kmer_off, int64_t* block_sums,
-                         hipStream_t s) {
+                         const int64_t* dyn, hipStream_t s) {
     const uint64_t nb = scan_block_count(n_reads);
-    if (nb) scan_local_kernel<<<(unsigned)nb, kScanBlock, 0, s>>>(off, n_reads, K, kmer_off, block_sums);
+    if (nb) scan_local_kernel<<<(unsigned)nb, kScanBlock, 0, s>>>(off, n_reads, K, kmer_off, block_sums, dyn);
     scan_blocks_kernel<<<1, kScanBlock, 0, s>>>(block_sums, nb, kmer_off + n_reads);
     if (nb) scan_add_kernel<<<(unsigned)nb, kScanBlock, 0, s>>>(kmer_off, n_reads, block_sums);
 }
@@ -933,40 +993,42 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
 
 void launch_claim(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, uint8_t* fresh,
                   uint64_t* counters) {
-    claim_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, cm, fresh, counters);
+    GT_TILE_LAUNCH(L, claim_kernel, R, T, cm, fresh, counters);
 }
 
 void launch_apply(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, const uint8_t* fresh,
                   uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
-    apply_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, cm, fresh, n_new, new_flags, counters);
+    GT_TILE_LAUNCH(L, apply_kernel, R, T, cm, fresh, n_new, new_flags, counters);
 }
 
 void launch_exact_lookup(const Launch& L, const Reads& R, const ExactSet& E, uint8_t* fresh, uint64_t* counters) {
-    exact_lookup_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, counters);
+    GT_TILE_LAUNCH(L, exact_lookup_kernel, R, E, fresh, counters);
 }
 
-void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                          uint64_t* counters) {
-    exact_insert_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, gen, counters);
+    GT_TILE_LAUNCH(L, exact_insert_kernel, R, E, fresh, counters);
 }
 
-void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                           uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
-    exact_resolve_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, E, fresh, gen, n_new, new_flags, counters);
+    GT_TILE_LAUNCH(L, exact_resolve_kernel, R, E, fresh, n_new, new_flags, counters);
 }
+
+void launch_bump_gen(uint64_t* counters, hipStream_t s) { bump_gen_kernel<<<1, 64, 0, s>>>(counters); }
 
 void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,
                   uint64_t* hashes) {
-    query_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, E, counts, hashes);
+    GT_TILE_LAUNCH(L, query_kernel, R, T, E, counts, hashes);
 }
 
 void launch_hash(const Launch& L, const Reads& R, uint64_t* fw, uint64_t* rc, uint8_t* valid) {
-    hash_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, fw, rc, valid);
+    GT_TILE_LAUNCH(L, hash_kernel, R, fw, rc, valid);
 }
 
 void launch_degrees(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int min_count,
                     const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
-    degree_kernel<<<tile_grid(L, R), kBlock, 0, L.stream>>>(R, T, E, min_count, only, in_deg, out_deg);
+    GT_TILE_LAUNCH(L, degree_kernel, R, T, E, min_count, only, in_deg, out_deg);
 }
 
 void launch_hash_claim(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
@@ -985,13 +1047,13 @@ void launch_hash_exact_lookup(const Launch& L, const uint64_t* h, uint64_t n, co
 }
 
 void launch_hash_exact_insert(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
-                              const uint8_t* fresh, uint32_t gen, uint64_t* counters) {
-    hash_exact_insert_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, gen, counters);
+                              const uint8_t* fresh, uint64_t* counters) {
+    hash_exact_insert_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, counters);
 }
 
 void launch_hash_exact_resolve(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
-                               const uint8_t* fresh, uint32_t gen, uint8_t* is_new, uint64_t* counters) {
-    hash_exact_resolve_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, gen, is_new, counters);
+                               const uint8_t* fresh, uint8_t* is_new, uint64_t* counters) {
+    hash_exact_resolve_kernel<<<flat_grid(L, n), kBlock, 0, L.stream>>>(h, n, E, fresh, is_new, counters);
 }
 
 void launch_hash_query(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ExactSet& E,

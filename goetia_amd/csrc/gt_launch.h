@@ -8,12 +8,14 @@ namespace gt {
 struct Launch {
     int grid;              // workgroups for tile walkers (persistent grid)
     hipStream_t stream;
+    int per;               // windows per thread: 8, 16 or 32
 };
 
 void launch_tile_read(const int64_t* off, uint64_t n_reads, int64_t n_bases, uint32_t* tile_read,
-                      int64_t n_tiles, hipStream_t s);
+                      int64_t n_tiles, const int64_t* dyn, hipStream_t s);
 void launch_kmer_offsets(const int64_t* off, uint64_t n_reads, int K, int64_t* kmer_off,
-                         int64_t* block_sums, hipStream_t s);
+                         int64_t* block_sums, const int64_t* dyn, hipStream_t s);
+void launch_bump_gen(uint64_t* counters, hipStream_t s);
 uint64_t scan_block_count(uint64_t n_reads);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 
@@ -22,9 +24,9 @@ void launch_claim(const Launch& L, const Reads& R, const Tables& T, const ClaimM
 void launch_apply(const Launch& L, const Reads& R, const Tables& T, const ClaimMap& cm, const uint8_t* fresh,
                   uint64_t* n_new, uint8_t* new_flags, uint64_t* counters);
 void launch_exact_lookup(const Launch& L, const Reads& R, const ExactSet& E, uint8_t* fresh, uint64_t* counters);
-void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                          uint64_t* counters);
-void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh, uint32_t gen,
+void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                           uint64_t* n_new, uint8_t* new_flags, uint64_t* counters);
 void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,
                   uint64_t* hashes);
@@ -39,9 +41,9 @@ void launch_hash_apply(const Launch& L, const uint64_t* h, uint64_t n, const Tab
 void launch_hash_exact_lookup(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E, uint8_t* fresh,
                               uint64_t* counters);
 void launch_hash_exact_insert(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
-                              const uint8_t* fresh, uint32_t gen, uint64_t* counters);
+                              const uint8_t* fresh, uint64_t* counters);
 void launch_hash_exact_resolve(const Launch& L, const uint64_t* h, uint64_t n, const ExactSet& E,
-                               const uint8_t* fresh, uint32_t gen, uint8_t* is_new, uint64_t* counters);
+                               const uint8_t* fresh, uint8_t* is_new, uint64_t* counters);
 void launch_hash_query(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ExactSet& E,
                        int16_t* counts);
 

@@ -212,6 +212,44 @@ int gt_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t*
 int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n,
                                int16_t* counts);
 
+/* ------------------------------------------------------------ tuning */
+
+/* Tile shape of the k-mer kernels per storage kind (0 = hashing without a
+ * storage): windows rolled per thread (8, 16 or 32; a workgroup of 256
+ * threads owns 256 x per start positions) and persistent workgroups per CU.
+ * No counterpart in goetia. */
+int gt_tune_set(int kind, int per, int blocks_per_cu);
+int gt_tune_get(int kind, int* per, int* blocks_per_cu);
+/* Autotuner: times the storage's read-only k-mer kernels (the first-touch
+ * claim pass, or the exact-set lookup pass, plus query) on a batch of reads
+ * in device memory for every tile shape, keeps the fastest for the storage's
+ * kind and returns it.  The storage's contents are not changed. */
+int gt_autotune(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, int* per,
+                int* blocks_per_cu, void* stream);
+
+/* ------------------------------------------------------------ pipeline */
+
+/* Batched insert engine (no counterpart in goetia; it replaces calling
+ * dBG::insert_sequence read by read, processors.hh:304-331).  Batches of at
+ * most max_reads reads / max_bases bases arrive in host memory; each is
+ * copied into one of two pinned + device buffer pairs on a copy stream while
+ * the previous batch runs, and its device work (read layout, one first-touch
+ * sub-batch, insert, per-read n_new) is a hipGraph captured once per buffer
+ * and replayed for every batch (batch sizes are read from device memory).
+ * Results equal gt_dbg_insert over the same reads in submission order.
+ * gt_pipeline_submit returns once the batch is copied to pinned memory;
+ * n_new (host, n_reads entries, may be NULL) is filled when the batch
+ * completes, at the latest by gt_pipeline_flush.  max_bases must fit one
+ * first-touch sub-batch (at most 2^26 start positions for 4 tables). */
+typedef struct gt_pipeline gt_pipeline;
+int gt_pipeline_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_t max_reads,
+                    uint64_t max_bases, gt_pipeline** out);
+int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                       uint64_t* n_new);
+int gt_pipeline_flush(gt_pipeline* p);
+int gt_pipeline_free(gt_pipeline* p);
+
 /* ------------------------------------------------------------ timing */
 
 /* Kernel timing (no counterpart in goetia): while enabled, every kernel this
