@@ -44,7 +44,8 @@ int main() {
     // KmerIterator (GPU) == per-k-mer host hashes
     const std::string seq = "TCACCTGTGTTGTGCTACTTGCGGCGCAAGTCCATGATCGATCGGGTAC";
     KmerIterator<CanLemireShifter> it(seq, 21);
-    for (size_t i = 0; !it.done(); ++i) CHECK(it.next() == sh.hash(seq.substr(i, 21)));
+    CanLemireShifter sh21(21);
+    for (size_t i = 0; !it.done(); ++i) CHECK(it.next() == sh21.hash(seq.substr(i, 21)));
 
     CHECK(run_presence(BitStorage::build(100000, 4)) == 0);
     CHECK(run_presence(SparseppSetStorage::build()) == 0);
