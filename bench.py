@@ -52,6 +52,11 @@ def parse():
     p.add_argument('--no-chunks', action='store_true')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
+    p.add_argument('--workload', default='cms_20m',
+                   choices=['cms_20m', 'bloom_100k', 'exact_mixed', 'neighbors_5m', 'serving_64'],
+                   help='cms_20m is the headline; the others print their own JSON line')
+    p.add_argument('--plain-chunk-kmers', type=int, default=500_000,
+                   help='chunk size of the plain path in k-mers (goetia IntervalCounter default)')
     return p.parse_args()
 
 
@@ -88,6 +93,9 @@ def main():
         print(json.dumps({'error': 'this build targets one GPU; multi-GPU sharding is a design '
                                    'described in DESIGN.md and is not run', 'n_gpus': args.gpus}))
         sys.exit(2)
+    if args.workload != 'cms_20m':
+        import bench_configs
+        return bench_configs.run(args)
     import torch
     from goetia_amd import _native as N
     from goetia_amd import libgoetia as lg

@@ -1,0 +1,221 @@
+"""The non-headline configurations of bench.py (--workload ...).
+
+bloom_100k    1 Mbp genome, 100k x 150 bp, K=21, BitStorage(1e7, 4): insert
+              with per-read n_new (tables are cache-resident).
+exact_mixed   uneven reads (200k lognormal, median 8 kb, + 2M x 150 bp), K=25,
+              exact set at load factor ~0.5: batched pipeline (hipGraph
+              replays, autotuned tile shape) against the plain per-chunk path.
+neighbors_5m  repeats + SNP bubbles, 5M x 150 bp, K=31, ByteStorage(2^28, 4):
+              insert, then degrees (count >= 2) of every new k-mer and the
+              decision nodes.
+serving_64    64 sample streams, see goetia_amd/serving.py.
+"""
+import json
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def bloom(args):
+    import torch
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    cfg, batch = workload.bloom_100k(args.scale)
+    K = cfg['K']
+    d = batch.to_device()
+    store = lg.BitStorage.build(cfg['max_table'], cfg['n_tables'])
+    g = lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(store, K)
+    n_new = torch.zeros(batch.n_reads, dtype=torch.int64, device='cuda')
+    nk = int(batch.kmer_counts(K).sum())
+    stream = torch.cuda.current_stream()
+
+    def step():
+        store.reset()
+        g.insert_device(d, n_new=n_new, stream=stream)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    _sync()
+    N.call('gt_profile_enable', 1)
+    N.call('gt_profile_reset')
+    steps = max(args.steps, 10)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    _sync()
+    dt = (time.perf_counter() - t0) / steps
+    N.call('gt_profile_enable', 0)
+    # the same reads through the host-pointer entry point (PCIe included)
+    g2 = lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage.build(cfg['max_table'], 4), K)
+    t0 = time.perf_counter()
+    nn_host = g2.insert_sequences(batch)
+    dt_host = time.perf_counter() - t0
+    same = bool(np.array_equal(nn_host, n_new.cpu().numpy().astype(np.uint64)))
+    out = {'metric': 'canonical k-mers/s insert with per-read n_new (1 GPU)', 'workload': 'bloom_100k',
+           'value': round(nk / dt, 1), 'unit': 'kmers/s', 'ms_per_step': round(dt * 1e3, 3),
+           'kmers': nk, 'n_unique': g.n_unique(), 'host_api_kmers_per_s': round(nk / dt_host, 1),
+           'host_api_same_n_new': same, 'tables_bytes': sum(store.table_bytes(i) for i in range(4)),
+           'note': 'tables (5 MB) stay in L2 / Infinity Cache; not an HBM-roofline case'}
+    print(json.dumps(out), flush=True)
+
+
+def exact(args):
+    import torch
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    from goetia_amd.pipeline import Pipeline, autotune, tune_get
+    from goetia_amd.reads import ReadBatch
+    t0 = time.perf_counter()
+    cfg, batch = workload.exact_mixed(args.scale)
+    K = cfg['K']
+    log(f'generated {batch.n_reads} reads, {batch.n_bases} bases in {time.perf_counter() - t0:.1f}s')
+    nk = int(batch.kmer_counts(K).sum())
+    hint = int(2_000_000_000 * args.scale) + 1
+    lens = batch.lengths()
+
+    def read_ranges(max_bases, max_reads):
+        """Consecutive read ranges of at most max_bases bases / max_reads reads."""
+        ranges, start, acc = [], 0, 0
+        for r in range(batch.n_reads):
+            L = int(lens[r])
+            if r > start and (acc + L > max_bases or r - start >= max_reads):
+                ranges.append((start, r))
+                start, acc = r, 0
+            acc += L
+        ranges.append((start, batch.n_reads))
+        return ranges
+
+    def sub(a, b):
+        return ReadBatch(batch.seq[batch.offsets[a]:batch.offsets[b]], batch.offsets[a:b + 1] - batch.offsets[a])
+
+    # plain path: goetia's processor granularity, one synchronous host call per chunk
+    chunk_bases = args.plain_chunk_kmers + 0
+    plain_ranges = read_ranges(chunk_bases, 1 << 30)
+    store = lg.SparseppSetStorage.build(capacity_hint=hint)
+    g = lg.dBG[lg.SparseppSetStorage, lg.CanLemireShifter].build(store, K)
+    plain_new = np.zeros(batch.n_reads, np.uint64)
+    subs = [sub(a, b) for a, b in plain_ranges]
+    _sync()
+    t0 = time.perf_counter()
+    for (a, b), sb in zip(plain_ranges, subs):
+        plain_new[a:b] = g.insert_sequences(sb)
+    _sync()
+    t_plain = time.perf_counter() - t0
+    plain_unique = g.n_unique()
+    cap = g.n_occupied()
+    del subs, g, store
+    log(f'plain: {len(plain_ranges)} chunks, {t_plain:.2f}s, n_unique {plain_unique}')
+
+    # batched path: autotuned tile shape, pipeline of large batches, graph replays
+    store = lg.SparseppSetStorage.build(capacity_hint=hint)
+    g = lg.dBG[lg.SparseppSetStorage, lg.CanLemireShifter].build(store, K)
+    probe = sub(*read_ranges(64 << 20, 1 << 30)[0])
+    shape = autotune(g, probe.to_device())
+    log(f'autotuned tile shape (per, blocks/CU) = {shape}')
+    max_bases = min(256 << 20, batch.n_bases)
+    ranges = read_ranges(max_bases, 1 << 26)
+    subs = [sub(a, b) for a, b in ranges]
+    pipe = Pipeline(g, max(b - a for a, b in ranges), max(s.n_bases for s in subs))
+    batched_new = np.zeros(batch.n_reads, np.uint64)
+    outs = []
+    _sync()
+    t0 = time.perf_counter()
+    for (a, b), sb in zip(ranges, subs):
+        outs.append(((a, b), pipe.submit(sb)))
+    pipe.flush()
+    t_batched = time.perf_counter() - t0
+    for (a, b), o in outs:
+        batched_new[a:b] = o[:b - a]
+    pipe.close()
+    identical = bool(np.array_equal(plain_new, batched_new)) and g.n_unique() == plain_unique
+    out = {'metric': 'canonical k-mers/s exact-set insert with per-read n_new, batched vs plain (1 GPU)',
+           'workload': 'exact_mixed', 'value': round(nk / t_batched, 1), 'unit': 'kmers/s',
+           'plain_kmers_per_s': round(nk / t_plain, 1), 'batched_over_plain': round(t_plain / t_batched, 3),
+           'kmers': nk, 'reads': batch.n_reads, 'bases': batch.n_bases,
+           'n_unique': plain_unique, 'slots': cap, 'load_factor': round(plain_unique / cap, 3),
+           'sum_n_new_equals_n_unique': int(batched_new.sum()) == plain_unique,
+           'plain_equals_batched': identical, 'tile_shape': {'per': shape[0], 'blocks_per_cu': shape[1]},
+           'plain': f'{len(plain_ranges)} synchronous gt_dbg_insert calls of <= {chunk_bases} bases (host memory)',
+           'batched': f'{len(ranges)} pipeline batches of <= {max_bases} bases (host memory, pinned double '
+                      f'buffer, H2D overlapped, hipGraph replay)'}
+    print(json.dumps(out), flush=True)
+
+
+def neighbors(args):
+    import torch
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    t0 = time.perf_counter()
+    cfg, batch = workload.neighbors_5m(args.scale)
+    K = cfg['K']
+    log(f'generated {batch.n_reads} reads in {time.perf_counter() - t0:.1f}s')
+    d = batch.to_device()
+    nk = int(batch.kmer_counts(K).sum())
+    store = lg.ByteStorage.build(cfg['max_table'], cfg['n_tables'])
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(store, K)
+    flags = torch.zeros(nk, dtype=torch.uint8, device='cuda')
+    ind = torch.empty(nk, dtype=torch.uint8, device='cuda')
+    outd = torch.empty(nk, dtype=torch.uint8, device='cuda')
+    stream = torch.cuda.current_stream()
+    N.call('gt_profile_enable', 1)
+    N.call('gt_profile_reset')
+    _sync()
+    t0 = time.perf_counter()
+    g.insert_device(d, new_flags=flags, stream=stream)
+    _sync()
+    t_ins = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.degrees_device(d, ind, outd, min_count=cfg['min_count'], only=flags, stream=stream)
+    _sync()
+    t_deg = time.perf_counter() - t0
+    N.call('gt_profile_enable', 0)
+    import ctypes as C
+    cnt, ms = C.c_uint64(), C.c_double()
+    N.call('gt_profile_get', b'degrees', C.byref(cnt), C.byref(ms))
+    n_new = int(flags.sum().item())
+    sel = flags.bool()
+    dec = sel & ((ind > 1) | (outd > 1)) & (ind != 255)
+    n_dec_windows = int(dec.sum().item())
+    # decision nodes as canonical hashes (unique)
+    hashes = torch.empty(nk, dtype=torch.int64, device='cuda')
+    counts = torch.empty(nk, dtype=torch.int16, device='cuda')
+    g.query_device(d, counts, hashes=hashes, stream=stream)
+    _sync()
+    dec_nodes = int(torch.unique(hashes[dec]).numel())
+    algo = n_new * 2048
+    achieved = algo / (ms.value / 1e3) / 1e9
+    out = {'metric': 'canonical k-mers/s insert + decision-node degrees of new k-mers (1 GPU)',
+           'workload': 'neighbors_5m', 'value': round(nk / (t_ins + t_deg), 1), 'unit': 'kmers/s',
+           'kmers': nk, 'new_kmers': n_new, 'n_unique': g.n_unique(),
+           'decision_windows': n_dec_windows, 'decision_nodes': dec_nodes,
+           'insert_ms': round(t_ins * 1e3, 2), 'degrees_ms': round(t_deg * 1e3, 2),
+           'roofline': {'bound': 'hbm', 'kernel': 'degrees', 'achieved': round(achieved, 1),
+                        'peak': HBM_PEAK_GBPS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4),
+                        'bytes_per_new_kmer': 2048},
+           'min_count': cfg['min_count']}
+    print(json.dumps(out), flush=True)
+
+
+def serving(args):
+    from goetia_amd import serving as S
+    out = S.bench(args)
+    print(json.dumps(out), flush=True)
+
+
+def run(args):
+    return {'bloom_100k': bloom, 'exact_mixed': exact, 'neighbors_5m': neighbors,
+            'serving_64': serving}[args.workload](args)

@@ -93,6 +93,8 @@ SIGNATURES = {
     'gt_storage_write_table': [st_p, u32, vp],
     'gt_storage_update_from': [st_p, st_p],
     'gt_storage_exact_keys': [st_p, vp, u64, u64p],
+    'gt_storage_save_async': [st_p, vp, vp, vp],
+    'gt_storage_load_async': [st_p, vp, vp, vp],
     'gt_kmer_offsets_dev': [u16, vp, u64, vp, vp],
     'gt_hash_reads_dev': [u16, cint, vp, vp, u64, u64, vp, vp, vp, vp],
     'gt_hash_reads': [u16, cint, vp, vp, u64, vp, vp, vp],
@@ -119,6 +121,17 @@ SIGNATURES = {
     'gt_pipeline_submit': [vp, vp, vp, u64, vp],
     'gt_pipeline_flush': [vp],
     'gt_pipeline_free': [vp],
+    'gt_workspace_new': [C.POINTER(vp)],
+    'gt_workspace_free': [vp],
+    'gt_dbg_insert_ws_dev': [st_p, vp, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
+    'gt_dbg_query_ws_dev': [st_p, vp, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
+    'gt_server_new': [cint, u64p, u32, u16, cint, cint, u32, u64, u32, cint, C.POINTER(vp)],
+    'gt_server_submit': [vp, u32, vp, vp, u64, vp, u64p],
+    'gt_server_wait': [vp, u64, C.POINTER(C.c_double)],
+    'gt_server_drain': [vp],
+    'gt_server_get_stats': [vp, vp],
+    'gt_server_stream_table': [vp, u32, u32, vp, vp],
+    'gt_server_free': [vp],
     'gt_profile_enable': [cint],
     'gt_profile_reset': [],
     'gt_profile_get': [C.c_char_p, u64p, C.POINTER(C.c_double)],
@@ -144,8 +157,9 @@ def load():
         raise ImportError(f'{LIB_PATH} is missing: build it with `python -m goetia_amd.build` '
                           '(goetia_amd has no CPU fallback)')
     lib = C.CDLL(LIB_PATH)
-    lib.gt_last_error.restype = C.c_char_p
-    lib.gt_last_error.argtypes = []
+    for name in ('gt_last_error', 'gt_server_last_error'):
+        getattr(lib, name).restype = C.c_char_p
+        getattr(lib, name).argtypes = []
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = C.c_int
@@ -156,7 +170,8 @@ def load():
 
 def check(rc, what=''):
     if rc != GT_OK:
-        msg = load().gt_last_error().decode(errors='replace')
+        err = load().gt_server_last_error if what.startswith('gt_server') else load().gt_last_error
+        msg = err().decode(errors='replace')
         exc = _ERRORS.get(rc, GoetiaException)
         raise exc(f'{what}: {msg}' if what else msg)
 

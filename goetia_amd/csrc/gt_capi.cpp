@@ -191,6 +191,20 @@ uint32_t log2u(uint64_t p) {
 
 }  // namespace
 
+// ------------------------------------------------------------ scratch
+
+// Per-call device scratch: the first-touch claim map with its own
+// generation counter, the fresh masks of one sub-batch, and the read layout
+// (tile_read, kmer_off).  Owned by a storage (its default), by a pipeline, or
+// by a gt_workspace that many storages share (the serving engine's slots).
+struct Scratch {
+    DevBuf claim_keys, claim_vals, claim_gen;
+    uint64_t claim_cap = 0;
+    uint64_t claim_gen_issued = 0;     // bumps enqueued so far (the device counter follows it)
+    DevBuf fresh, tile_read, kmer_off, scan_blocks;
+    bool fixed = false;                // buffers preallocated and never moved (captured graphs)
+};
+
 // ------------------------------------------------------------ storage
 
 struct gt_storage {
@@ -205,15 +219,11 @@ struct gt_storage {
     DevBuf ekeys, estamp, eones;
     uint64_t ecap = 0;
     uint64_t ecount_ub = 0;            // host upper bound on the set size
-    // scratch
-    DevBuf claim_keys, claim_vals;
-    uint64_t claim_cap = 0;
-    DevBuf fresh;
-    DevBuf kmer_off, tile_read, scan_blocks;
+    Scratch own;                       // default scratch of calls on this storage
     DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
     uint64_t gen_issued = 0;           // generation bumps enqueued so far (device kGen follows it)
     uint64_t epoch = 0;                // bumped when device table pointers change (exact-set growth)
-    std::vector<struct gt_pipeline*> pipelines;   // attached pipelines (their claim maps are cleared on wrap)
+    std::vector<struct gt_pipeline*> pipelines;   // attached pipelines
     std::recursive_mutex mu;
 
     Tables tables_view() const {
@@ -269,8 +279,14 @@ struct gt_pipeline {
     hipStream_t compute = nullptr, copy = nullptr;
     Slot slot[2];
     int next = 0;
-    DevBuf claim_keys, claim_vals, fresh, tile_read;
-    uint64_t claim_cap = 0;
+    Scratch sc;
+};
+
+// Shared scratch for storages that are used one call at a time (no
+// counterpart in goetia): see gt_workspace_new.
+struct gt_workspace {
+    Scratch sc;
+    std::recursive_mutex mu;
 };
 
 namespace {
@@ -333,24 +349,28 @@ int check_kernel_errors(gt_storage* st, hipStream_t s) {
 }
 
 // Prepares tile_read (and kmer_off when wanted) for one call.
-int prepare_reads(gt_storage* st, Reads& R, bool want_kmer_off, hipStream_t s) {
+int prepare_reads(Scratch& sc, Reads& R, bool want_kmer_off, hipStream_t s) {
     const int64_t n_tiles = (R.n_bases + kTileGrain - 1) / kTileGrain;
-    GT_TRY(st->tile_read.ensure((size_t)(n_tiles + 1) * sizeof(uint32_t)));
+    GT_TRY(sc.tile_read.ensure((size_t)(n_tiles + 1) * sizeof(uint32_t)));
     // tile_read entries are all written for well-formed offsets; zero first so
     // malformed offsets stay in bounds.
-    launch_fill_u64(st->tile_read.as<uint64_t>(), (st->tile_read.bytes) / 8, 0, s);
-    launch_tile_read(R.off, R.n_reads, R.n_bases, st->tile_read.as<uint32_t>(), n_tiles, R.dyn, s);
-    R.tile_read = st->tile_read.as<uint32_t>();
+    launch_fill_u64(sc.tile_read.as<uint64_t>(), (sc.tile_read.bytes) / 8, 0, s);
+    launch_tile_read(R.off, R.n_reads, R.n_bases, sc.tile_read.as<uint32_t>(), n_tiles, R.dyn, s);
+    R.tile_read = sc.tile_read.as<uint32_t>();
     R.kmer_off = nullptr;
     if (want_kmer_off) {
-        GT_TRY(st->kmer_off.ensure((R.n_reads + 1) * sizeof(int64_t)));
-        GT_TRY(st->scan_blocks.ensure((scan_block_count(R.n_reads) + 1) * sizeof(int64_t)));
-        launch_kmer_offsets(R.off, R.n_reads, R.K, st->kmer_off.as<int64_t>(), st->scan_blocks.as<int64_t>(), R.dyn,
+        GT_TRY(sc.kmer_off.ensure((R.n_reads + 1) * sizeof(int64_t)));
+        GT_TRY(sc.scan_blocks.ensure((scan_block_count(R.n_reads) + 1) * sizeof(int64_t)));
+        launch_kmer_offsets(R.off, R.n_reads, R.K, sc.kmer_off.as<int64_t>(), sc.scan_blocks.as<int64_t>(), R.dyn,
                             s);
-        R.kmer_off = st->kmer_off.as<int64_t>();
+        R.kmer_off = sc.kmer_off.as<int64_t>();
     }
     GT_HIP(hipGetLastError());
     return GT_OK;
+}
+
+int prepare_reads(gt_storage* st, Reads& R, bool want_kmer_off, hipStream_t s) {
+    return prepare_reads(st->own, R, want_kmer_off, s);
 }
 
 int check_read_args(uint16_t K, int mode, const void* seq, const int64_t* offsets, uint64_t n_reads,
@@ -389,69 +409,88 @@ uint64_t sub_batch_cap() {
     return x ? std::max<uint64_t>((x + kTileMax - 1) / kTileMax * kTileMax, kTileMax) : 0;
 }
 
+// Allocates a claim map of `want` slots (zeroed) with its generation counter.
+int alloc_claims(Scratch& sc, uint64_t want, hipStream_t s) {
+    GT_TRY(sc.claim_keys.ensure(want * sizeof(uint64_t)));
+    GT_TRY(sc.claim_vals.ensure(want * sizeof(uint64_t)));
+    GT_TRY(sc.claim_gen.ensure(sizeof(uint64_t)));
+    launch_fill_u64(sc.claim_keys.as<uint64_t>(), want, 0, s);
+    launch_fill_u64(sc.claim_vals.as<uint64_t>(), want, 0, s);
+    launch_fill_u64(sc.claim_gen.as<uint64_t>(), 1, 0, s);
+    GT_HIP(hipGetLastError());
+    sc.claim_cap = want;
+    sc.claim_gen_issued = 0;
+    return GT_OK;
+}
+
 // Largest sub-batch (in start positions, a multiple of kTileMax) whose claims fit
 // the claim map at load factor <= 1/2; grows the claim map and fresh buffer
-// as needed for this call.
-int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t s) {
+// as needed for this call (fixed scratch only checks).
+int prepare_claims(Scratch& sc, int kind, uint32_t n_tables, uint64_t n_starts, uint64_t* sub, hipStream_t s) {
     const uint64_t kMaxClaimCap = 1ULL << 29;      // 8 GiB of claim slots
     const uint64_t kMinClaimCap = 1ULL << 16;
-    const uint64_t per = std::max<uint32_t>(st->n_tables, 1);
+    const uint64_t per = std::max<uint32_t>(n_tables, 1);
     const uint64_t cap = sub_batch_cap();
-    if (st->kind == kExact) {
-        uint64_t sb = std::min<uint64_t>(n_starts, 1ULL << 31);
-        sb = (sb + kTileMax - 1) / kTileMax * kTileMax;
+    const uint64_t starts = (n_starts + kTileMax - 1) / kTileMax * kTileMax;
+    if (kind == kExact) {
+        uint64_t sb = std::min<uint64_t>(starts, 1ULL << 31);
         if (cap) sb = std::min(sb, cap);
-        GT_TRY(st->fresh.ensure(sb));
+        if (sc.fixed) {
+            if (sc.fresh.bytes < sb) sb = sc.fresh.bytes / kTileMax * kTileMax;
+        } else {
+            GT_TRY(sc.fresh.ensure(sb));
+        }
+        if (sb == 0) return fail(GT_ERR_INTERNAL, "scratch too small");
+        if (!sc.claim_cap) GT_TRY(alloc_claims(sc, 2, s));   // unused by the exact set; a valid view
         *sub = sb;
         return GT_OK;
     }
     uint64_t want = std::min(kMaxClaimCap, std::max(kMinClaimCap, next_pow2(2 * per * (n_starts + kTileMax))));
-    if (want > st->claim_cap) {
-        GT_TRY(st->claim_keys.ensure(want * sizeof(uint64_t)));
-        GT_TRY(st->claim_vals.ensure(want * sizeof(uint64_t)));
-        launch_fill_u64(st->claim_keys.as<uint64_t>(), want, 0, s);
-        launch_fill_u64(st->claim_vals.as<uint64_t>(), want, 0, s);
-        GT_HIP(hipGetLastError());
-        st->claim_cap = want;
-    }
-    uint64_t sb = st->claim_cap / (2 * per);
+    if (want > sc.claim_cap && !sc.fixed) GT_TRY(alloc_claims(sc, want, s));
+    uint64_t sb = sc.claim_cap / (2 * per);
     sb = std::min<uint64_t>(sb, 1ULL << 31) / kTileMax * kTileMax;
     if (cap) sb = std::min(sb, cap);
+    if (sc.fixed) {
+        sb = std::min<uint64_t>(sb, sc.fresh.bytes / kTileMax * kTileMax);
+    } else {
+        GT_TRY(sc.fresh.ensure(std::min<uint64_t>(sb, starts)));
+    }
     if (sb == 0) return fail(GT_ERR_INTERNAL, "claim map too small");
-    GT_TRY(st->fresh.ensure(std::min<uint64_t>(sb, (n_starts + kTileMax - 1) / kTileMax * kTileMax)));
     *sub = sb;
     return GT_OK;
 }
 
-// Enqueues the next first-touch generation.  Claim-map keys keep 28 bits of
-// it, so the map is cleared whenever that field would wrap, and generation
+int prepare_claims(gt_storage* st, uint64_t n_starts, uint64_t* sub, hipStream_t s) {
+    return prepare_claims(st->own, st->kind, st->n_tables, n_starts, sub, s);
+}
+
+// Enqueues the next first-touch generation of the storage (exact-set stamps,
+// 32 bits) and of the claim map.  Claim-map keys keep 28 bits of the map's
+// generation, so the map is cleared whenever that field would wrap, and
 // values that are 0 in those bits are skipped (cleared keys read as gen 0).
-int next_gen(gt_storage* st, hipStream_t s) {
-    if (st->gen_issued + 2 >= 0xFFFFFFFFULL) return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
+int next_gen(gt_storage* st, Scratch& sc, hipStream_t s) {
+    if (st->gen_issued + 2 >= 0xFFFFFFFFULL || sc.claim_gen_issued + 2 >= 0xFFFFFFFFULL)
+        return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
     st->gen_issued++;
-    launch_bump_gen(st->counters.as<uint64_t>(), s);
-    if ((st->gen_issued & kGenKeyMask) == 0) {
-        if (st->claim_cap) {
-            launch_fill_u64(st->claim_keys.as<uint64_t>(), st->claim_cap, 0, s);
-            launch_fill_u64(st->claim_vals.as<uint64_t>(), st->claim_cap, 0, s);
-        }
-        for (auto* p : st->pipelines) {
-            launch_fill_u64(p->claim_keys.as<uint64_t>(), p->claim_cap, 0, s);
-            launch_fill_u64(p->claim_vals.as<uint64_t>(), p->claim_cap, 0, s);
-        }
-        st->gen_issued++;
-        launch_bump_gen(st->counters.as<uint64_t>(), s);
+    sc.claim_gen_issued++;
+    launch_bump_gen(st->counters.as<uint64_t>() + kGen, sc.claim_gen.as<uint64_t>(), s);
+    if ((sc.claim_gen_issued & kGenKeyMask) == 0) {
+        launch_fill_u64(sc.claim_keys.as<uint64_t>(), sc.claim_cap, 0, s);
+        launch_fill_u64(sc.claim_vals.as<uint64_t>(), sc.claim_cap, 0, s);
+        sc.claim_gen_issued++;
+        launch_bump_gen(nullptr, sc.claim_gen.as<uint64_t>(), s);
     }
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
 
-ClaimMap claim_view(const gt_storage* st) {
+ClaimMap claim_view(const Scratch& sc) {
     ClaimMap cm{};
-    cm.keys = st->claim_keys.as<uint64_t>();
-    cm.vals = st->claim_vals.as<uint64_t>();
-    cm.cap = st->claim_cap;
-    cm.log2cap = log2u(st->claim_cap);
+    cm.keys = sc.claim_keys.as<uint64_t>();
+    cm.vals = sc.claim_vals.as<uint64_t>();
+    cm.gen = sc.claim_gen.as<uint64_t>();
+    cm.cap = sc.claim_cap;
+    cm.log2cap = log2u(sc.claim_cap);
     return cm;
 }
 
@@ -490,10 +529,12 @@ Launch make_launch(const gt_storage* st, hipStream_t s) { return make_launch(st-
 // Device work of one first-touch sub-batch [Rs.b0, Rs.b1) with the given
 // claim map and fresh buffer.  For the exact set, `growth` allows a
 // synchronising capacity check (never while a graph is being captured).
-int enqueue_subbatch(gt_storage* st, const Launch& L, const Reads& Rs, const ClaimMap& cm, uint8_t* fresh,
-                     uint64_t* n_new, uint8_t* new_flags, hipStream_t s, bool growth) {
+int enqueue_subbatch(gt_storage* st, Scratch& sc, const Launch& L, const Reads& Rs, uint64_t* n_new,
+                     uint8_t* new_flags, hipStream_t s, bool growth) {
     uint64_t* ctr = st->counters.as<uint64_t>();
-    GT_TRY(next_gen(st, s));
+    uint8_t* fresh = sc.fresh.as<uint8_t>();
+    GT_TRY(next_gen(st, sc, s));
+    const ClaimMap cm = claim_view(sc);
     if (st->kind == kExact) {
         const uint64_t nstarts = (uint64_t)(Rs.b1 - Rs.b0);
         if (growth && 2 * (st->ecount_ub + nstarts) > st->ecap) {
@@ -538,23 +579,27 @@ int enqueue_subbatch(gt_storage* st, const Launch& L, const Reads& Rs, const Cla
     return GT_OK;
 }
 
-int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
+int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
     if (R.n_reads == 0 || R.n_bases == 0) {
         if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
         return GT_OK;
     }
-    GT_TRY(prepare_reads(st, R, new_flags != nullptr, s));
+    GT_TRY(prepare_reads(sc, R, new_flags != nullptr, s));
     if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
     uint64_t sub = 0;
-    GT_TRY(prepare_claims(st, (uint64_t)R.n_bases, &sub, s));
+    GT_TRY(prepare_claims(sc, st->kind, st->n_tables, (uint64_t)R.n_bases, &sub, s));
     const Launch L = make_launch(st, s);
     for (int64_t b0 = 0; b0 < R.n_bases; b0 += (int64_t)sub) {
         Reads Rs = R;
         Rs.b0 = b0;
         Rs.b1 = std::min<int64_t>(R.n_bases, b0 + (int64_t)sub);
-        GT_TRY(enqueue_subbatch(st, L, Rs, claim_view(st), st->fresh.as<uint8_t>(), n_new, new_flags, s, true));
+        GT_TRY(enqueue_subbatch(st, sc, L, Rs, n_new, new_flags, s, true));
     }
     return GT_OK;
+}
+
+int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
+    return dbg_insert_impl(st, st->own, R, n_new, new_flags, s);
 }
 
 int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* is_new, hipStream_t s) {
@@ -565,8 +610,8 @@ int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* i
     GT_TRY(prepare_claims(st, n, &sub, s));
     for (uint64_t i0 = 0; i0 < n; i0 += sub) {
         const uint64_t m = std::min<uint64_t>(sub, n - i0);
-        GT_TRY(next_gen(st, s));
-        uint8_t* fresh = st->fresh.as<uint8_t>();
+        GT_TRY(next_gen(st, st->own, s));
+        uint8_t* fresh = st->own.fresh.as<uint8_t>();
         if (st->kind == kExact) {
             GT_TRY(exact_reserve(st, m, s));
             st->ecount_ub += m;
@@ -576,7 +621,7 @@ int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* i
             launch_hash_exact_resolve(L, h + i0, m, E, fresh, is_new ? is_new + i0 : nullptr, ctr);
         } else {
             const Tables T = st->tables_view();
-            const ClaimMap cm = claim_view(st);
+            const ClaimMap cm = claim_view(st->own);
             launch_hash_claim(L, h + i0, m, T, cm, fresh, ctr);
             launch_hash_apply(L, h + i0, m, T, cm, fresh, is_new ? is_new + i0 : nullptr, ctr);
         }
@@ -859,6 +904,41 @@ int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint
     return GT_OK;
 }
 
+int gt_storage_save_async(gt_storage* st, uint8_t* const* host_tables, uint64_t* counters, void* stream) {
+    if (!valid_storage(st) || st->kind == kExact) return fail(GT_ERR_INVALID_ARG, "a Bloom or count-min storage is needed");
+    if (!host_tables || !counters) return fail(GT_ERR_INVALID_ARG, "null host buffers");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    for (uint32_t i = 0; i < st->n_tables; ++i) {
+        GT_HIP(hipMemcpyAsync(host_tables[i], st->tables.as<uint8_t>() + st->table_off[i], st->nbytes[i],
+                              hipMemcpyDeviceToHost, s));
+    }
+    GT_HIP(hipMemcpyAsync(counters, st->counters.p, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    return GT_OK;
+}
+
+int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, const uint64_t* counters,
+                          void* stream) {
+    if (!valid_storage(st) || st->kind == kExact) return fail(GT_ERR_INVALID_ARG, "a Bloom or count-min storage is needed");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    if (!host_tables) {
+        launch_fill_u64(st->tables.as<uint64_t>(), st->tables.bytes / 8, 0, s);
+        launch_fill_u64(st->counters.as<uint64_t>(), 2, 0, s);
+        GT_HIP(hipGetLastError());
+        return GT_OK;
+    }
+    if (!counters) return fail(GT_ERR_INVALID_ARG, "null counters");
+    for (uint32_t i = 0; i < st->n_tables; ++i) {
+        GT_HIP(hipMemcpyAsync(st->tables.as<uint8_t>() + st->table_off[i], host_tables[i], st->nbytes[i],
+                              hipMemcpyHostToDevice, s));
+    }
+    GT_HIP(hipMemcpyAsync(st->counters.p, counters, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    return GT_OK;
+}
+
 // ------------------------------------------------------------ hashing
 
 int gt_kmer_offsets_dev(uint16_t K, const int64_t* offsets, uint64_t n_reads, int64_t* kmer_offsets, void* stream) {
@@ -1060,7 +1140,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         // of the same bins: hash the batch, then apply it in k-mer order.
         GT_TRY(prepare_reads(st, R, true, s));
         int64_t nk = 0;
-        GT_HIP(hipMemcpyAsync(&nk, st->kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
+        GT_HIP(hipMemcpyAsync(&nk, st->own.kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
         GT_HIP(hipStreamSynchronize(s));
         if (nk <= 0) return GT_OK;
         DevBuf fw, rc, valid;
@@ -1293,9 +1373,9 @@ int gt_autotune(gt_storage* st, uint16_t K, int canonical, int mode, const uint8
                 // read-only passes: the claim (or lookup) pass writes only
                 // scratch; a later insert starts a new generation first.
                 if (st->kind == kExact) {
-                    launch_exact_lookup(L, R, st->exact_view(), st->fresh.as<uint8_t>(), ctr);
+                    launch_exact_lookup(L, R, st->exact_view(), st->own.fresh.as<uint8_t>(), ctr);
                 } else {
-                    launch_claim(L, R, st->tables_view(), claim_view(st), st->fresh.as<uint8_t>(), ctr);
+                    launch_claim(L, R, st->tables_view(), claim_view(st->own), st->own.fresh.as<uint8_t>(), ctr);
                 }
                 launch_query(L, R, st->tables_view(), st->exact_or_empty(), counts.as<int16_t>(), nullptr);
                 GT_HIP(hipEventRecord(e1, s));
@@ -1351,17 +1431,11 @@ int pipeline_enqueue(gt_pipeline* p, gt_pipeline::Slot& sl) {
                          p->canonical);
     R.dyn = sl.d_dyn.as<int64_t>();
     const int64_t n_grains = ((int64_t)p->max_bases + kTileGrain - 1) / kTileGrain;
-    launch_fill_u64(p->tile_read.as<uint64_t>(), p->tile_read.bytes / 8, 0, s);
-    launch_tile_read(R.off, R.n_reads, R.n_bases, p->tile_read.as<uint32_t>(), n_grains, R.dyn, s);
-    R.tile_read = p->tile_read.as<uint32_t>();
+    launch_fill_u64(p->sc.tile_read.as<uint64_t>(), p->sc.tile_read.bytes / 8, 0, s);
+    launch_tile_read(R.off, R.n_reads, R.n_bases, p->sc.tile_read.as<uint32_t>(), n_grains, R.dyn, s);
+    R.tile_read = p->sc.tile_read.as<uint32_t>();
     launch_fill_u64(sl.d_nnew.as<uint64_t>(), p->max_reads, 0, s);
-    ClaimMap cm{};
-    cm.keys = p->claim_keys.as<uint64_t>();
-    cm.vals = p->claim_vals.as<uint64_t>();
-    cm.cap = p->claim_cap;
-    cm.log2cap = log2u(p->claim_cap);
-    return enqueue_subbatch(st, make_launch(st, s), R, cm, p->fresh.as<uint8_t>(), sl.d_nnew.as<uint64_t>(), nullptr,
-                            s, false);
+    return enqueue_subbatch(st, p->sc, make_launch(st, s), R, sl.d_nnew.as<uint64_t>(), nullptr, s, false);
 }
 
 int pipeline_capture(gt_pipeline* p, gt_pipeline::Slot& sl) {
@@ -1369,14 +1443,16 @@ int pipeline_capture(gt_pipeline* p, gt_pipeline::Slot& sl) {
         (void)hipGraphExecDestroy(sl.exec);
         sl.exec = nullptr;
     }
-    const uint64_t gen_before = p->st->gen_issued, ub_before = p->st->ecount_ub;
+    const uint64_t gen_before = p->st->gen_issued, claim_before = p->sc.claim_gen_issued;
+    const uint64_t ub_before = p->st->ecount_ub;
     GT_HIP(hipStreamBeginCapture(p->compute, hipStreamCaptureModeThreadLocal));
     int rc = pipeline_enqueue(p, sl);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(p->compute, &graph);
-    // Capturing ran nothing: the generation bump and the set growth happen
+    // Capturing ran nothing: the generation bumps and the set growth happen
     // on replay and are counted there.
     p->st->gen_issued = gen_before;
+    p->sc.claim_gen_issued = claim_before;
     p->st->ecount_ub = ub_before;
     if (rc != GT_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -1445,21 +1521,10 @@ int gt_pipeline_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
         if (rc == GT_OK) rc = sl.d_dyn.ensure(2 * sizeof(int64_t));
         if (rc == GT_OK) rc = sl.d_nnew.ensure(max_reads * sizeof(uint64_t));
     }
-    if (rc == GT_OK) rc = p->tile_read.ensure(((max_bases + kTileGrain - 1) / kTileGrain + 1) * sizeof(uint32_t));
-    if (rc == GT_OK) rc = p->fresh.ensure((max_bases + kTileMax - 1) / kTileMax * kTileMax);
-    if (rc == GT_OK && st->kind != kExact) {
-        p->claim_cap = cap;
-        rc = p->claim_keys.ensure(cap * sizeof(uint64_t));
-        if (rc == GT_OK) rc = p->claim_vals.ensure(cap * sizeof(uint64_t));
-        if (rc == GT_OK) {
-            launch_fill_u64(p->claim_keys.as<uint64_t>(), cap, 0, p->compute);
-            launch_fill_u64(p->claim_vals.as<uint64_t>(), cap, 0, p->compute);
-        }
-    } else if (rc == GT_OK) {
-        p->claim_cap = 2;   // unused by the exact set; a valid view
-        rc = p->claim_keys.ensure(16);
-        if (rc == GT_OK) rc = p->claim_vals.ensure(16);
-    }
+    if (rc == GT_OK) rc = p->sc.tile_read.ensure(((max_bases + kTileGrain - 1) / kTileGrain + 1) * sizeof(uint32_t));
+    if (rc == GT_OK) rc = p->sc.fresh.ensure((max_bases + kTileMax - 1) / kTileMax * kTileMax);
+    if (rc == GT_OK) rc = alloc_claims(p->sc, st->kind != kExact ? cap : 2, p->compute);
+    p->sc.fixed = true;
     if (rc == GT_OK) hip_ok(hipStreamSynchronize(p->compute), "hipStreamSynchronize");
     if (rc != GT_OK) {
         pipeline_release(p);
@@ -1504,11 +1569,13 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
     GT_HIP(hipMemcpyAsync(sl.d_dyn.p, sl.h_dyn, 2 * sizeof(int64_t), hipMemcpyHostToDevice, p->copy));
     GT_HIP(hipEventRecord(sl.copied, p->copy));
     GT_HIP(hipStreamWaitEvent(p->compute, sl.copied, 0));
-    // The replay bumps the generation once; skip the values that would wrap
-    // the claim-map key field (next_gen clears the maps there).
-    if (((st->gen_issued + 1) & kGenKeyMask) == 0) GT_TRY(next_gen(st, p->compute));
+    // The replay bumps both generations once; skip the claim-map value that
+    // would wrap its key field (next_gen clears the map there).
+    if (((p->sc.claim_gen_issued + 1) & kGenKeyMask) == 0) GT_TRY(next_gen(st, p->sc, p->compute));
     if (!sl.exec || sl.epoch != st->epoch) GT_TRY(pipeline_capture(p, sl));
+    if (st->gen_issued + 2 >= 0xFFFFFFFFULL) return fail(GT_ERR_INTERNAL, "batch generation counter exhausted");
     st->gen_issued++;
+    p->sc.claim_gen_issued++;
     if (st->kind == kExact) st->ecount_ub += n_bases;
     GT_HIP(hipGraphLaunch(sl.exec, p->compute));
     GT_HIP(hipMemcpyAsync(sl.h_nnew, sl.d_nnew.p, n_reads * sizeof(uint64_t), hipMemcpyDeviceToHost, p->compute));
@@ -1540,6 +1607,55 @@ int gt_pipeline_free(gt_pipeline* p) {
     }
     pipeline_release(p);
     delete p;
+    return GT_OK;
+}
+
+// ------------------------------------------------------------ workspaces
+
+int gt_workspace_new(gt_workspace** out) {
+    if (!out) return fail(GT_ERR_INVALID_ARG, "null output");
+    GT_TRY(ensure_device());
+    *out = new (std::nothrow) gt_workspace();
+    if (!*out) return fail(GT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    return GT_OK;
+}
+
+int gt_workspace_free(gt_workspace* ws) {
+    if (!ws) return GT_OK;
+    (void)hipDeviceSynchronize();
+    delete ws;
+    return GT_OK;
+}
+
+int gt_dbg_insert_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                         const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint64_t* n_new,
+                         uint8_t* new_flags, void* stream) {
+    if (!valid_storage(st) || !ws) return fail(GT_ERR_INVALID_ARG, "invalid storage or workspace");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lw(ws->mu);
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    return dbg_insert_impl(st, ws->sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
+                           new_flags, (hipStream_t)stream);
+}
+
+int gt_dbg_query_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                        const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, int16_t* counts,
+                        uint64_t* hashes, void* stream) {
+    if (!valid_storage(st) || !ws) return fail(GT_ERR_INVALID_ARG, "invalid storage or workspace");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0 || n_bases == 0) return GT_OK;
+    std::lock_guard<std::recursive_mutex> lw(ws->mu);
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    GT_TRY(prepare_reads(ws->sc, R, true, s));
+    {
+        ProfScope ps("query", s);
+        launch_query(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
+    }
+    GT_HIP(hipGetLastError());
     return GT_OK;
 }
 

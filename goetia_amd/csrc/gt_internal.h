@@ -26,9 +26,9 @@ constexpr int kReadCache = 1024;    // read offsets cached in LDS per tile
 enum Kind : int { kBit = 1, kByte = 2, kExact = 3 };
 enum Mode : int { kRaw = 0, kSkipN = 1 };
 
-// Device counters of one storage (u64 each).  kGen is the first-touch
-// generation: bumped on the device once per sub-batch, so replays of a
-// captured batch never reuse a generation.
+// Device counters of one storage (u64 each).  kGen is the storage's
+// first-touch generation (exact-set stamps): bumped on the device once per
+// sub-batch, so replays of a captured batch never reuse a generation.
 enum Counter : int { kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kGen = 4, kNCounters = 5 };
 
 // Error bits OR-ed into counters[kErrors] by kernels.
@@ -57,9 +57,12 @@ struct ExactSet {
 
 // First-touch claim map for Bloom / count-min batches: key word =
 // gen:28 | table:4 | bin:32, value word = gen:32 | ~ordinal:32 (atomicMax).
+// The generation is a device counter owned with the map (a map may serve
+// several storages, so it cannot use a storage's generation).
 struct ClaimMap {
     uint64_t* keys;
     uint64_t* vals;
+    const uint64_t* gen;
     uint64_t cap;
     uint32_t log2cap;
 };

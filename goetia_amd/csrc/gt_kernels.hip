@@ -66,9 +66,14 @@ __device__ __forceinline__ uint64_t claim_slot(const ClaimMap& cm, uint64_t key)
     return ((key & ((1ULL << 36) - 1)) * kGolden) >> (64 - cm.log2cap);
 }
 
-// The generation of the running sub-batch (uniform across the grid).
+// The generation of the running sub-batch (uniform across the grid): the
+// storage's (exact-set stamps) or a claim map's.
 __device__ __forceinline__ uint32_t current_gen(const uint64_t* counters) {
     return (uint32_t)__hip_atomic_load(counters + kGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t claim_gen(const ClaimMap& cm) {
+    return (uint32_t)__hip_atomic_load(cm.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ void claim_insert(const ClaimMap& cm, uint64_t key, uint64_t stamp, uint64_t* errors) {
@@ -593,7 +598,7 @@ template <int PER>
 __global__ __launch_bounds__(kBlock) void claim_kernel(Reads R, Tables T, ClaimMap cm, uint8_t* fresh,
                                                      uint64_t* counters) {
     __shared__ TileShared sh;
-    ClaimOp op{T, cm, current_gen(counters), fresh, R.b0, R.canonical, counters + kErrors};
+    ClaimOp op{T, cm, claim_gen(cm), fresh, R.b0, R.canonical, counters + kErrors};
     walk_tiles<PER>(R, sh, op);
     add_counter(counters + kFresh, op.n_fresh);
 }
@@ -602,7 +607,7 @@ template <int PER>
 __global__ __launch_bounds__(kBlock) void apply_kernel(Reads R, Tables T, ClaimMap cm, const uint8_t* fresh,
                                                      uint64_t* n_new, uint8_t* new_flags, uint64_t* counters) {
     __shared__ TileShared sh;
-    ApplyOp op{T, cm, current_gen(counters), fresh, R.b0, R.canonical, R.kmer_off, new_flags, ReadAcc{n_new}};
+    ApplyOp op{T, cm, claim_gen(cm), fresh, R.b0, R.canonical, R.kmer_off, new_flags, ReadAcc{n_new}};
     walk_tiles<PER>(R, sh, op);
     add_counter(counters + kNUnique, op.n_new);
     add_counter(counters + kNOccupied, op.n_occ);
@@ -664,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, Exact
 __global__ __launch_bounds__(kBlock) void hash_claim_kernel(const uint64_t* h, uint64_t n, Tables T, ClaimMap cm,
                                                           uint8_t* fresh, uint64_t* counters) {
     uint64_t nf = 0;
-    const uint32_t gen = current_gen(counters);
+    const uint32_t gen = claim_gen(cm);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint8_t mask = 0;
         GT_FOR_TABLES(t, T) {
@@ -685,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void hash_apply_kernel(const uint64_t* h, u
                                                           const uint8_t* fresh, uint8_t* is_new,
                                                           uint64_t* counters) {
     uint64_t nn = 0, occ = 0;
-    const uint32_t gen = current_gen(counters);
+    const uint32_t gen = claim_gen(cm);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         const uint8_t mask = fresh[i];
         const uint64_t stamp = claim_stamp(gen, (uint32_t)i);
@@ -890,10 +895,13 @@ __global__ __launch_bounds__(kScanBlock) void scan_add_kernel(int64_t* kmer_off,
     }
 }
 
-// Starts a new first-touch generation (one lane; stream-ordered before the
-// sub-batch that uses it).
-__global__ void bump_gen_kernel(uint64_t* counters) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kGen] += 1;
+// Starts a new first-touch generation of a storage and of a claim map (one
+// lane; stream-ordered before the sub-batch that uses them).
+__global__ void bump_gen_kernel(uint64_t* storage_gen, uint64_t* claim_gen) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (storage_gen) *storage_gen += 1;
+        if (claim_gen) *claim_gen += 1;
+    }
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
@@ -1015,7 +1023,9 @@ void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, co
     GT_TILE_LAUNCH(L, exact_resolve_kernel, R, E, fresh, n_new, new_flags, counters);
 }
 
-void launch_bump_gen(uint64_t* counters, hipStream_t s) { bump_gen_kernel<<<1, 64, 0, s>>>(counters); }
+void launch_bump_gen(uint64_t* storage_gen, uint64_t* claim_gen, hipStream_t s) {
+    bump_gen_kernel<<<1, 64, 0, s>>>(storage_gen, claim_gen);
+}
 
 void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,
                   uint64_t* hashes) {

@@ -15,7 +15,7 @@ void launch_tile_read(const int64_t* off, uint64_t n_reads, int64_t n_bases, uin
                       int64_t n_tiles, const int64_t* dyn, hipStream_t s);
 void launch_kmer_offsets(const int64_t* off, uint64_t n_reads, int K, int64_t* kmer_off,
                          int64_t* block_sums, const int64_t* dyn, hipStream_t s);
-void launch_bump_gen(uint64_t* counters, hipStream_t s);
+void launch_bump_gen(uint64_t* storage_gen, uint64_t* claim_gen, hipStream_t s);
 uint64_t scan_block_count(uint64_t n_reads);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 

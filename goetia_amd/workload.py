@@ -56,3 +56,65 @@ def make(config, scale=1.0):
     g = genome(c['seed'], c['genome'])
     b = reads(c['seed'] + 1000, g, n, c['len'][0], c['len'][1], c['sub'], c['n_rate'])
     return c, b
+
+
+# ---------------------------------------------------------------- configurations
+
+def _scaled(n, scale, lo=1):
+    return max(lo, int(round(n * scale)))
+
+
+def bloom_100k(scale=1.0):
+    """1 Mbp genome, 100k x 150 bp reads at 1% substitutions, K=21,
+    BitStorage(10^7, 4), per-read n_new."""
+    g = genome(1, _scaled(1_000_000, scale, 10_000))
+    b = reads(1001, g, _scaled(100_000, scale), 150, 150, 0.01, 0.0)
+    return dict(K=21, storage='BitStorage', max_table=10_000_000, n_tables=4, mode=0), b
+
+
+def exact_mixed(scale=1.0):
+    """Uneven reads: 200k lognormal(median 8 kb, sigma 1.0) reads clipped to
+    150 bp..100 kb and 2M x 150 bp reads, interleaved in a seeded random
+    order, 5% substitutions, from a 50 Mbp genome; K=25 into the exact set."""
+    g = genome(3, _scaled(50_000_000, scale, 100_000))
+    n_long = _scaled(200_000, scale)
+    n_short = _scaled(2_000_000, scale)
+    lens = np.concatenate([lognormal_lengths(3, n_long, 8000, 1.0, 150, 100_000),
+                           np.full(n_short, 150, np.uint32)])
+    lens = lens[np.random.default_rng(33).permutation(len(lens))]
+    b = reads(1003, g, len(lens), 0, 0, 0.05, 0.0, lengths=lens)
+    return dict(K=25, storage='SparseppSetStorage', mode=0, n_long=n_long, n_short=n_short), b
+
+
+def repeat_genome(seed, length, unit_len, copies, snp_rate):
+    """A genome with `copies` copies of one repeat unit at random
+    non-overlapping sites, and a second haplotype with `snp_rate`
+    substitutions (SNP bubbles)."""
+    g = genome(seed, length).copy()
+    unit = genome(seed + 1, unit_len)
+    rng = np.random.default_rng(seed)
+    slots = length // unit_len
+    for site in rng.choice(slots, size=min(copies, slots), replace=False):
+        g[site * unit_len:(site + 1) * unit_len] = unit
+    hap2 = g.copy()
+    m = rng.random(length) < snp_rate
+    acgt = np.frombuffer(b'ACGT', np.uint8)
+    idx = np.searchsorted(acgt, hap2[m])
+    hap2[m] = acgt[(idx + rng.integers(1, 4, int(m.sum()))) % 4]
+    return g, hap2
+
+
+def neighbors_5m(scale=1.0):
+    """20 Mbp genome with 1,000 copies of a 2 kb repeat and a 0.1% SNP
+    haplotype; 5M x 150 bp reads (half from each haplotype) at 1% error;
+    K=31, ByteStorage(2^28, 4); decision nodes at count threshold 2."""
+    L = _scaled(20_000_000, scale, 200_000)
+    h1, h2 = repeat_genome(4, L, 2000, _scaled(1000, scale, 10), 0.001)
+    n = _scaled(5_000_000, scale)
+    a = reads(1004, h1, n // 2, 150, 150, 0.01, 0.0)
+    b = reads(2004, h2, n - n // 2, 150, 150, 0.01, 0.0)
+    seq = np.concatenate([a.seq, b.seq])
+    off = np.concatenate([a.offsets, b.offsets[1:] + a.offsets[-1]])
+    max_table = 2 ** 28 if scale >= 1.0 else max(100_000, int(2 ** 28 * scale))
+    return dict(K=31, storage='ByteStorage', max_table=max_table, n_tables=4, mode=0,
+                min_count=2), ReadBatch(seq, off)

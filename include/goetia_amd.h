@@ -119,6 +119,15 @@ int gt_storage_update_from(gt_storage* st, const gt_storage* other);
 /* Exact set contents (unordered) into host memory: writes min(n, cap) keys
  * and sets *n to the set size. */
 int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint64_t* n);
+/* Asynchronous copy of a Bloom / count-min storage's tables (reference
+ * layout, host_tables[i] of gt_storage_table_bytes(i) bytes) and its
+ * n_unique / n_occupied counters (counters[2]) to host memory on `stream`,
+ * and the reverse (host_tables NULL = empty tables and zero counters).  Used
+ * to evict a storage from HBM and bring it back; pinned host memory lets the
+ * copies overlap other work. */
+int gt_storage_save_async(gt_storage* st, uint8_t* const* host_tables, uint64_t* counters, void* stream);
+int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, const uint64_t* counters,
+                          void* stream);
 
 /* ------------------------------------------------------------ hashing */
 
@@ -249,6 +258,59 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
                        uint64_t* n_new);
 int gt_pipeline_flush(gt_pipeline* p);
 int gt_pipeline_free(gt_pipeline* p);
+
+/* ------------------------------------------------------------ workspaces */
+
+/* Device scratch (first-touch claim map, fresh masks, read layout) that many
+ * storages can share, one call at a time (no counterpart in goetia; the
+ * serving engine gives each of its concurrent slots one, so resident
+ * storages hold only their tables).  Calls through one workspace are
+ * serialised; use one workspace per concurrent stream. */
+typedef struct gt_workspace gt_workspace;
+int gt_workspace_new(gt_workspace** out);
+int gt_workspace_free(gt_workspace* ws);
+/* gt_dbg_insert_dev / gt_dbg_query_dev with the workspace's scratch. */
+int gt_dbg_insert_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canonical, int mode,
+                         const uint8_t* seq, const int64_t* offsets, uint64_t n_reads, uint64_t n_bases,
+                         uint64_t* n_new, uint8_t* new_flags, void* stream);
+int gt_dbg_query_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canonical, int mode,
+                        const uint8_t* seq, const int64_t* offsets, uint64_t n_reads, uint64_t n_bases,
+                        int16_t* counts, uint64_t* hashes, void* stream);
+
+/* ------------------------------------------------------------ serving */
+
+/* Serving engine (no counterpart in goetia; it serves many sample streams,
+ * each its own dBG, as goetia's per-sample processors would one at a time).
+ * All streams share one storage shape (Bloom or count-min tables of
+ * `tablesizes`); floor(hbm_budget / tables bytes) storages stay resident in
+ * HBM and streams are bound to them on demand, evicting the least recently
+ * used idle stream to pinned host memory.  A scheduler thread keeps n_slots
+ * chunks in flight (continuous batching): a chunk starts as soon as a slot is
+ * free and its stream is idle and resident, chunks of one stream run in
+ * submission order, and each stream's results equal inserting its chunks
+ * alone.  policy bit 0 starts runnable chunks of resident streams before
+ * older chunks that need a table load (a chunk waiting over 0.25 s goes
+ * first); policy 0 with n_slots = 1 is the plain one-chunk-at-a-time path. */
+typedef struct gt_server gt_server;
+typedef struct gt_server_stats {
+    uint64_t submitted, completed, loads, evictions, kmers, resident_slots, bytes_per_stream;
+    double latency_sum_s, elapsed_s;
+} gt_server_stats;
+const char* gt_server_last_error(void);
+int gt_server_new(int kind, const uint64_t* tablesizes, uint32_t n_tables, uint16_t K, int canonical, int mode,
+                  uint32_t n_streams, uint64_t hbm_budget, uint32_t n_slots, int policy, gt_server** out);
+/* Queues a chunk of host reads (copied before returning) for `stream`;
+ * n_new (host, n_reads entries, may be NULL) receives dBG::insert_sequence's
+ * per-read n_new when the chunk completes. */
+int gt_server_submit(gt_server* S, uint32_t stream, const uint8_t* seq, const int64_t* offsets,
+                     uint64_t n_reads, uint64_t* n_new, uint64_t* ticket);
+/* Blocks until the chunk completes; latency = completion - submission. */
+int gt_server_wait(gt_server* S, uint64_t ticket, double* latency_s);
+int gt_server_drain(gt_server* S);
+int gt_server_get_stats(gt_server* S, gt_server_stats* out);
+/* Table i and the counters of a stream, wherever they live (drains first). */
+int gt_server_stream_table(gt_server* S, uint32_t stream, uint32_t i, uint8_t* host_dst, uint64_t* counters);
+int gt_server_free(gt_server* S);
 
 /* ------------------------------------------------------------ timing */
 

@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
     assert len(names) >= 40
     for name in names:
         assert hasattr(lib, name), name
-    assert set(names) == set(N.SIGNATURES) | {'gt_last_error'}
+    assert set(names) == set(N.SIGNATURES) | {'gt_last_error', 'gt_server_last_error'}
     assert lib.gt_abi_version() == 1
 
 

@@ -1,0 +1,165 @@
+"""GPU parity for the benchmark configurations: full-size where the oracle
+finishes in seconds, 1/100 scale otherwise, and size-independent properties
+at full size (order independence of tables, n_unique = sum of n_new).
+Serving: every stream's results are the same alone and in the mix."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from goetia_amd import libgoetia as lg  # noqa: E402
+from goetia_amd import _native as N  # noqa: E402
+from goetia_amd import workload  # noqa: E402
+from goetia_amd.reads import DeviceReads, ReadBatch  # noqa: E402
+
+
+def sub(batch, a, b):
+    return ReadBatch(batch.seq[batch.offsets[a]:batch.offsets[b]], batch.offsets[a:b + 1] - batch.offsets[a])
+
+
+def test_bloom_config_full_size_vs_oracle(oracle):
+    cfg, batch = workload.bloom_100k(1.0)
+    K = cfg['K']
+    sizes = lg.get_n_primes_near_x(4, cfg['max_table'])
+    g = lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage(sizes), K)
+    n_new = g.insert_sequences(batch)
+    o = oracle.Storage(oracle.BIT, sizes)
+    _, o_new = o.insert_reads(batch.seq, batch.offsets, K, True, 0)
+    np.testing.assert_array_equal(n_new, o_new)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    assert int(n_new.sum()) == g.n_unique()
+
+
+def test_exact_mixed_small_vs_oracle(oracle):
+    cfg, batch = workload.exact_mixed(0.01)
+    K = cfg['K']
+    g = lg.dBG[lg.SparseppSetStorage, lg.CanLemireShifter].build(lg.SparseppSetStorage.build(), K)
+    n_new = g.insert_sequences(batch)
+    o = oracle.Storage(oracle.EXACT)
+    _, o_new = o.insert_reads(batch.seq, batch.offsets, K, True, 0)
+    np.testing.assert_array_equal(n_new, o_new)
+    np.testing.assert_array_equal(g.S.keys(), o.exact_keys())
+    assert g.n_unique() == o.n_unique == int(n_new.sum())
+
+
+def test_neighbors_small_vs_oracle(oracle):
+    cfg, batch = workload.neighbors_5m(0.01)
+    K, mc = cfg['K'], cfg['min_count']
+    sizes = lg.get_n_primes_near_x(4, cfg['max_table'])
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    n_new, flags = g.insert_sequences(batch, new_flags=True)
+    ind, outd = g.degrees(batch, min_count=mc, only=flags)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp_flags = []
+    for r in range(batch.n_reads):
+        _, _, fl = o.insert_sequence(batch.read(r), K, True, flags=True)
+        exp_flags.append(fl)
+    exp_flags = np.concatenate(exp_flags).astype(bool)
+    np.testing.assert_array_equal(flags, exp_flags)
+    ko = batch.kmer_offsets(K)
+    for r in range(0, batch.n_reads, 7):
+        ei, eo = o.degrees_sequence(batch.read(r), K, True, min_count=mc)
+        sel = exp_flags[ko[r]:ko[r + 1]]
+        np.testing.assert_array_equal(ind[ko[r]:ko[r + 1]][sel], ei[sel])
+        np.testing.assert_array_equal(outd[ko[r]:ko[r + 1]][sel], eo[sel])
+        assert np.all(ind[ko[r]:ko[r + 1]][~sel] == 0xFF)
+    dec = flags & ((ind > 1) | (outd > 1)) & (ind != 0xFF)
+    assert dec.sum() > 0   # the repeats and SNP bubbles do create decision nodes
+
+
+def test_cms_full_size_properties():
+    """The count-min workload at full size: tables do not depend on insert
+    order, n_unique = sum of per-read n_new, n_occupied = nonzero bins of
+    table 0, and every inserted k-mer queries >= 1."""
+    import torch
+    cfg, batch = workload.make('cms_20m', 1.0)
+    K = cfg['K']
+    d = batch.to_device()
+    g1 = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(cfg['max_table'], 4), K)
+    g1.mode = N.MODE_SKIP_N
+    nn = torch.zeros(batch.n_reads, dtype=torch.int64, device='cuda')
+    g1.insert_device(d, n_new=nn)
+    torch.cuda.synchronize()
+    assert int(nn.sum().item()) == g1.n_unique()
+    t1 = g1.get_raw()
+    assert int(np.count_nonzero(t1[0])) == g1.n_occupied()
+    nk = int(batch.kmer_counts(K).sum())
+    counts = torch.empty(nk, dtype=torch.int16, device='cuda')
+    g1.query_device(d, counts)
+    torch.cuda.synchronize()
+    valid = counts >= 0
+    assert int(valid.sum().item()) > 0.99 * nk
+    assert int((counts[valid] >= 1).sum().item()) == int(valid.sum().item())
+    del g1
+    # the same reads in 20 chunks, inserted last chunk first
+    g2 = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(cfg['max_table'], 4), K)
+    g2.mode = N.MODE_SKIP_N
+    edges = np.linspace(0, batch.n_reads, 21).astype(int)
+    for a, b in reversed(list(zip(edges[:-1], edges[1:]))):
+        b0, b1 = int(batch.offsets[a]), int(batch.offsets[b])
+        g2.insert_device(DeviceReads(d.seq[b0:b1], d.offsets[a:b + 1] - b0, b1 - b0))
+    torch.cuda.synchronize()
+    for x, y in zip(t1, g2.get_raw()):
+        assert np.array_equal(x, y)
+
+
+def test_workspace_shared_across_storages():
+    import ctypes as C
+    import torch
+    g = workload.genome(71, 50_000)
+    batches = [workload.reads(72 + i, g, 800, 80, 160, 0.02, 0.0) for i in range(6)]
+    ws = C.c_void_p()
+    N.call('gt_workspace_new', C.byref(ws))
+    try:
+        sizes = lg.get_n_primes_near_x(3, 50_000)
+        shared = [lg.ByteStorage(sizes), lg.BitStorage(sizes)]
+        alone = [lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), 21),
+                 lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage(sizes), 21)]
+        stream = torch.cuda.current_stream().cuda_stream
+        for i, b in enumerate(batches):
+            j = i % 2
+            d = b.to_device()
+            out = torch.zeros(b.n_reads, dtype=torch.int64, device='cuda')
+            N.call('gt_dbg_insert_ws_dev', shared[j].handle, ws, 21, 1, 0, N.ptr(d.seq), N.ptr(d.offsets),
+                   d.n_reads, d.n_bases, N.ptr(out), None, stream)
+            torch.cuda.synchronize()
+            exp = alone[j].insert_sequences(b)
+            np.testing.assert_array_equal(out.cpu().numpy().astype(np.uint64), exp)
+        for s, a in zip(shared, alone):
+            for x, y in zip(s.get_raw_tables(), a.get_raw()):
+                np.testing.assert_array_equal(x, y)
+    finally:
+        N.call('gt_workspace_free', ws)
+
+
+@pytest.mark.parametrize('slots,policy', [(1, 0), (4, 1), (3, 0)])
+def test_serving_results_same_alone_and_in_mix(slots, policy):
+    from goetia_amd.serving import Server, make_chunks, make_streams, run
+    n_streams, K = 6, 25
+    sizes = lg.get_n_primes_near_x(4, 300_000)
+    per_stream = sum(s // 8 + 1 for s in sizes)
+    genomes = make_streams(n_streams, seed=300, min_mbp=1, max_mbp=2, scale=0.05)
+    chunks = make_chunks(genomes, 40, seed=301, lo=50, hi=3000, median=400, sigma=1.0)
+    sv = Server(lg.BitStorage, sizes, K, n_streams, hbm_budget=2 * per_stream, n_slots=slots, policy=policy)
+    outs = []
+    for s, b in chunks:
+        outs.append(sv.submit(s, b))
+    lat = [sv.wait(t) for t, _ in outs]
+    sv.drain()
+    st = sv.stats()
+    assert st['completed'] == len(chunks) and st['resident_slots'] == 2
+    assert st['evictions'] > 0 and st['loads'] > 0   # the budget forces evictions
+    assert all(x >= 0 for x in lat)
+    # each stream alone, its chunks in order, through the plain dBG path
+    alone = {s: lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage(sizes), K) for s in range(n_streams)}
+    for (s, b), (_, out) in zip(chunks, outs):
+        exp = alone[s].insert_sequences(b)
+        np.testing.assert_array_equal(out[:b.n_reads], exp)
+    for s in range(n_streams):
+        tabs, ctr = sv.stream_tables(s)
+        for x, y in zip(tabs, alone[s].get_raw()):
+            np.testing.assert_array_equal(x, y)
+        assert ctr == [alone[s].n_unique(), alone[s].n_occupied()]
+    sv.close()
