@@ -156,6 +156,17 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f'{LIB_PATH} is missing: build it with `python -m goetia_amd.build` '
                           '(goetia_amd has no CPU fallback)')
+    # torch-ROCm ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    # libamdhip64.so.7) and links it by the unversioned name.  Loaded first,
+    # it satisfies this library's libamdhip64.so.7 dependency, so the process
+    # holds one HIP runtime.  Loaded the other way round, the dynamic linker
+    # maps /opt/rocm's runtime for us and then a second copy for torch: two
+    # HSA runtimes in one process, and torch's later device initialisation
+    # fails ("No HIP GPUs are available").
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name in ('gt_last_error', 'gt_server_last_error'):
         getattr(lib, name).restype = C.c_char_p

@@ -91,7 +91,8 @@ def test_cms_full_size_properties():
     torch.cuda.synchronize()
     valid = counts >= 0
     assert int(valid.sum().item()) > 0.99 * nk
-    assert int((counts[valid] >= 1).sum().item()) == int(valid.sum().item())
+    # 2.4e9 counts: more than torch's boolean indexing handles, so no counts[valid]
+    assert int((valid & (counts < 1)).sum().item()) == 0
     del g1
     # the same reads in 20 chunks, inserted last chunk first
     g2 = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(cfg['max_table'], 4), K)
