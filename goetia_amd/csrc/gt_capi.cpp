@@ -202,6 +202,10 @@ struct Scratch {
     uint64_t claim_cap = 0;
     uint64_t claim_gen_issued = 0;     // bumps enqueued so far (the device counter follows it)
     DevBuf fresh, tile_read, kmer_off, scan_blocks;
+    // binned insert: events of the two partition passes, first-pass digit
+    // offsets, second-pass counts / starts and their scan sums, new-window
+    // flags and the sub-batch's first packed window index
+    DevBuf bin_ent1, bin_ent2, bin_off1, bin_cnt2, bin_sums, bin_flags, bin_kbase;
     bool fixed = false;                // buffers preallocated and never moved (captured graphs)
 };
 
@@ -579,13 +583,136 @@ int enqueue_subbatch(gt_storage* st, Scratch& sc, const Launch& L, const Reads& 
     return GT_OK;
 }
 
+// ------------------------------------------------------------ binned insert
+
+// Buckets of the binned insert over a storage's tables (0: not eligible).
+uint32_t bin_buckets(const gt_storage* st) {
+    if ((st->kind != kByte && st->kind != kBit) || st->n_tables == 0 || st->n_tables > kBinMaxTables) return 0;
+    uint64_t nb = 0;
+    for (uint32_t i = 0; i < st->n_tables; ++i) nb += (st->sizes[i] + kBucketBins - 1) / kBucketBins;
+    return nb <= kMaxBuckets ? (uint32_t)nb : 0;
+}
+
+// Whether an insert goes through the binned path.  GOETIA_AMD_BINNED=0 / 1
+// forces it off / on (where the tables allow it); by default it is used for
+// batches of at least 2^20 bases with at least one base per 8 table bins,
+// where replaying buckets in LDS beats random atomics on the tables.
+bool use_binned(const gt_storage* st, const Scratch& sc, const Reads& R) {
+    if (R.dyn || sc.fixed || !bin_buckets(st)) return false;
+    const char* v = std::getenv("GOETIA_AMD_BINNED");
+    if (v && *v) return v[0] != '0';
+    uint64_t bins = 0;
+    for (uint64_t p : st->sizes) bins += p;
+    return (uint64_t)R.n_bases >= (1ULL << 20) && (uint64_t)R.n_bases * 8 >= bins;
+}
+
+// Binned sub-batch in start positions, a multiple of kTileMax: 2^27 (or
+// GOETIA_AMD_BIN_SUB, at most 2^28), capped by GOETIA_AMD_SUB_BATCH.
+uint64_t bin_sub_batch(uint64_t n_bases) {
+    uint64_t sb = 1ULL << 27;
+    if (const char* v = std::getenv("GOETIA_AMD_BIN_SUB")) {
+        const unsigned long long x = std::strtoull(v, nullptr, 10);
+        if (x) sb = std::min<uint64_t>(x, 1ULL << 28);
+    }
+    if (const uint64_t cap = sub_batch_cap()) sb = std::min(sb, cap);
+    sb = std::max<uint64_t>((sb + kTileMax - 1) / kTileMax * kTileMax, kTileMax);
+    return std::min<uint64_t>(sb, (n_bases + kTileMax - 1) / kTileMax * kTileMax);
+}
+
+// Binned insert of prepared reads (tile_read and kmer_off set): per
+// sub-batch, partition the (window, table) events by bin and replay each
+// bucket in LDS; then per-read new counts from the window flags.
+int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_new, uint8_t* new_flags,
+                       hipStream_t s) {
+    const uint32_t nt = st->n_tables;
+    BinPlan P{};
+    P.n_tables = nt;
+    uint32_t nb = 0;
+    for (uint32_t i = 0; i < nt; ++i) {
+        P.bfirst[i] = nb;
+        P.gbase[i] = nb << kBinBits;
+        nb += (uint32_t)((st->sizes[i] + kBucketBins - 1) / kBucketBins);
+    }
+    P.bfirst[nt] = nb;
+    P.n_buckets = nb;
+    uint32_t bits = 0;
+    while ((1u << bits) < nb) ++bits;
+    P.shift2 = bits > (uint32_t)kDigitBits1 ? bits - (uint32_t)kDigitBits1 : 0u;
+    P.n_d1 = (nb + (1u << P.shift2) - 1) >> P.shift2;
+
+    const uint64_t sub = bin_sub_batch((uint64_t)R.n_bases);
+    const uint64_t tiles_max = sub / kBinTile;
+    const uint64_t groups_max = (tiles_max + kBinGroup - 1) / kBinGroup;
+    const uint64_t n_ent = tiles_max * nt * kBinTile;
+    const uint64_t cnt_max = ((uint64_t)P.n_d1 << P.shift2) * groups_max;
+    GT_TRY(sc.bin_ent1.ensure(n_ent * sizeof(uint64_t)));
+    GT_TRY(sc.bin_ent2.ensure(n_ent * sizeof(uint64_t)));
+    GT_TRY(sc.bin_off1.ensure((kDigits1 + 1) * tiles_max * sizeof(uint16_t)));
+    GT_TRY(sc.bin_cnt2.ensure((cnt_max + 1) * sizeof(uint32_t)));
+    GT_TRY(sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t)));
+    GT_TRY(sc.bin_kbase.ensure(sizeof(uint64_t)));
+    uint8_t* flags = new_flags;
+    if (!flags) {
+        GT_TRY(sc.bin_flags.ensure((size_t)R.n_bases));
+        flags = sc.bin_flags.as<uint8_t>();
+    }
+    launch_zero_bytes(flags, R.kmer_off + R.n_reads, (uint64_t)R.n_bases, s);
+    P.ent1 = sc.bin_ent1.as<uint64_t>();
+    P.ent2 = sc.bin_ent2.as<uint64_t>();
+    P.off1 = sc.bin_off1.as<uint16_t>();
+    P.cnt2 = sc.bin_cnt2.as<uint32_t>();
+    P.kbase = sc.bin_kbase.as<uint64_t>();
+    P.flags = flags;
+
+    const Tables T = st->tables_view();
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    const int grid_l1 = std::max(1, g_dev.cus * 2);     // 70 KB of LDS per workgroup
+    const int grid = std::max(1, g_dev.cus * 8);
+    for (int64_t b0 = 0; b0 < R.n_bases; b0 += (int64_t)sub) {
+        Reads Rs = R;
+        Rs.b0 = b0;
+        Rs.b1 = std::min<int64_t>(R.n_bases, b0 + (int64_t)sub);
+        P.tile0 = b0 / kBinTile;
+        P.tiles1 = (uint32_t)((Rs.b1 - b0 + kBinTile - 1) / kBinTile);
+        P.n_groups = (P.tiles1 + kBinGroup - 1) / kBinGroup;
+        const uint64_t cnt_len = ((uint64_t)P.n_d1 << P.shift2) * P.n_groups;
+        launch_bin_kbase(R, b0, sc.bin_kbase.as<uint64_t>(), s);
+        {
+            ProfScope ps("bin_hash", s);
+            launch_bin_l1(Rs, T, P, grid_l1, s);
+        }
+        {
+            ProfScope ps("bin_count", s);
+            launch_bin_l2_count(P, grid, s);
+            launch_scan_u32(P.cnt2, cnt_len, sc.bin_sums.as<uint32_t>(), s);
+        }
+        {
+            ProfScope ps("bin_route", s);
+            launch_bin_l2_scatter(P, grid, s);
+        }
+        {
+            ProfScope ps("bin_apply", s);
+            launch_bin_apply(P, T, ctr, grid, s);
+        }
+        GT_HIP(hipGetLastError());
+    }
+    {
+        ProfScope ps("bin_new", s);
+        launch_bin_read_new(R.kmer_off, R.n_reads, flags, n_new, ctr, s);
+    }
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
 int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
     if (R.n_reads == 0 || R.n_bases == 0) {
         if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
         return GT_OK;
     }
-    GT_TRY(prepare_reads(sc, R, new_flags != nullptr, s));
+    const bool binned = use_binned(st, sc, R);
+    GT_TRY(prepare_reads(sc, R, new_flags != nullptr || binned, s));
     if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
+    if (binned) return binned_insert_impl(st, sc, R, n_new, new_flags, s);
     uint64_t sub = 0;
     GT_TRY(prepare_claims(sc, st->kind, st->n_tables, (uint64_t)R.n_bases, &sub, s));
     const Launch L = make_launch(st, s);

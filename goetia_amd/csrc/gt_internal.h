@@ -70,6 +70,46 @@ struct ClaimMap {
 constexpr uint64_t kMaxBins = 1ULL << 32;       // bins per table
 constexpr uint64_t kGenKeyMask = (1ULL << 28) - 1;
 
+// Binned insert of Bloom / count-min tables: the (table, bin) events of a
+// sub-batch are partitioned by bin in two stable passes (256-way, then up to
+// 512-way) into buckets of kBucketBins consecutive bins of one table, then
+// each bucket is replayed in event order against an LDS copy of its slice.
+constexpr int kBinBits = 13;
+constexpr int kBucketBins = 1 << kBinBits;        // bins per bucket
+constexpr int kDigitBits1 = 8;
+constexpr int kDigits1 = 1 << kDigitBits1;        // first-pass fan-out
+constexpr int kDigitBits2 = 9;
+constexpr int kDigits2 = 1 << kDigitBits2;        // largest second-pass fan-out
+constexpr int kBinPer = 8;                        // windows per thread in the first pass
+constexpr int kBinTile = kBlock * kBinPer;        // starts per first-pass tile
+constexpr uint32_t kBinMaxTables = 4;
+constexpr uint32_t kBinGroup = 1024;              // first-pass tiles per second-pass work item
+constexpr int kBinChunk = 4096;                   // events staged in LDS per second-pass step
+constexpr uint32_t kMaxBuckets = 1u << (kDigitBits1 + kDigitBits2);
+constexpr uint32_t kNoBin = 0xFFFFFFFFu;
+
+// One binned sub-batch.  An event is the u64 (global bin << 32 | ordinal):
+// global bin = gbase[table] + bin, ordinal = packed window index - *kbase.
+// Bucket = global bin >> kBinBits = d1 << shift2 | d2 (first / second pass
+// digits).
+struct BinPlan {
+    uint32_t n_tables;
+    uint32_t shift2;
+    uint32_t gbase[kBinMaxTables];
+    uint32_t bfirst[kBinMaxTables + 1];   // first bucket of each table; [n_tables] = n_buckets
+    uint32_t n_buckets;
+    uint32_t n_d1;                        // first-pass digits in use
+    int64_t tile0;                        // first first-pass tile (absolute) of the sub-batch
+    uint32_t tiles1;                      // first-pass tiles of the sub-batch
+    uint32_t n_groups;                    // ceil(tiles1 / kBinGroup)
+    uint64_t* ent1;                       // per tile a region of n_tables * kBinTile events, by d1
+    uint64_t* ent2;                       // events by bucket, event order within a bucket
+    uint16_t* off1;                       // (kDigits1 + 1) x tiles1, digit-major: start of d1 in its tile
+    uint32_t* cnt2;                       // (n_d1 << shift2) x n_groups (+1): counts, then exclusive starts
+    const uint64_t* kbase;                // packed index of the sub-batch's first window
+    uint8_t* flags;                       // per packed window: 1 = new k-mer
+};
+
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the
 // ordinal of the k-mer at s is s - b0.  With `dyn` set, n_reads and n_bases
 // are upper bounds and the actual sizes are read from device memory, so one

@@ -18,6 +18,8 @@
 #include "gt_internal.h"
 #include "gt_hash.h"
 
+#include <algorithm>
+
 namespace gt {
 
 __constant__ uint64_t c_char_hash[256] = GT_CHAR_HASH_INIT;
@@ -186,16 +188,20 @@ __device__ int16_t storage_query(const Tables& T, const ExactSet& E, uint64_t h)
 
 // ------------------------------------------------------------ tile walker
 
-struct TileShared {
+// LDS of the tile walker for tiles of up to TILE starts.
+template <int TILE>
+struct TileSharedT {
     uint64_t tf[256];                 // T[c]
     uint64_t tr[256];                 // T[complement(c)]
     int64_t roff[kReadCache + 1];
-    uint8_t bytes[kTileMax + kMaxK];
+    uint8_t bytes[TILE + kMaxK];
     uint32_t r_lo, r_hi;
     int cached;
 };
+using TileShared = TileSharedT<kTileMax>;
 
-__device__ __forceinline__ void load_char_tables(TileShared& sh) {
+template <class Sh>
+__device__ __forceinline__ void load_char_tables(Sh& sh) {
     for (int i = threadIdx.x; i < 256; i += kBlock) {
         sh.tf[i] = c_char_hash[i];
         sh.tr[i] = c_char_hash[gt_complement((unsigned char)i)];
@@ -217,9 +223,19 @@ __device__ __forceinline__ void batch_sizes(const Reads& R, uint64_t& n_reads, i
 // Visits every k-mer window whose start lies in [R.b0, R.b1).  For windows
 // inside a read, op(s, read, read_start, fw, rc, valid) is called in start
 // order per thread; valid is false for skip-N windows holding a non-ACGT byte.
-template <int PER, class Op>
-__device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
+// Ops may define end_tile(tile): called by every thread of the workgroup after
+// the tile's windows (uniform control flow, so it may synchronise).
+template <class Op>
+__device__ __forceinline__ auto call_end_tile(Op& op, int64_t tile, int) -> decltype(op.end_tile(tile), void()) {
+    op.end_tile(tile);
+}
+template <class Op>
+__device__ __forceinline__ void call_end_tile(Op&, int64_t, long) {}
+
+template <int PER, class Op, class Sh>
+__device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
     constexpr int kTile = kBlock * PER;
+    static_assert(sizeof(sh.bytes) >= (size_t)(kTile + kMaxK), "tile LDS too small for PER");
     const int tid = threadIdx.x;
     const int K = R.K;
     const unsigned Kr = (unsigned)K & 63u;
@@ -271,53 +287,70 @@ __device__ void walk_tiles(const Reads& R, TileShared& sh, Op& op) {
         const int64_t s0 = t0 + (int64_t)tid * PER;
         int64_t s_end = s0 + PER;
         if (s_end > b1) s_end = b1;
-        if (s0 >= s_end) continue;
-
-        auto OFF = [&](uint32_t r) -> int64_t { return cached ? sh.roff[r - rl] : R.off[r]; };
-        uint32_t lo = rl, hi = rh;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (OFF(mid) <= s0) lo = mid; else hi = mid - 1;
-        }
-        uint32_t r = lo;
-        int64_t rstart = OFF(r);
-        int64_t rend = OFF(r + 1);
-
-        const uint8_t* B = sh.bytes;
-        const int li = (int)(s0 - t0);
-        uint64_t fw = 0, rc = 0;
-        int bad = 0;
-        for (int k = 0; k < K; ++k) {
-            const uint8_t c = B[li + k];
-            fw = gt_rotl(fw, 1) ^ sh.tf[c];
-            bad += !gt_is_acgt(c);
-        }
-        for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
-
-        for (int j = 0; j < PER; ++j) {
-            const int64_t s = s0 + j;
-            if (s >= s_end) break;
-            if (j > 0) {
-                const uint8_t out = B[li + j - 1], in = B[li + j + K - 1];
-                // CyclicHash::update (fw) and reverse_update on complements (rc)
-                fw = gt_rotl(fw, 1) ^ gt_rotl(sh.tf[out], Kr) ^ sh.tf[in];
-                rc = gt_rotr(rc ^ gt_rotl(sh.tr[in], Kr) ^ sh.tr[out], 1);
-                bad += (int)!gt_is_acgt(in) - (int)!gt_is_acgt(out);
+        if (s0 < s_end) {
+            auto OFF = [&](uint32_t r) -> int64_t { return cached ? sh.roff[r - rl] : R.off[r]; };
+            uint32_t lo = rl, hi = rh;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (OFF(mid) <= s0) lo = mid; else hi = mid - 1;
             }
-            while (s >= rend && r < rh) {
-                ++r;
-                rstart = rend;
-                rend = OFF(r + 1);
+            uint32_t r = lo;
+            int64_t rstart = OFF(r);
+            int64_t rend = OFF(r + 1);
+
+            const uint8_t* B = sh.bytes;
+            const int li = (int)(s0 - t0);
+            uint64_t fw = 0, rc = 0;
+            int bad = 0;
+            for (int k = 0; k < K; ++k) {
+                const uint8_t c = B[li + k];
+                fw = gt_rotl(fw, 1) ^ sh.tf[c];
+                bad += !gt_is_acgt(c);
             }
-            if (s < rstart || s + K > rend) continue;
-            op(s, r, rstart, fw, rc, R.mode == kRaw || bad == 0);
+            for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
+
+            for (int j = 0; j < PER; ++j) {
+                const int64_t s = s0 + j;
+                if (s >= s_end) break;
+                if (j > 0) {
+                    const uint8_t out = B[li + j - 1], in = B[li + j + K - 1];
+                    // CyclicHash::update (fw) and reverse_update on complements (rc)
+                    fw = gt_rotl(fw, 1) ^ gt_rotl(sh.tf[out], Kr) ^ sh.tf[in];
+                    rc = gt_rotr(rc ^ gt_rotl(sh.tr[in], Kr) ^ sh.tr[out], 1);
+                    bad += (int)!gt_is_acgt(in) - (int)!gt_is_acgt(out);
+                }
+                while (s >= rend && r < rh) {
+                    ++r;
+                    rstart = rend;
+                    rend = OFF(r + 1);
+                }
+                if (s < rstart || s + K > rend) continue;
+                op(s, r, rstart, fw, rc, R.mode == kRaw || bad == 0);
+            }
+            op.flush();
         }
-        op.flush();
+        call_end_tile(op, tile, 0);
     }
 }
 
 __device__ __forceinline__ uint64_t kmer_value(uint64_t fw, uint64_t rc, int canonical) {
     return canonical ? (fw < rc ? fw : rc) : fw;
+}
+
+// Packed index of the first window starting at or after base x (x a multiple
+// of kTileGrain, or the end of the batch).  Packed indices of the windows
+// starting in [x, y) are then exactly [first_kidx(x), first_kidx(y)).
+__device__ int64_t first_kidx(const Reads& R, uint64_t n_reads, int64_t n_bases, int64_t x) {
+    if (x >= n_bases) return R.kmer_off[n_reads];
+    uint32_t r = R.tile_read[x / kTileGrain];
+    if (r >= n_reads) r = (uint32_t)(n_reads - 1);
+    while (r > 0 && R.off[r] > x) --r;
+    while (r + 1 < n_reads && R.off[r + 1] <= x) ++r;
+    const int64_t len = R.off[r + 1] - R.off[r];
+    const int64_t wins = len >= R.K ? len - R.K + 1 : 0;
+    int64_t d = x - R.off[r];
+    d = d < 0 ? 0 : (d > wins ? wins : d);
+    return R.kmer_off[r] + d;
 }
 
 // Per-read accumulation of new k-mers: one atomic per run of the same read.
@@ -504,26 +537,54 @@ struct ExactResolveOp {
 
 // query_sequence (dbg.hh:364-376): per window count (or -1 when skipped) and
 // hash value.
+// Counts are staged in LDS per tile and written out as one contiguous run
+// (the tile's windows have consecutive packed indices).
+struct QueryStage {
+    int16_t out[kTileMax];
+    int64_t k0, k1;                   // packed indices of the tile's windows
+};
+
 struct QueryOp {
     Tables T;
     ExactSet E;
-    int canonical;
-    const int64_t* kmer_off;
+    Reads R;
+    int tile_len;
     int16_t* counts;
     uint64_t* hashes;
+    QueryStage* st;
+    int64_t k0 = 0;
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
-        const int64_t k = kmer_off[r] + (s - rstart);
+        const int64_t k = R.kmer_off[r] + (s - rstart);
         if (!valid) {
-            if (counts) counts[k] = -1;
+            st->out[k - k0] = -1;
             if (hashes) hashes[k] = 0;
             return;
         }
-        const uint64_t h = kmer_value(fw, rc, canonical);
-        if (counts) counts[k] = storage_query(T, E, h);
+        const uint64_t h = kmer_value(fw, rc, R.canonical);
+        st->out[k - k0] = storage_query(T, E, h);
         if (hashes) hashes[k] = h;
     }
-    __device__ void begin_tile(int64_t) {}
+    __device__ void begin_tile(int64_t t0) {
+        if (threadIdx.x == 0) {
+            uint64_t n_reads;
+            int64_t n_bases;
+            batch_sizes(R, n_reads, n_bases);
+            int64_t t1 = t0 + tile_len;
+            if (t1 > R.b1) t1 = R.b1;
+            st->k0 = first_kidx(R, n_reads, n_bases, t0);
+            st->k1 = first_kidx(R, n_reads, n_bases, t1);
+        }
+        __syncthreads();
+        k0 = st->k0;
+    }
     __device__ void flush() {}
+    __device__ void end_tile(int64_t) {
+        __syncthreads();
+        if (counts) {
+            const int n = (int)(st->k1 - k0);
+            for (int i = threadIdx.x; i < n; i += kBlock) counts[k0 + i] = st->out[i];
+        }
+    }
 };
 
 // Hashing only: fw / rc / validity per window.
@@ -644,7 +705,8 @@ template <int PER>
 __global__ __launch_bounds__(kBlock) void query_kernel(Reads R, Tables T, ExactSet E, int16_t* counts,
                                                      uint64_t* hashes) {
     __shared__ TileShared sh;
-    QueryOp op{T, E, R.canonical, R.kmer_off, counts, hashes};
+    __shared__ QueryStage st;
+    QueryOp op{T, E, R, kBlock * PER, counts, hashes, &st};
     walk_tiles<PER>(R, sh, op);
 }
 
@@ -945,6 +1007,543 @@ __global__ void exact_keys_kernel(ExactSet E, uint64_t* out, uint64_t cap_out, u
     }
 }
 
+// ------------------------------------------------------------ binned insert
+//
+// Bloom / count-min insert of a large sub-batch without global atomics on the
+// tables.  Every (window, table) event is routed by its bin:
+//
+//   pass 1  hash the tile's windows, sort the tile's events by first digit in
+//           LDS and write them to the tile's own region (coalesced);
+//   pass 2  per (first digit, group of kBinGroup tiles): count, then scatter
+//           the events by second digit into bucket order (stable);
+//   apply   one wavefront per bucket of kBucketBins bins loads the table slice
+//           into LDS and replays the bucket's events in event order.
+//
+// Both partitions are stable, so a bucket holds its events in window order.
+// The replay takes 64 events at a time: lanes that hit the same bin are found
+// with one ballot per bin bit, and a lane's rank among them is the number of
+// earlier events of the chunk on that bin, so the value the bin held just
+// before the event is (slice value at chunk start) + rank.  That is goetia's
+// sequential insert, event by event (bytestorage.cc:60-113,
+// bitstorage.hh:195-219): the count becomes min(255, count + hits), an event
+// is new iff some table's bin held 0 just before it, and table-0 bins going
+// from 0 to nonzero count as occupied.
+
+// Mask of the lanes whose v equals this lane's (all 64 lanes must be active).
+template <int BITS>
+__device__ __forceinline__ uint64_t match_mask(uint32_t v) {
+    uint64_t m = ~0ULL;
+#pragma unroll
+    for (int k = 0; k < BITS; ++k) {
+        const bool bit = (v >> k) & 1u;
+        const uint64_t b = __ballot(bit);
+        m &= bit ? b : ~b;
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint64_t lanes_below() { return (1ULL << (threadIdx.x & 63)) - 1ULL; }
+
+// Mask of the active lanes of this wave whose key equals this lane's, built
+// in a per-wave LDS slot array indexed by key: clear, OR in the lane bit, read
+// back.  A wave's LDS instructions execute in issue order, so the three steps
+// need no barrier beyond keeping the compiler from moving them.
+__device__ __forceinline__ uint64_t match_lds(uint64_t* slots, uint32_t key, bool active) {
+    if (active) slots[key] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (active)
+        __hip_atomic_fetch_or(&slots[key], 1ULL << (threadIdx.x & 63), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t m = active ? slots[key] : 0ULL;
+    __builtin_amdgcn_wave_barrier();
+    return m;
+}
+
+// Packed index of the first window starting at or after b0 (one lane).
+__global__ void bin_kbase_kernel(const int64_t* off, const int64_t* kmer_off, uint64_t n_reads, int K, int64_t b0,
+                                 uint64_t* out) {
+    if (blockIdx.x || threadIdx.x) return;
+    uint64_t lo = 0, hi = n_reads - 1;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b0) lo = mid; else hi = mid - 1;
+    }
+    const int64_t len = off[lo + 1] - off[lo];
+    const int64_t wins = len >= K ? len - K + 1 : 0;
+    int64_t d = b0 - off[lo];
+    d = d < 0 ? 0 : (d > wins ? wins : d);
+    *out = (uint64_t)(kmer_off[lo] + d);
+}
+
+// Exclusive scan over the workgroup (one value per thread): a shuffle scan
+// per wave, then the wave totals through `part` (kBlock / 64 entries); the
+// workgroup total goes to *total.  Ends with a barrier, so `part` can be
+// reused at once.
+__device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* total = nullptr) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) part[wave] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t p = part[w];
+        off += w < wave ? p : 0u;
+        tot += p;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + x - v;
+}
+
+struct BinL1Shared {
+    uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
+    uint32_t ord[kBinTile];
+    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
+    uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
+    uint32_t part[kBlock];
+};
+static_assert(sizeof(int64_t) * (kReadCache + 1) >= sizeof(uint64_t) * 4 * kDigits1,
+              "the first pass borrows the walker's read-offset cache for its match slots");
+
+__device__ __forceinline__ uint32_t bin_d1(uint32_t g, uint32_t shift2) {
+    return g == kNoBin ? (uint32_t)kDigits1 : (g >> kBinBits) >> shift2;
+}
+
+struct BinL1Op {
+    Tables T;
+    BinPlan P;
+    const int64_t* kmer_off;
+    int canonical;
+    uint64_t kbase;
+    BinL1Shared* sh;
+    uint64_t* slots;          // 4 x kDigits1 match slots (the walker's read-offset cache, idle here)
+    int64_t t0 = 0;
+
+    __device__ void begin_tile(int64_t tile_base) {
+        t0 = tile_base;
+        const int w0 = threadIdx.x * kBinPer;
+        for (uint32_t t = 0; t < P.n_tables; ++t)
+            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
+    }
+
+    __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
+        if (!valid) return;
+        const int w = (int)(s - t0);
+        const uint64_t h = kmer_value(fw, rc, canonical);
+        sh->ord[w] = (uint32_t)((uint64_t)(kmer_off[r] + (s - rstart)) - kbase);
+#pragma unroll
+        for (uint32_t i = 0; i < kBinMaxTables; ++i)
+            if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + (uint32_t)gt_fastmod(h, T.p[i], T.m[i]);
+    }
+
+    __device__ void flush() {}
+
+    // Stable sort of the tile's events by first digit: each wave ranks a
+    // quarter of the events, the sorted order is built in LDS and the region
+    // is written out contiguously.
+    __device__ void end_tile(int64_t tile) {
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const uint64_t lt = lanes_below();
+        const int per_wave = (int)(P.n_tables * kBinTile) / 4;
+        const int e0 = wave * per_wave;
+        uint32_t* wc = sh->wpos[wave];
+        uint64_t* sl = slots + wave * kDigits1;
+        for (int d = lane; d <= kDigits1; d += 64) wc[d] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int c = 0; c < per_wave; c += 64) {
+            const uint32_t d = bin_d1(sh->g[e0 + c + lane], P.shift2);
+            if (d != (uint32_t)kDigits1) atomicAdd(&wc[d], 1u);
+        }
+        __syncthreads();
+        const uint32_t lt1 = (uint32_t)(tile - P.tile0);
+        const int d = threadIdx.x;                         // kDigits1 == kBlock
+        const uint32_t tot = (uint32_t)sh->wpos[0][d] + sh->wpos[1][d] + sh->wpos[2][d] + sh->wpos[3][d];
+        uint32_t n_ev;
+        const uint32_t start = block_exclusive_u32(tot, sh->part, &n_ev);
+        P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
+        if (d == kBlock - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
+        uint32_t p = start;
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t c = sh->wpos[w][d];
+            sh->wpos[w][d] = p;
+            p += c;
+        }
+        __syncthreads();
+        for (int c = 0; c < per_wave; c += 64) {
+            const int e = e0 + c + lane;
+            const uint32_t dd = bin_d1(sh->g[e], P.shift2);
+            const bool act = dd != (uint32_t)kDigits1;
+            const uint64_t m = match_lds(sl, dd, act);
+            if (act) sh->perm[wc[dd] + (uint32_t)__popcll(m & lt)] = (uint16_t)e;
+            __builtin_amdgcn_wave_barrier();
+            if (act && !(m & lt)) wc[dd] += (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        uint64_t* dst = P.ent1 + (size_t)lt1 * P.n_tables * kBinTile;
+        for (uint32_t i = threadIdx.x; i < n_ev; i += kBlock) {
+            const uint32_t e = sh->perm[i];
+            dst[i] = ((uint64_t)sh->g[e] << 32) | sh->ord[e & (kBinTile - 1)];
+        }
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
+    __shared__ TileSharedT<kBinTile> sh;
+    __shared__ BinL1Shared bs;
+    BinL1Op op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff)};
+    walk_tiles<kBinPer>(R, sh, op);
+}
+
+// Second pass: work item (d1, group) = the d1 runs of kBinGroup consecutive
+// first-pass tiles, concatenated in tile order.
+constexpr int kRunsPerThread = kBinGroup / kBlock;
+
+struct BinRuns {
+    uint32_t rstart[kBinGroup];       // position in ent1 of each tile's run
+    uint32_t rpre[kBinGroup + 1];     // exclusive prefix of the run lengths
+    uint32_t part[kBlock];
+};
+
+// Loads the runs of item (d1, grp); returns its event count.
+__device__ uint32_t bin_load_runs(const BinPlan& P, uint32_t d1, uint32_t grp, BinRuns& sh) {
+    uint32_t len[kRunsPerThread];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < kRunsPerThread; ++k) {
+        const uint32_t j = threadIdx.x * kRunsPerThread + k;
+        const uint32_t lt = grp * kBinGroup + j;
+        uint32_t st = 0;
+        len[k] = 0;
+        if (lt < P.tiles1) {
+            const uint32_t a = P.off1[(size_t)d1 * P.tiles1 + lt];
+            const uint32_t b = P.off1[(size_t)(d1 + 1) * P.tiles1 + lt];
+            st = lt * P.n_tables * kBinTile + a;
+            len[k] = b - a;
+        }
+        sh.rstart[j] = st;
+        tot += len[k];
+    }
+    uint32_t run = block_exclusive_u32(tot, sh.part);
+#pragma unroll
+    for (int k = 0; k < kRunsPerThread; ++k) {
+        sh.rpre[threadIdx.x * kRunsPerThread + k] = run;
+        run += len[k];
+    }
+    if (threadIdx.x == kBlock - 1) sh.rpre[kBinGroup] = run;
+    __syncthreads();
+    return sh.rpre[kBinGroup];
+}
+
+// Run holding item event i: the last k with rpre[k] <= i.
+__device__ __forceinline__ uint32_t bin_run_of(const BinRuns& sh, uint32_t i) {
+    uint32_t lo = 0, hi = kBinGroup - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (sh.rpre[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Events [c0, c0 + m) of the loaded item, m <= kBinChunk: one binary search per
+// 64 events (into `first`, 64 entries), then each lane steps forward from its
+// block's run; lanes read consecutive events.  f(i - c0, event i).
+template <class F>
+__device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& sh, uint32_t* first, uint32_t c0,
+                                               uint32_t m, F&& f) {
+    if (threadIdx.x < (unsigned)(kBinChunk / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+        const uint32_t i = c0 + j;
+        uint32_t k = first[j >> 6];
+        while (sh.rpre[k + 1] <= i) ++k;
+        f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
+    }
+}
+
+__device__ __forceinline__ uint32_t bin_d2(uint64_t e, uint32_t mask2) {
+    return ((uint32_t)(e >> 32) >> kBinBits) & mask2;
+}
+
+__global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
+    __shared__ BinRuns sh;
+    __shared__ uint32_t hist[kDigits2];
+    __shared__ uint32_t first[kBinChunk / 64];
+    const uint32_t n_items = P.n_d1 * P.n_groups;
+    const uint32_t mask2 = (1u << P.shift2) - 1u;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
+        __syncthreads();
+        for (int d = threadIdx.x; d < kDigits2; d += kBlock) hist[d] = 0;
+        const uint32_t n = bin_load_runs(P, d1, grp, sh);
+        for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
+            const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
+            bin_for_events(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
+            __syncthreads();
+        }
+        for (uint32_t d2 = threadIdx.x; d2 <= mask2; d2 += kBlock)
+            P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] = hist[d2];
+    }
+}
+
+struct BinL2Shared {
+    BinRuns runs;
+    uint64_t ev[kBinChunk];                 // the chunk's events, item order
+    uint16_t perm[kBinChunk];               // chunk position by second digit -> index in ev
+    uint32_t wpos[4][kDigits2 + 1];
+    uint64_t slots[4][kDigits2];            // per-wave match slots
+    uint32_t lstart[kDigits2 + 1];          // chunk-local start of each second digit
+    uint32_t cur[kDigits2];                 // next ent2 position of each second digit for this item
+    uint32_t first[kBinChunk / 64];         // run of the first event of each 64 in the chunk
+};
+
+// Stable scatter of each item by second digit, kBinChunk events at a time:
+// the chunk is staged in LDS, ranked per wave, sorted in LDS and written out
+// as one contiguous run per digit.
+__global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
+    __shared__ BinL2Shared sh;
+    const uint32_t n_items = P.n_d1 * P.n_groups;
+    const uint32_t mask2 = (1u << P.shift2) - 1u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanes_below();
+    uint32_t* wc = sh.wpos[wave];
+    uint64_t* sl = sh.slots[wave];
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
+        __syncthreads();
+        for (uint32_t d2 = threadIdx.x; d2 < (uint32_t)kDigits2; d2 += kBlock)
+            sh.cur[d2] = d2 <= mask2 ? P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] : 0u;
+        const uint32_t n = bin_load_runs(P, d1, grp, sh.runs);
+        for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
+            const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
+            bin_for_events(P, sh.runs, sh.first, c0, m, [&](uint32_t j, uint64_t e) { sh.ev[j] = e; });
+            for (int d = lane; d <= kDigits2; d += 64) wc[d] = 0;
+            __syncthreads();
+            const uint32_t per_wave = (m + 255u) / 256u * 64u;
+            const uint32_t i0 = (uint32_t)wave * per_wave;
+            for (uint32_t c = 0; c < per_wave; c += 64) {
+                const uint32_t j = i0 + c + (uint32_t)lane;
+                if (j < m) atomicAdd(&wc[bin_d2(sh.ev[j], mask2)], 1u);
+            }
+            __syncthreads();
+            // chunk-local digit starts (two digits per thread)
+            uint32_t a[2], tot = 0;
+            for (int k = 0; k < 2; ++k) {
+                const int dd = 2 * threadIdx.x + k;
+                a[k] = (uint32_t)sh.wpos[0][dd] + sh.wpos[1][dd] + sh.wpos[2][dd] + sh.wpos[3][dd];
+                tot += a[k];
+            }
+            uint32_t run = block_exclusive_u32(tot, sh.runs.part);
+            for (int k = 0; k < 2; ++k) {
+                const int dd = 2 * threadIdx.x + k;
+                sh.lstart[dd] = run;
+                uint32_t p = run;
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t c = sh.wpos[w][dd];
+                    sh.wpos[w][dd] = p;
+                    p += c;
+                }
+                run += a[k];
+            }
+            if (threadIdx.x == kBlock - 1) sh.lstart[kDigits2] = run;
+            __syncthreads();
+            for (uint32_t c = 0; c < per_wave; c += 64) {
+                const uint32_t j = i0 + c + (uint32_t)lane;
+                const bool act = j < m;
+                const uint32_t d = act ? bin_d2(sh.ev[j], mask2) : 0u;
+                const uint64_t mk = match_lds(sl, d, act);
+                if (act) sh.perm[wc[d] + (uint32_t)__popcll(mk & lt)] = (uint16_t)j;
+                __builtin_amdgcn_wave_barrier();
+                if (act && !(mk & lt)) wc[d] += (uint32_t)__popcll(mk);
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
+                const uint64_t e = sh.ev[sh.perm[i]];
+                const uint32_t d = bin_d2(e, mask2);
+                P.ent2[sh.cur[d] + (i - sh.lstart[d])] = e;
+            }
+            __syncthreads();
+            for (uint32_t dd = threadIdx.x; dd < (uint32_t)kDigits2; dd += kBlock)
+                sh.cur[dd] += sh.lstart[dd + 1] - sh.lstart[dd];
+        }
+    }
+}
+
+// Replay: one wavefront per bucket (four per workgroup).
+__global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, uint64_t* counters) {
+    __shared__ uint32_t slice[4][kBucketBins / 4];
+    __shared__ uint64_t slots[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanes_below();
+    const uint64_t kb = *P.kbase;
+    const bool bytes = T.kind == kByte;
+    uint32_t* sw = slice[wave];
+    uint8_t* sb = reinterpret_cast<uint8_t*>(sw);
+    uint64_t occ = 0;
+    for (uint32_t b = blockIdx.x * 4u + (uint32_t)wave; b < P.n_buckets; b += gridDim.x * 4u) {
+        const uint32_t lo_e = P.cnt2[(size_t)b * P.n_groups];
+        const uint32_t hi_e = P.cnt2[(size_t)(b + 1) * P.n_groups];
+        if (lo_e == hi_e) continue;
+        uint32_t t = 0;
+        uint64_t pt = T.p[0];
+        uint8_t* base = T.base[0];
+#pragma unroll
+        for (uint32_t i = 1; i < kBinMaxTables; ++i) {
+            if (i < P.n_tables && b >= P.bfirst[i]) {
+                t = i;
+                pt = T.p[i];
+                base = T.base[i];
+            }
+        }
+        const uint64_t bin0 = (uint64_t)(b - P.bfirst[t]) << kBinBits;
+        const uint64_t left = pt - bin0;
+        const uint32_t nb = (uint32_t)(left < (uint64_t)kBucketBins ? left : (uint64_t)kBucketBins);
+        const uint32_t nw = bytes ? (nb + 3u) / 4u : (nb + 31u) / 32u;
+        uint32_t* src = reinterpret_cast<uint32_t*>(bytes ? base + bin0 : base + bin0 / 8);
+        for (uint32_t w = lane; w < nw; w += 64) sw[w] = src[w];
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t c = lo_e; c < hi_e; c += 64) {
+            const uint32_t i = c + (uint32_t)lane;
+            const bool v = i < hi_e;
+            const uint64_t e = v ? P.ent2[i] : 0;
+            const uint32_t key = v ? ((uint32_t)(e >> 32) & (kBucketBins - 1)) : 0u;
+            // lanes on the same bin: low 8 bits through LDS slots, the rest by ballot
+            uint64_t m = match_lds(slots[wave], key & 255u, v);
+#pragma unroll
+            for (int k = 8; k < kBinBits; ++k) {
+                const bool bit = (key >> k) & 1u;
+                const uint64_t b = __ballot(bit);
+                m &= bit ? b : ~b;
+            }
+            const uint32_t rank = (uint32_t)__popcll(m & lt);
+            if (v) {
+                if (bytes) {
+                    const uint32_t pre = sb[key];
+                    if (pre + rank == 0) P.flags[kb + (uint32_t)e] = 1;
+                    if (rank == 0) {
+                        const uint32_t post = pre + (uint32_t)__popcll(m);
+                        sb[key] = (uint8_t)(post < 255u ? post : 255u);
+                        occ += (t == 0 && pre == 0);
+                    }
+                } else {
+                    const uint32_t bit = 1u << (key & 31u);
+                    const bool pre = (sw[key >> 5] & bit) != 0;
+                    if (!pre && rank == 0) {
+                        P.flags[kb + (uint32_t)e] = 1;
+                        atomicOr(&sw[key >> 5], bit);
+                        occ += (t == 0);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        for (uint32_t w = lane; w < nw; w += 64) src[w] = sw[w];
+        __builtin_amdgcn_wave_barrier();
+    }
+    add_counter(counters + kNOccupied, occ);
+}
+
+// Per-read new k-mers from the window flags; adds the total to n_unique.  A
+// workgroup takes kBlock consecutive reads; their flags are one contiguous
+// range, staged in LDS with coalesced loads when it fits.
+constexpr int kReadNewStage = 32768;
+
+__global__ __launch_bounds__(kBlock) void bin_read_new_kernel(const int64_t* kmer_off, uint64_t n_reads,
+                                                            const uint8_t* flags, uint64_t* n_new,
+                                                            uint64_t* counters) {
+    __shared__ uint8_t fl[kReadNewStage];
+    uint64_t tot = 0;
+    for (uint64_t r0 = (uint64_t)blockIdx.x * kBlock; r0 < n_reads; r0 += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t r1 = r0 + kBlock < n_reads ? r0 + kBlock : n_reads;
+        const int64_t k0 = kmer_off[r0], k1 = kmer_off[r1];
+        const bool staged = k1 - k0 <= kReadNewStage;
+        __syncthreads();
+        if (staged)
+            for (int64_t i = threadIdx.x; i < k1 - k0; i += kBlock) fl[i] = flags[k0 + i];
+        __syncthreads();
+        const uint64_t r = r0 + threadIdx.x;
+        if (r < r1) {
+            uint64_t c = 0;
+            const int64_t a = kmer_off[r], b = kmer_off[r + 1];
+            if (staged) {
+                for (int64_t k = a; k < b; ++k) c += fl[k - k0];
+            } else {
+                for (int64_t k = a; k < b; ++k) c += flags[k];
+            }
+            if (n_new) n_new[r] += c;
+            tot += c;
+        }
+    }
+    add_counter(counters + kNUnique, tot);
+}
+
+// Zeroes p[0, *n).
+__global__ void zero_bytes_kernel(uint8_t* p, const int64_t* n) {
+    const uint64_t len = (uint64_t)*n;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Exclusive scan of a u32 array in place, with the total stored at a[n].
+constexpr int kScanU32Per = 16;
+constexpr int kScanU32Chunk = kBlock * kScanU32Per;
+
+
+__global__ __launch_bounds__(kBlock) void scan_u32_sums_kernel(const uint32_t* a, uint64_t n, uint32_t* sums) {
+    __shared__ uint32_t part[kBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanU32Chunk + (uint64_t)threadIdx.x * kScanU32Per;
+    uint32_t s = 0;
+    for (int j = 0; j < kScanU32Per; ++j) s += base + j < n ? a[base + j] : 0u;
+    uint32_t tot;
+    block_exclusive_u32(s, part, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_u32_blocks_kernel(uint32_t* sums, uint64_t nb, uint32_t* total) {
+    __shared__ uint32_t part[kBlock];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < nb; base += kBlock) {
+        const uint64_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t x = block_exclusive_u32(v, part, &tot);
+        if (i < nb) sums[i] = carry + x;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_u32_apply_kernel(uint32_t* a, uint64_t n, const uint32_t* sums) {
+    __shared__ uint32_t part[kBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanU32Chunk + (uint64_t)threadIdx.x * kScanU32Per;
+    uint32_t v[kScanU32Per];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanU32Per; ++j) {
+        v[j] = base + j < n ? a[base + j] : 0u;
+        s += v[j];
+    }
+    uint32_t run = block_exclusive_u32(s, part) + sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanU32Per; ++j) {
+        if (base + j < n) a[base + j] = run;
+        run += v[j];
+    }
+}
+
 }  // namespace gt
 
 // ------------------------------------------------------------ launchers
@@ -1091,6 +1690,57 @@ void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8
 void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s) {
     const uint64_t g = (E.cap + 255) / 256;
     exact_keys_kernel<<<(unsigned)(g < 16384 ? g : 16384), 256, 0, s>>>(E, out, cap_out, n);
+}
+
+}  // namespace gt
+
+// ------------------------------------------------------------ binned insert launchers
+
+namespace gt {
+
+void launch_bin_kbase(const Reads& R, int64_t b0, uint64_t* out, hipStream_t s) {
+    bin_kbase_kernel<<<1, 64, 0, s>>>(R.off, R.kmer_off, R.n_reads, R.K, b0, out);
+}
+
+void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(P.tiles1, 1), grid);
+    bin_l1_kernel<<<g, kBlock, 0, s>>>(R, T, P);
+}
+
+void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((int64_t)P.n_d1 * P.n_groups, 1), grid);
+    bin_l2_count_kernel<<<g, kBlock, 0, s>>>(P);
+}
+
+void launch_bin_l2_scatter(const BinPlan& P, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((int64_t)P.n_d1 * P.n_groups, 1), grid);
+    bin_l2_scatter_kernel<<<g, kBlock, 0, s>>>(P);
+}
+
+void launch_bin_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((P.n_buckets + 3) / 4, 1), grid);
+    bin_apply_kernel<<<g, kBlock, 0, s>>>(P, T, counters);
+}
+
+void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_t* flags, uint64_t* n_new,
+                         uint64_t* counters, hipStream_t s) {
+    const uint64_t g = (n_reads + kBlock - 1) / kBlock;
+    bin_read_new_kernel<<<(unsigned)(g < 4096 ? (g ? g : 1) : 4096), kBlock, 0, s>>>(kmer_off, n_reads, flags, n_new,
+                                                                                  counters);
+}
+
+void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {
+    const uint64_t g = (max_n + 255) / 256;
+    zero_bytes_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(p, n);
+}
+
+uint64_t scan_u32_block_count(uint64_t n) { return (n + kScanU32Chunk - 1) / kScanU32Chunk; }
+
+void launch_scan_u32(uint32_t* a, uint64_t n, uint32_t* sums, hipStream_t s) {
+    const uint64_t nb = scan_u32_block_count(n);
+    if (nb) scan_u32_sums_kernel<<<(unsigned)nb, kBlock, 0, s>>>(a, n, sums);
+    scan_u32_blocks_kernel<<<1, kBlock, 0, s>>>(sums, nb, a + n);
+    if (nb) scan_u32_apply_kernel<<<(unsigned)nb, kBlock, 0, s>>>(a, n, sums);
 }
 
 }  // namespace gt

@@ -54,4 +54,17 @@ void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8
                              const Tables& T, int16_t* counts, uint8_t* is_new, uint64_t* counters, hipStream_t s);
 void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s);
 
+// Binned insert of Bloom / count-min tables (one sub-batch = kbase, l1,
+// l2_count, scan, l2_scatter, apply; read_new once per call).
+void launch_bin_kbase(const Reads& R, int64_t b0, uint64_t* out, hipStream_t s);
+void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, hipStream_t s);
+void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s);
+void launch_bin_l2_scatter(const BinPlan& P, int grid, hipStream_t s);
+void launch_bin_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int grid, hipStream_t s);
+void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_t* flags, uint64_t* n_new,
+                         uint64_t* counters, hipStream_t s);
+void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
+uint64_t scan_u32_block_count(uint64_t n);
+void launch_scan_u32(uint32_t* a, uint64_t n, uint32_t* sums, hipStream_t s);
+
 }  // namespace gt

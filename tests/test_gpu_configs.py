@@ -69,6 +69,34 @@ def test_neighbors_small_vs_oracle(oracle):
     assert dec.sum() > 0   # the repeats and SNP bubbles do create decision nodes
 
 
+@pytest.mark.parametrize('kind', ['ByteStorage', 'BitStorage'])
+def test_binned_equals_atomic_insert(kind, monkeypatch):
+    """The count-min workload at 1/20 scale (120M k-mers, 2^28-bin tables):
+    the binned insert and the atomic insert give identical tables, per-read
+    n_new and counters."""
+    import torch
+    cfg, batch = workload.make('cms_20m', 0.05)
+    K = cfg['K']
+    d = batch.to_device()
+    out = []
+    for mode in ('0', '1'):
+        monkeypatch.setenv('GOETIA_AMD_BINNED', mode)
+        cls = getattr(lg, kind)
+        g = lg.dBG[cls, lg.CanLemireShifter].build(cls.build(cfg['max_table'], 4), K)
+        g.mode = N.MODE_SKIP_N
+        nn = torch.zeros(batch.n_reads, dtype=torch.int64, device='cuda')
+        g.insert_device(d, n_new=nn)
+        torch.cuda.synchronize()
+        out.append((nn.cpu().numpy(), g.get_raw(), g.n_unique(), g.n_occupied()))
+        del g
+    (n0, t0, u0, o0), (n1, t1, u1, o1) = out
+    np.testing.assert_array_equal(n0, n1)
+    assert (u0, o0) == (u1, o1)
+    assert u0 == int(n0.sum())
+    for a, b in zip(t0, t1):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_cms_full_size_properties():
     """The count-min workload at full size: tables do not depend on insert
     order, n_unique = sum of per-read n_new, n_occupied = nonzero bins of

@@ -107,10 +107,12 @@ def test_hash_many_tiny_reads():
 
 # ------------------------------------------------------------------ storages
 
-def test_storages_vs_reference(golden, golden_meta):
+@pytest.mark.parametrize('binned', ['0', '1'])
+def test_storages_vs_reference(golden, golden_meta, binned, monkeypatch):
     """Per-read n_new, byte-identical tables, counters and query counts for
     every golden run (Bloom, count-min, exact set; canonical and forward;
-    raw and skip-N)."""
+    raw and skip-N), through the atomic and the binned insert."""
+    monkeypatch.setenv('GOETIA_AMD_BINNED', binned)
     for name, meta in golden_meta['runs'].items():
         if 'byteiq' in name:
             continue
@@ -132,11 +134,16 @@ def test_storages_vs_reference(golden, golden_meta):
             np.testing.assert_array_equal(q, golden[f'{name}__query'])
 
 
+@pytest.mark.parametrize('binned', ['0', '1'])
 @pytest.mark.parametrize('kind', [1, 2, 3])
 @pytest.mark.parametrize('sub_batch', [0, 4096, 3 * 4096])
-def test_new_flags_and_sub_batches_vs_oracle(oracle, kind, sub_batch, monkeypatch):
+def test_new_flags_and_sub_batches_vs_oracle(oracle, kind, sub_batch, binned, monkeypatch):
     """Per-k-mer new flags match the sequential oracle, also when the batch
-    is split into several first-touch sub-batches and the exact set grows."""
+    is split into several first-touch sub-batches and the exact set grows
+    (Bloom and count-min through both insert paths)."""
+    if binned == '1' and kind == 3:
+        pytest.skip('the exact set has one insert path')
+    monkeypatch.setenv('GOETIA_AMD_BINNED', binned)
     if sub_batch:
         monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', str(sub_batch))
     rng = np.random.default_rng(kind + sub_batch)
@@ -305,3 +312,36 @@ def test_reset_clone_update_from_and_oxli(tmp_path):
     assert c.n_occupied() == g.n_occupied()
     g.reset()
     assert all(t.max() == 0 for t in g.get_raw())
+
+
+@pytest.mark.parametrize('kind', [1, 2])
+@pytest.mark.parametrize('sub_batch', [0, 8192 * 5])
+def test_binned_insert_vs_oracle(oracle, kind, sub_batch, monkeypatch):
+    """The binned insert on tables spanning more than 512 buckets (two-level
+    routing), with hot repeated k-mers (ranks within one 64-event chunk and
+    counts past 255), N bases and several sub-batches: tables, per-read n_new,
+    flags and counters equal the sequential oracle's."""
+    from goetia_amd import workload
+    monkeypatch.setenv('GOETIA_AMD_BINNED', '1')
+    if sub_batch:
+        monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', str(sub_batch))
+    genome = workload.genome(31 + kind, 40_000)
+    gen = workload.reads(32 + kind, genome, 3000, 40, 300, 0.01, 0.002)
+    reads = [gen.read(r) for r in range(gen.n_reads)] + [b'ACGTTGCA' * 40] * 60 + [b'A' * 200] * 40
+    batch = ReadBatch.from_strings(reads)
+    sizes = oracle.primes_near_x(4, 3_000_000 if kind == 2 else 9_000_000)
+    store = (lg.ByteStorage if kind == 2 else lg.BitStorage)(sizes)
+    g = lg.dBG[type(store), lg.CanLemireShifter].build(store, 25)
+    g.mode = N.MODE_SKIP_N
+    n_new, flags = g.insert_sequences(batch, new_flags=True)
+    o = oracle.Storage(kind, sizes)
+    o_new, o_flags = [], []
+    for r in reads:
+        _, nn, fl = o.insert_sequence(r, 25, True, oracle.MODE_SKIP_N, flags=True)
+        o_new.append(nn)
+        o_flags.append(fl)
+    np.testing.assert_array_equal(n_new, o_new)
+    np.testing.assert_array_equal(flags, np.concatenate(o_flags).astype(bool))
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
