@@ -5,15 +5,16 @@ Workload (the reference's count-min configuration at full size): a seeded
 0.01% N bases (k-mers spanning N are skipped), canonical k = 31, ByteStorage
 with 4 tables of the primes below 2^28 (1.07 GB of saturating 8-bit
 counters).  One step = reset the tables, insert every read (per-read n_new,
-exact first-touch semantics) and query_sequence every read (per-k-mer
-counts).  Inputs are resident in HBM before the timed region.
+exact first-touch semantics; the binned insert) and query_sequence every read
+(per-k-mer counts).  Inputs are resident in HBM before the timed region.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line: k-mers/s = hashed k-mers / (insert + query time), the
 roofline of the dominant kernel (algorithmic bytes per k-mer over its
-HIP-event-timed duration), the plain chunk-at-a-time latency and the CPU
-oracle timed on a bounded sample of the same workload.
+HIP-event-timed duration, with the per-kernel breakdown), the one-pass
+insert_and_query rate on the same reads, the plain chunk-at-a-time latency
+and the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -27,13 +28,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic HBM bytes per k-mer: one 64-byte granule per table touched;
-# a read-modify-write counts the granule twice.
-BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64}
+# Kernels of one step: the binned insert (hash + first partition pass, second
+# pass count + scatter, per-bucket replay, per-read new counts) or, for small
+# batches, the atomic insert (claim + apply), then the query.
+KERNELS = ['bin_hash', 'bin_count', 'bin_route', 'bin_apply', 'bin_new', 'claim', 'apply', 'query']
+# Algorithmic HBM bytes per k-mer.  Random table accesses count one 64-byte
+# granule per table (twice for a read-modify-write).  The binned passes move
+# 8-byte events, one per table per k-mer: write once when routed, read once
+# by the next pass; the replay also reads and writes each table slice once per
+# sub-batch (2 x 1.07 GB per 2^27 starts, 16 B per k-mer).
+BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64,
+                  'bin_hash': 4 * 8, 'bin_count': 4 * 8, 'bin_route': 2 * 4 * 8,
+                  'bin_apply': 4 * 8 + 16, 'bin_new': 1}
 # HBM bytes per k-mer (read + write) measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload and scaled by a 1 GiB copy (the table and the
 # commands are in profiles/cms_20m_hbm_traffic.md).
-MEASURED_BYTES_PER_KMER = {'claim': 562 + 52, 'apply': 572 + 257, 'query': 513 + 32}
+MEASURED_BYTES_PER_KMER = {'bin_hash': 36.2, 'bin_count': 37.1, 'bin_route': 83.7, 'bin_apply': 64.0, 'bin_new': 1.2,
+                           'query': 515.3}
 
 
 def log(*a):
@@ -50,6 +61,7 @@ def parse():
     p.add_argument('--cpu-sample-reads', type=int, default=500_000)
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-chunks', action='store_true')
+    p.add_argument('--no-iq', action='store_true', help='skip the one-pass insert_and_query timing')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
     p.add_argument('--workload', default='cms_20m',
@@ -143,7 +155,7 @@ def main():
     n_kmers = int((counts >= 0).sum().item())
     total_new = int(n_new.sum().item())
     log(f'{n_kmers} k-mers per step, {total_new} new, n_unique={g.n_unique()}')
-    kt = kernel_times(N, ['claim', 'apply', 'query'])
+    kt = {k: v for k, v in kernel_times(N, KERNELS).items() if v[0]}
     log('kernel ms/step: ' + ', '.join(f'{k} {v[1] / args.steps:.1f} ({v[0]} launches)' for k, v in kt.items()))
 
     value = n_kmers * args.steps / elapsed
@@ -158,7 +170,24 @@ def main():
                 'algorithmic_bytes_per_launch': round(BYTES_PER_KMER[dom] * n_kmers / launches),
                 'launches_per_step': launches,
                 'bytes_per_kmer': BYTES_PER_KMER[dom],
-                'kernel_ms_per_step': {k: round(v[1] / args.steps, 2) for k, v in kt.items()}}
+                'kernel_ms_per_step': {k: round(v[1] / args.steps, 2) for k, v in kt.items()},
+                'kernel_gbps': {k: round(n_kmers * BYTES_PER_KMER[k] / (v[1] / args.steps / 1e3) / 1e9, 1)
+                                for k, v in kt.items()}}
+
+    # One-pass insert_and_query (dbg.hh:327-340 semantics: each k-mer's count
+    # just after its own insert), same reads, reset tables.
+    iq_rate = None
+    if not args.no_iq:
+        iq_times = []
+        for _ in range(2):
+            store.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.insert_and_query_device(reads, counts, stream=stream)
+            torch.cuda.synchronize()
+            iq_times.append(time.perf_counter() - t0)
+        iq_rate = round(n_kmers / min(iq_times), 1)
+        log(f'insert_and_query: {min(iq_times) * 1e3:.1f} ms, {iq_rate / 1e9:.2f} G k-mers/s')
 
     chunk = None
     if not args.no_chunks:
@@ -203,6 +232,7 @@ def main():
                    'reads': batch.n_reads, 'kmers_per_step': n_kmers, 'K': K,
                    'storage': 'ByteStorage', 'tablesizes': store.get_tablesizes()},
         'roofline': roofline,
+        'insert_and_query_kmers_per_s': iq_rate,
         'chunk_latency': chunk,
         'cpu_baseline': cpu,
     }

@@ -205,7 +205,7 @@ struct Scratch {
     // binned insert: events of the two partition passes, first-pass digit
     // offsets, second-pass counts / starts and their scan sums, new-window
     // flags and the sub-batch's first packed window index
-    DevBuf bin_ent1, bin_ent2, bin_off1, bin_cnt2, bin_sums, bin_flags, bin_kbase;
+    DevBuf bin_ent1, bin_ent2, bin_off1, bin_cnt2, bin_sums, bin_flags, bin_kbase, bin_vq;
     bool fixed = false;                // buffers preallocated and never moved (captured graphs)
 };
 
@@ -621,9 +621,10 @@ uint64_t bin_sub_batch(uint64_t n_bases) {
 
 // Binned insert of prepared reads (tile_read and kmer_off set): per
 // sub-batch, partition the (window, table) events by bin and replay each
-// bucket in LDS; then per-read new counts from the window flags.
+// bucket in LDS; then per-read new counts from the window flags, or, with
+// iq_counts (count-min only), the insert_and_query count of every window.
 int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_new, uint8_t* new_flags,
-                       hipStream_t s) {
+                       int16_t* iq_counts, hipStream_t s) {
     const uint32_t nt = st->n_tables;
     BinPlan P{};
     P.n_tables = nt;
@@ -651,18 +652,26 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     GT_TRY(sc.bin_cnt2.ensure((cnt_max + 1) * sizeof(uint32_t)));
     GT_TRY(sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t)));
     GT_TRY(sc.bin_kbase.ensure(sizeof(uint64_t)));
+    const bool iq = iq_counts != nullptr;
     uint8_t* flags = new_flags;
-    if (!flags) {
-        GT_TRY(sc.bin_flags.ensure((size_t)R.n_bases));
-        flags = sc.bin_flags.as<uint8_t>();
+    if (iq) {
+        GT_TRY(sc.bin_vq.ensure((size_t)R.n_bases * kBinMaxTables));
+        launch_zero_words(sc.bin_vq.as<uint32_t>(), R.kmer_off + R.n_reads, (uint64_t)R.n_bases, s);
+        flags = nullptr;
+    } else {
+        if (!flags) {
+            GT_TRY(sc.bin_flags.ensure((size_t)R.n_bases));
+            flags = sc.bin_flags.as<uint8_t>();
+        }
+        launch_zero_bytes(flags, R.kmer_off + R.n_reads, (uint64_t)R.n_bases, s);
     }
-    launch_zero_bytes(flags, R.kmer_off + R.n_reads, (uint64_t)R.n_bases, s);
     P.ent1 = sc.bin_ent1.as<uint64_t>();
     P.ent2 = sc.bin_ent2.as<uint64_t>();
     P.off1 = sc.bin_off1.as<uint16_t>();
     P.cnt2 = sc.bin_cnt2.as<uint32_t>();
     P.kbase = sc.bin_kbase.as<uint64_t>();
     P.flags = flags;
+    P.vq = iq ? sc.bin_vq.as<uint8_t>() : nullptr;
 
     const Tables T = st->tables_view();
     uint64_t* ctr = st->counters.as<uint64_t>();
@@ -696,7 +705,11 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
         }
         GT_HIP(hipGetLastError());
     }
-    {
+    if (iq) {
+        ProfScope ps("bin_iq", s);
+        launch_bin_iq_combine(sc.bin_vq.as<uint32_t>(), R.kmer_off + R.n_reads, (uint64_t)R.n_bases, nt, iq_counts,
+                              ctr, s);
+    } else {
         ProfScope ps("bin_new", s);
         launch_bin_read_new(R.kmer_off, R.n_reads, flags, n_new, ctr, s);
     }
@@ -712,7 +725,7 @@ int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8
     const bool binned = use_binned(st, sc, R);
     GT_TRY(prepare_reads(sc, R, new_flags != nullptr || binned, s));
     if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
-    if (binned) return binned_insert_impl(st, sc, R, n_new, new_flags, s);
+    if (binned) return binned_insert_impl(st, sc, R, n_new, new_flags, nullptr, s);
     uint64_t sub = 0;
     GT_TRY(prepare_claims(sc, st->kind, st->n_tables, (uint64_t)R.n_bases, &sub, s));
     const Launch L = make_launch(st, s);
@@ -1262,9 +1275,16 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
     std::lock_guard<std::recursive_mutex> lk(st->mu);
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
-    if (st->kind == kByte) {
+    if (st->kind == kByte && bin_buckets(st)) {
         // Count-min results depend on the exact order of every earlier insert
-        // of the same bins: hash the batch, then apply it in k-mer order.
+        // of the same bins: the binned replay gives each event its bins'
+        // counts just before it.
+        GT_TRY(prepare_reads(st, R, true, s));
+        return binned_insert_impl(st, st->own, R, nullptr, nullptr, counts, s);
+    }
+    if (st->kind == kByte) {
+        // More than four tables or 2^30 bins: hash the batch, then apply it in
+        // k-mer order on one lane.
         GT_TRY(prepare_reads(st, R, true, s));
         int64_t nk = 0;
         GT_HIP(hipMemcpyAsync(&nk, st->own.kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));

@@ -108,6 +108,8 @@ struct BinPlan {
     uint32_t* cnt2;                       // (n_d1 << shift2) x n_groups (+1): counts, then exclusive starts
     const uint64_t* kbase;                // packed index of the sub-batch's first window
     uint8_t* flags;                       // per packed window: 1 = new k-mer
+    uint8_t* vq;                          // insert_and_query: per packed window, kBinMaxTables
+                                          // counts just after the event (flags unused then)
 };
 
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the

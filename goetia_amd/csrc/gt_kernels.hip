@@ -1428,7 +1428,14 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
             if (v) {
                 if (bytes) {
                     const uint32_t pre = sb[key];
-                    if (pre + rank == 0) P.flags[kb + (uint32_t)e] = 1;
+                    if (P.vq) {
+                        // insert_and_query (bytestorage.cc:142-150): this table's
+                        // count just after the event
+                        const uint32_t after = pre + rank + 1;
+                        P.vq[(kb + (uint32_t)e) * kBinMaxTables + t] = (uint8_t)(after < 255u ? after : 255u);
+                    } else if (pre + rank == 0) {
+                        P.flags[kb + (uint32_t)e] = 1;
+                    }
                     if (rank == 0) {
                         const uint32_t post = pre + (uint32_t)__popcll(m);
                         sb[key] = (uint8_t)(post < 255u ? post : 255u);
@@ -1484,6 +1491,39 @@ __global__ __launch_bounds__(kBlock) void bin_read_new_kernel(const int64_t* kme
         }
     }
     add_counter(counters + kNUnique, tot);
+}
+
+// insert_and_query results from the per-table counts after each event:
+// count = min over tables (the reference returns 1 for a new k-mer, and a
+// k-mer is new exactly when that minimum is 1); windows without events (N
+// holding, skipped) read -1.  Adds the new k-mers to n_unique.
+__global__ void bin_iq_combine_kernel(const uint32_t* vq, const int64_t* nk, uint32_t n_tables, int16_t* counts,
+                                      uint64_t* counters) {
+    const uint64_t n = (uint64_t)*nk;
+    const uint32_t mask = n_tables >= 4 ? 0xFFFFFFFFu : ((1u << (8 * n_tables)) - 1u);
+    uint64_t n_new = 0;
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t w = vq[k] & mask;
+        int16_t c = -1;
+        if (w) {
+            uint32_t mn = 255;
+            for (uint32_t t = 0; t < n_tables; ++t) {
+                const uint32_t b = (w >> (8 * t)) & 0xFFu;
+                mn = b < mn ? b : mn;
+            }
+            c = (int16_t)mn;
+            n_new += mn == 1;
+        }
+        if (counts) counts[k] = c;
+    }
+    add_counter(counters + kNUnique, n_new);
+}
+
+// Zeroes p[0, *n) (32-bit words).
+__global__ void zero_words_kernel(uint32_t* p, const int64_t* n) {
+    const uint64_t len = (uint64_t)*n;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
 }
 
 // Zeroes p[0, *n).
@@ -1732,6 +1772,18 @@ void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_
 void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {
     const uint64_t g = (max_n + 255) / 256;
     zero_bytes_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(p, n);
+}
+
+void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,
+                           uint64_t* counters, hipStream_t s) {
+    const uint64_t g = (max_n + 255) / 256;
+    bin_iq_combine_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(vq, nk, n_tables, counts,
+                                                                                   counters);
+}
+
+void launch_zero_words(uint32_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {
+    const uint64_t g = (max_n + 255) / 256;
+    zero_words_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(p, n);
 }
 
 uint64_t scan_u32_block_count(uint64_t n) { return (n + kScanU32Chunk - 1) / kScanU32Chunk; }

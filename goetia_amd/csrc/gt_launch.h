@@ -64,6 +64,9 @@ void launch_bin_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int
 void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_t* flags, uint64_t* n_new,
                          uint64_t* counters, hipStream_t s);
 void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
+void launch_zero_words(uint32_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
+void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,
+                           uint64_t* counters, hipStream_t s);
 uint64_t scan_u32_block_count(uint64_t n);
 void launch_scan_u32(uint32_t* a, uint64_t n, uint32_t* sums, hipStream_t s);
 

@@ -345,3 +345,28 @@ def test_binned_insert_vs_oracle(oracle, kind, sub_batch, monkeypatch):
     assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
     for a, b in zip(g.get_raw(), o.tables()):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize('sub_batch', [0, 8192 * 3])
+@pytest.mark.parametrize('n_tables', [1, 3, 4])
+def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatch):
+    """Count-min insert_and_query through the binned replay: every window's
+    count just after its own insert (1 when new), with hot k-mers past 255,
+    skip-N windows (-1) and several sub-batches, equals the sequential
+    oracle; counters and tables too."""
+    from goetia_amd import workload
+    if sub_batch:
+        monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', str(sub_batch))
+    genome = workload.genome(41 + n_tables, 20_000)
+    gen = workload.reads(42 + n_tables, genome, 1500, 30, 250, 0.01, 0.003)
+    reads = [gen.read(r) for r in range(gen.n_reads)] + [b'ACGTTGCA' * 30] * 50 + [b'C' * 150] * 30
+    sizes = oracle.primes_near_x(n_tables, 200_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), 21)
+    g.mode = N.MODE_SKIP_N
+    got = g.insert_and_query_sequences(ReadBatch.from_strings(reads))
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp = np.concatenate([o.insert_and_query_sequence(r, 21, True, oracle.MODE_SKIP_N) for r in reads])
+    np.testing.assert_array_equal(got, exp)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
