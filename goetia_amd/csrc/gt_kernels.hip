@@ -1158,9 +1158,13 @@ struct BinL1Op {
         uint64_t* sl = slots + wave * kDigits1;
         for (int d = lane; d <= kDigits1; d += 64) wc[d] = 0;
         __builtin_amdgcn_wave_barrier();
-        for (int c = 0; c < per_wave; c += 64) {
-            const uint32_t d = bin_d1(sh->g[e0 + c + lane], P.shift2);
-            if (d != (uint32_t)kDigits1) atomicAdd(&wc[d], 1u);
+        for (int c = 0; c < per_wave; c += 4 * 64) {        // per_wave is a multiple of 512
+            uint32_t d[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[e0 + c + q * 64 + lane], P.shift2);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (d[q] != (uint32_t)kDigits1) atomicAdd(&wc[d[q]], 1u);
         }
         __syncthreads();
         const uint32_t lt1 = (uint32_t)(tile - P.tile0);
@@ -1189,9 +1193,18 @@ struct BinL1Op {
         }
         __syncthreads();
         uint64_t* dst = P.ent1 + (size_t)lt1 * P.n_tables * kBinTile;
-        for (uint32_t i = threadIdx.x; i < n_ev; i += kBlock) {
-            const uint32_t e = sh->perm[i];
-            dst[i] = ((uint64_t)sh->g[e] << 32) | sh->ord[e & (kBinTile - 1)];
+        for (uint32_t i0 = 0; i0 < n_ev; i0 += 4 * kBlock) {
+            uint32_t e[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                e[q] = i < n_ev ? sh->perm[i] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                if (i < n_ev) dst[i] = ((uint64_t)sh->g[e[q]] << 32) | sh->ord[e[q] & (kBinTile - 1)];
+            }
         }
     }
 };
@@ -1255,17 +1268,41 @@ __device__ __forceinline__ uint32_t bin_run_of(const BinRuns& sh, uint32_t i) {
 
 // Events [c0, c0 + m) of the loaded item, m <= kBinChunk: one binary search per
 // 64 events (into `first`, 64 entries), then each lane steps forward from its
-// block's run; lanes read consecutive events.  f(i - c0, event i).
-template <class F>
+// block's run; lanes read consecutive events.  f(i - c0, event i).  Batched:
+// a thread's kBinChunk / kBlock loads are all issued before the first use
+// (more registers, fewer serialised memory latencies).
+template <bool kBatched, class F>
 __device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& sh, uint32_t* first, uint32_t c0,
                                                uint32_t m, F&& f) {
     if (threadIdx.x < (unsigned)(kBinChunk / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
-        const uint32_t i = c0 + j;
-        uint32_t k = first[j >> 6];
-        while (sh.rpre[k + 1] <= i) ++k;
-        f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
+    if (!kBatched) {
+        for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+            const uint32_t i = c0 + j;
+            uint32_t k = first[j >> 6];
+            while (sh.rpre[k + 1] <= i) ++k;
+            f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
+        }
+        return;
+    }
+    // all of a thread's loads are issued before any is used
+    constexpr int Q = kBinChunk / kBlock;
+    uint64_t v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+        v[q] = 0;
+        if (j < m) {
+            const uint32_t i = c0 + j;
+            uint32_t k = first[j >> 6];
+            while (sh.rpre[k + 1] <= i) ++k;
+            v[q] = P.ent1[sh.rstart[k] + (i - sh.rpre[k])];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+        if (j < m) f(j, v[q]);
     }
 }
 
@@ -1286,7 +1323,7 @@ __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
         const uint32_t n = bin_load_runs(P, d1, grp, sh);
         for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
             const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
-            bin_for_events(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
+            bin_for_events<false>(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
             __syncthreads();
         }
         for (uint32_t d2 = threadIdx.x; d2 <= mask2; d2 += kBlock)
@@ -1324,7 +1361,7 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
         const uint32_t n = bin_load_runs(P, d1, grp, sh.runs);
         for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
             const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
-            bin_for_events(P, sh.runs, sh.first, c0, m, [&](uint32_t j, uint64_t e) { sh.ev[j] = e; });
+            bin_for_events<true>(P, sh.runs, sh.first, c0, m, [&](uint32_t j, uint64_t e) { sh.ev[j] = e; });
             for (int d = lane; d <= kDigits2; d += 64) wc[d] = 0;
             __syncthreads();
             const uint32_t per_wave = (m + 255u) / 256u * 64u;
@@ -1409,12 +1446,38 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
         const uint32_t nb = (uint32_t)(left < (uint64_t)kBucketBins ? left : (uint64_t)kBucketBins);
         const uint32_t nw = bytes ? (nb + 3u) / 4u : (nb + 31u) / 32u;
         uint32_t* src = reinterpret_cast<uint32_t*>(bytes ? base + bin0 : base + bin0 / 8);
-        for (uint32_t w = lane; w < nw; w += 64) sw[w] = src[w];
+        for (uint32_t w0 = 0; w0 < nw; w0 += 8 * 64) {
+            uint32_t x[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t w = w0 + (uint32_t)q * 64 + (uint32_t)lane;
+                x[q] = w < nw ? src[w] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t w = w0 + (uint32_t)q * 64 + (uint32_t)lane;
+                if (w < nw) sw[w] = x[q];
+            }
+        }
         __builtin_amdgcn_wave_barrier();
+        // events are loaded four chunks ahead
+        constexpr int kAhead = 4;
+        uint64_t pf[kAhead];
+#pragma unroll
+        for (int q = 0; q < kAhead; ++q) {
+            const uint32_t i = lo_e + (uint32_t)q * 64 + (uint32_t)lane;
+            pf[q] = i < hi_e ? P.ent2[i] : 0;
+        }
         for (uint32_t c = lo_e; c < hi_e; c += 64) {
             const uint32_t i = c + (uint32_t)lane;
             const bool v = i < hi_e;
-            const uint64_t e = v ? P.ent2[i] : 0;
+            const uint64_t e = pf[0];
+#pragma unroll
+            for (int q = 0; q + 1 < kAhead; ++q) pf[q] = pf[q + 1];
+            {
+                const uint32_t nx = c + (uint32_t)kAhead * 64 + (uint32_t)lane;
+                pf[kAhead - 1] = nx < hi_e ? P.ent2[nx] : 0;
+            }
             const uint32_t key = v ? ((uint32_t)(e >> 32) & (kBucketBins - 1)) : 0u;
             // lanes on the same bin: low 8 bits through LDS slots, the rest by ballot
             uint64_t m = match_lds(slots[wave], key & 255u, v);
