@@ -21,6 +21,7 @@ GT_ERR_UNSUPPORTED = 5
 GT_ERR_NO_DEVICE = 6
 GT_ERR_IO = 7
 GT_ERR_INTERNAL = 8
+GT_ERR_INVALID_CHARACTER = 9
 
 BIT, BYTE, EXACT = 1, 2, 3
 MODE_RAW, MODE_SKIP_N = 0, 1
@@ -42,6 +43,14 @@ class UninitializedShifterException(GoetiaException):
     """goetia::UninitializedShifterException (hashshifter.hh:133-145)."""
 
 
+class InvalidCharacterException(GoetiaException):
+    """goetia::InvalidCharacterException (alphabets.hh:41-55)."""
+
+
+class GoetiaFileException(GoetiaException, OSError):
+    """goetia::GoetiaFileException (goetia.hh): a file could not be read."""
+
+
 class DeviceError(GoetiaException):
     """The GPU path failed or no GPU is available."""
 
@@ -57,8 +66,9 @@ _ERRORS = {
     GT_ERR_DEVICE: DeviceError,
     GT_ERR_UNSUPPORTED: UnsupportedError,
     GT_ERR_NO_DEVICE: DeviceError,
-    GT_ERR_IO: OSError,
+    GT_ERR_IO: GoetiaFileException,
     GT_ERR_INTERNAL: GoetiaException,
+    GT_ERR_INVALID_CHARACTER: InvalidCharacterException,
 }
 
 u8p = C.POINTER(C.c_uint8)
@@ -137,6 +147,14 @@ SIGNATURES = {
     'gt_profile_get': [C.c_char_p, u64p, C.POINTER(C.c_double)],
     'gt_gen_genome': [u64, u64, vp],
     'gt_gen_reads': [u64, vp, u64, u64, vp, u32, u32, C.c_double, C.c_double, vp, vp, u64p],
+    'gt_fastx_open': [C.c_char_p, cint, u32, C.POINTER(vp)],
+    'gt_fastx_next_batch': [vp, u64, u64, vp, vp, vp, u64, vp, u64p],
+    'gt_fastx_stats': [vp, u64p, u64p, u64p, C.POINTER(cint)],
+    'gt_fastx_close': [vp],
+    'gt_inserter_new': [st_p, C.c_uint16, cint, cint, u64, u64, C.POINTER(vp)],
+    'gt_inserter_advance': [vp, vp, u64p, u64p, C.POINTER(cint)],
+    'gt_inserter_stats': [vp, u64p, u64p, u64p],
+    'gt_inserter_free': [vp],
 }
 
 _lib = None

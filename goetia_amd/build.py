@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libgoetia_amd.so')
 ARCH = os.environ.get('GOETIA_AMD_ARCH', 'gfx950')
 
-SOURCES = ['gt_kernels.hip', 'gt_capi.cpp', 'gt_serving.cpp']
+SOURCES = ['gt_kernels.hip', 'gt_capi.cpp', 'gt_serving.cpp', 'gt_fastx.cpp']
 HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-Wall', '-Wno-unused-function',
@@ -49,7 +49,7 @@ def build(verbose=False):
             raise RuntimeError(f'hipcc failed on {src}')
         objs.append(obj)
     tmp = LIB + '.tmp'
-    cmd = [hipcc(), '-shared', f'--offload-arch={ARCH}', '-o', tmp, *objs]
+    cmd = [hipcc(), '-shared', f'--offload-arch={ARCH}', '-o', tmp, *objs, '-lz']
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

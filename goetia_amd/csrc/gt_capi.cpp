@@ -807,9 +807,16 @@ int check_host_offsets(const int64_t* offsets, uint64_t n_reads) {
 
 // ============================================================ C ABI
 
+namespace gt {
+// Sets this thread's gt_last_error() message from the library's other
+// translation units; returns code.
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace gt
+
 extern "C" {
 
 const char* gt_last_error(void) { return g_err.c_str(); }
+
 
 int gt_abi_version(void) { return GT_ABI_VERSION; }
 

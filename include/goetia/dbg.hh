@@ -80,6 +80,8 @@ class dBG {
 
     // GT_MODE_RAW (goetia's behaviour) or GT_MODE_SKIP_N.
     void set_input_mode(int mode) { _mode = mode; }
+    int input_mode() const { return _mode; }
+    static constexpr int is_canonical() { return ShifterType::is_canonical ? 1 : 0; }
 
     // ------------------------------------------------------------ per k-mer
     const hash_type hash(const std::string& kmer) const { return _shifter.hash(kmer); }

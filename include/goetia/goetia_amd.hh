@@ -54,6 +54,12 @@ class UninitializedShifterException : public GoetiaException {
     UninitializedShifterException() : GoetiaException("Shifter used without hash_base") {}
 };
 
+// alphabets.hh:41-55
+class InvalidCharacterException : public GoetiaException {
+  public:
+    using GoetiaException::GoetiaException;
+};
+
 class DeviceException : public GoetiaException {
   public:
     using GoetiaException::GoetiaException;
@@ -67,6 +73,7 @@ inline void check(int rc, const char* what) {
     switch (rc) {
         case GT_ERR_SEQUENCE_LENGTH: throw SequenceLengthException(msg);
         case GT_ERR_IO: throw GoetiaFileException(msg);
+        case GT_ERR_INVALID_CHARACTER: throw InvalidCharacterException(msg);
         case GT_ERR_INVALID_ARG: throw std::invalid_argument(msg);
         case GT_ERR_OUT_OF_MEMORY: throw std::bad_alloc();
         case GT_ERR_DEVICE:

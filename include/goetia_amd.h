@@ -48,6 +48,7 @@ extern "C" {
 #define GT_ERR_NO_DEVICE       6
 #define GT_ERR_IO              7   /* GoetiaFileException, bitstorage.cc:186 */
 #define GT_ERR_INTERNAL        8
+#define GT_ERR_INVALID_CHARACTER 9 /* InvalidCharacterException, alphabets.hh:41-55 */
 
 #define GT_BIT_STORAGE   1         /* BitStorage, bitstorage.hh:93 */
 #define GT_BYTE_STORAGE  2         /* ByteStorage, bytestorage.hh:96 */
@@ -324,6 +325,51 @@ int gt_server_free(gt_server* S);
 int gt_profile_enable(int on);
 int gt_profile_reset(void);
 int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
+
+/* ------------------------------------------------------------ FASTX input */
+
+/* goetia's FastxParser<DNA_SIMPLE> (readers.hh:95-232; construction
+ * readers.cc:14-33): FASTA / FASTQ, plain or gzip, parsed record for record
+ * as the vendored kseq.h does, each sequence validated by DNA_SIMPLE
+ * (alphabets.hh:112-130: a/c/g/t uppercased, any other byte outside A/C/G/T
+ * rejects the record).  Rejected records are skipped and counted (n_skipped);
+ * with strict set the first one ends the call with GT_ERR_INVALID_CHARACTER
+ * (InvalidCharacterException) after the records before it.  Records shorter
+ * than min_length are skipped.  A FASTQ record whose quality length differs
+ * from its sequence (InvalidRead, readers.hh:205-208) is skipped and counted
+ * in n_invalid_reads (and n_skipped, as goetia counts it).  Host only. */
+typedef struct gt_fastx gt_fastx;
+int gt_fastx_open(const char* path, int strict, uint32_t min_length, gt_fastx** out);
+/* Fills one batch in the layout of every gt_dbg_* call: up to max_reads
+ * records and max_bases bases (seq, offsets with n + 1 entries), optionally
+ * their names (names / name_offsets, names_cap bytes).  *n_reads = 0 when the
+ * input is exhausted.  A record larger than the buffers fails with
+ * GT_ERR_INVALID_ARG and is kept for the next call (FastxParser::next,
+ * readers.hh:148-219, one batch at a time). */
+int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uint8_t* seq, int64_t* offsets,
+                        char* names, uint64_t names_cap, int64_t* name_offsets, uint64_t* n_reads);
+/* n_parsed / n_skipped (readers.hh:221-227), invalid reads, is_complete. */
+int gt_fastx_stats(const gt_fastx* p, uint64_t* n_parsed, uint64_t* n_skipped, uint64_t* n_invalid_reads,
+                   int* complete);
+int gt_fastx_close(gt_fastx* p);
+
+/* InserterProcessor<dBG<...>> (processors.hh:248-344) driving a FileProcessor
+ * (processors.hh:45-236) with an IntervalCounter (metrics.hh:101-145):
+ * gt_inserter_advance inserts parsed reads into the storage, batched through
+ * gt_dbg_insert, and returns right after the read whose k-mers (len - K + 1,
+ * 0 below K) bring the interval counter to `interval`, as FileProcessor::advance
+ * does: *n_sequences and *time_total are the running totals and *remaining is
+ * 0 once the parser is exhausted.  Records with invalid characters are
+ * skipped even from a strict parser (FileProcessor::handle_next,
+ * processors.hh:149-171).  batch_bases sizes the host batch (0: 64 MiB). */
+typedef struct gt_inserter gt_inserter;
+int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_t interval, uint64_t batch_bases,
+                    gt_inserter** out);
+int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequences, uint64_t* time_total,
+                        int* remaining);
+/* Running n_sequences, time (k-mers) and the interval counter. */
+int gt_inserter_stats(const gt_inserter* ins, uint64_t* n_sequences, uint64_t* time_total, uint64_t* counter);
+int gt_inserter_free(gt_inserter* ins);
 
 /* ------------------------------------------------------------ workload */
 

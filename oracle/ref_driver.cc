@@ -11,6 +11,7 @@
 #include "goetia/storage/sparseppstorage.hh"
 #include "goetia/hashing/hashshifter.hh"
 #include "goetia/hashing/kmeriterator.hh"
+#include "goetia/parsing/readers.hh"
 
 #include <cstdint>
 #include <cstring>
@@ -241,6 +242,47 @@ uint64_t ref_find_decision_kmers(void* hv, const char* seq, size_t len, uint32_t
         }
         return pos.size();
     });
+}
+
+// FastxParser<DNA_SIMPLE> (readers.hh:148-219) over a whole file, as a
+// caller looping on next() sees it: the records it returns (validated,
+// uppercased sequence; name), in order, then n_parsed / n_skipped and the
+// number of InvalidRead exceptions.  In strict mode the first invalid
+// character ends the parse and counts[3] is set.  Returns the record count
+// (records past max_records or the buffers are counted, not stored).
+uint64_t ref_fastx_parse(const char* path, int strict, uint32_t min_length, char* seq, uint64_t seq_cap,
+                         int64_t* offsets, char* names, uint64_t names_cap, int64_t* name_off,
+                         uint64_t max_records, uint64_t* counts) {
+    auto parser = goetia::FastxParser<goetia::DNA_SIMPLE>::build(path, strict != 0, min_length);
+    uint64_t n = 0, sl = 0, nl = 0;
+    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    offsets[0] = 0;
+    name_off[0] = 0;
+    while (!parser->is_complete()) {
+        std::optional<goetia::Record> rec;
+        try {
+            rec = parser->next();
+        } catch (goetia::InvalidRead&) {
+            counts[2]++;
+            continue;
+        } catch (goetia::InvalidCharacterException&) {
+            counts[3] = 1;
+            break;
+        }
+        if (!rec) continue;
+        if (n < max_records && sl + rec->sequence.size() <= seq_cap && nl + rec->name.size() <= names_cap) {
+            std::memcpy(seq + sl, rec->sequence.data(), rec->sequence.size());
+            sl += rec->sequence.size();
+            std::memcpy(names + nl, rec->name.data(), rec->name.size());
+            nl += rec->name.size();
+            offsets[n + 1] = (int64_t)sl;
+            name_off[n + 1] = (int64_t)nl;
+        }
+        n++;
+    }
+    counts[0] = parser->n_parsed();
+    counts[1] = parser->n_skipped();
+    return n;
 }
 
 void ref_free(void* hv) {

@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "goetia/dbg.hh"
+#include "goetia/parsing/readers.hh"
+#include "goetia/processors.hh"
 
 using namespace goetia;
 
@@ -33,7 +35,7 @@ int run_presence(std::shared_ptr<S> store) {
     return 0;
 }
 
-int main() {
+int main(int argc, char** argv) {
     // known-answer vector (goetia tests/test_hashing.py:15-18)
     CanLemireShifter sh(27);
     auto h = sh.hash_base("TCACCTGTGTTGTGCTACTTGCGGCGC");
@@ -97,6 +99,34 @@ int main() {
     auto c2 = g->clone();
     CHECK(c2->n_unique() == 0);
     CHECK(g->get_raw() != nullptr);
+
+    // FASTX input and the chunk driver (argv[1]: goetia's test-fastq-reads.fq)
+    if (argc > 1) {
+        auto parser = FastxParser<>::build(argv[1]);
+        uint64_t n_rec = 0, kmers = 0, expected_stops = 0;
+        IntervalCounter timer(1000);
+        while (auto rec = parser->next()) {
+            ++n_rec;
+            const uint64_t k = rec->sequence.size() >= 21 ? rec->sequence.size() - 20 : 0;
+            kmers += k;
+            expected_stops += timer.poll(k);
+        }
+        CHECK(n_rec == parser->n_parsed() && n_rec > 0 && parser->n_skipped() == 0);
+        auto g4 = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(100000, 4), 21);
+        auto proc = InserterProcessor<dBG<ByteStorage, CanLemireShifter>>::build(g4, 1000);
+        auto p2 = FastxParser<>::build(argv[1]);
+        uint64_t stops = 0, n_seq = 0, t = 0;
+        bool remaining = true;
+        while (remaining) {
+            std::tie(n_seq, t, remaining) = proc->advance(p2);
+            stops += remaining;
+        }
+        CHECK(n_seq == n_rec && t == kmers && stops == expected_stops && expected_stops > 0);
+        auto g5 = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(100000, 4), 21);
+        auto p3 = FastxParser<>::build(argv[1]);
+        for (auto b = p3->next_batch(); b.size(); b = p3->next_batch()) g5->insert_sequences(b);
+        CHECK(g4->n_unique() == g5->n_unique() && g4->n_occupied() == g5->n_occupied());
+    }
     std::printf("FACADE OK\n");
     return 0;
 }

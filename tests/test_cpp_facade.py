@@ -32,6 +32,7 @@ def test_facade_compiles_without_hip_headers(tmp_path):
 def test_facade_runs_on_gpu(tmp_path):
     exe = str(tmp_path / 'facade')
     _compile(exe)
-    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    fq = os.path.join(ROOT, 'tests', 'golden', 'data', 'test-fastq-reads.fq')
+    res = subprocess.run([exe, fq], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     assert 'FACADE OK' in res.stdout

@@ -1,0 +1,140 @@
+"""FASTX parsing (host code, no GPU): the oracle restatement and the native
+parser against what goetia's compiled FastxParser<DNA_SIMPLE> returned on the
+same files (tests/golden/fastx_ref.json), then the native parser against the
+oracle on seeded random files, at batch sizes that split records across
+calls."""
+import gzip
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from goetia_amd import _native as N
+from goetia_amd.parsing import FastxParser, NoMoreReadsAvailable
+from oracle import fastx as ofx
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+DATA = os.path.join(GOLDEN, 'data')
+REF = json.load(open(os.path.join(GOLDEN, 'fastx_ref.json')))
+CASES = sorted(REF)
+
+
+def product_parse(path, strict, min_length, max_reads=1 << 16, max_bases=1 << 24):
+    p = FastxParser(path, strict, min_length)
+    out = {'sequences': [], 'names': [], 'strict_stopped': 0}
+    while True:
+        try:
+            batch, names = p.next_batch(max_reads, max_bases, names=True)
+        except N.InvalidCharacterException:
+            out['strict_stopped'] = 1
+            break
+        if batch.n_reads == 0:
+            break
+        out['sequences'] += [batch.read(i).decode() for i in range(batch.n_reads)]
+        out['names'] += names
+    out['n_parsed'], out['n_skipped'], out['invalid_reads'], _ = p._stats()
+    p.close()
+    return out
+
+
+@pytest.mark.parametrize('case', CASES)
+def test_oracle_vs_reference(case):
+    rel, strict, min_length = case.split('|')
+    got = ofx.parse(os.path.join(DATA, rel), strict == '1', int(min_length))
+    assert got == REF[case]
+
+
+@pytest.mark.parametrize('case', CASES)
+def test_parser_vs_reference(case):
+    rel, strict, min_length = case.split('|')
+    got = product_parse(os.path.join(DATA, rel), int(strict), int(min_length))
+    assert got == REF[case]
+
+
+def _random_file(path, seed, fastq, gz):
+    rng = random.Random(seed)
+    lines = []
+    for i in range(300):
+        L = rng.choice([0, 1, 3, 20, 31, 60, 150, 301, 2000])
+        s = ''.join(rng.choice('ACGTacgt' if rng.random() < 0.3 else 'ACGT') for _ in range(L))
+        if L and rng.random() < 0.1:
+            j = rng.randrange(L)
+            s = s[:j] + rng.choice('N n.-RYX ') + s[j + 1:]
+        eol = '\r\n' if rng.random() < 0.1 else '\n'
+        name = f'r{i}' + (f' c{i}' if rng.random() < 0.5 else '') + ('\tx' if rng.random() < 0.1 else '')
+        if fastq:
+            q = 'I' * L
+            if rng.random() < 0.05:
+                q = q[:max(0, L - rng.randint(1, 5))]
+            w = rng.choice([L or 1, 40])
+            lines.append('@' + name + eol)
+            for k in range(0, max(L, 1), w):
+                lines.append(s[k:k + w] + eol)
+            lines.append('+' + (name if rng.random() < 0.3 else '') + eol)
+            for k in range(0, max(len(q), 1), w):
+                lines.append(q[k:k + w] + eol)
+        else:
+            w = rng.choice([60, 70, 1000])
+            lines.append('>' + name + eol)
+            for k in range(0, L, w):
+                lines.append(s[k:k + w] + eol)
+            if rng.random() < 0.05:
+                lines.append(eol)
+    data = ''.join(lines).encode()
+    if gz:
+        with gzip.GzipFile(path, 'wb', mtime=0) as f:
+            f.write(data)
+    else:
+        with open(path, 'wb') as f:
+            f.write(data)
+
+
+@pytest.mark.parametrize('fastq', [False, True])
+@pytest.mark.parametrize('gz', [False, True])
+@pytest.mark.parametrize('seed', [1, 2])
+def test_parser_vs_oracle_random(tmp_path, fastq, gz, seed):
+    path = str(tmp_path / (('r.fq' if fastq else 'r.fa') + ('.gz' if gz else '')))
+    _random_file(path, seed * 10 + fastq * 2 + gz, fastq, gz)
+    for strict, min_length in [(0, 0), (0, 25), (1, 0)]:
+        exp = ofx.parse(path, bool(strict), min_length)
+        got = product_parse(path, strict, min_length, max_reads=7, max_bases=2500)  # records split across calls
+        assert got == exp, (strict, min_length)
+
+
+def test_record_iteration_and_end():
+    p = FastxParser(os.path.join(DATA, 'random-20-a.fa'))
+    recs = list(p)
+    ref = REF['random-20-a.fa|0|0']
+    assert [r.sequence for r in recs] == ref['sequences']
+    assert [r.name for r in recs] == ref['names']
+    assert p.is_complete() and p.n_parsed() == ref['n_parsed']
+    with pytest.raises(NoMoreReadsAvailable):
+        p.next()
+
+
+def test_record_larger_than_buffers_is_kept(tmp_path):
+    path = str(tmp_path / 'big.fa')
+    with open(path, 'w') as f:
+        f.write('>a\nACGT\n>b\n' + 'A' * 5000 + '\n>c\nGG\n')
+    p = FastxParser(path)
+    b = p.next_batch(16, 100)
+    assert [b.read(i) for i in range(b.n_reads)] == [b'ACGT']
+    with pytest.raises(ValueError):
+        p.next_batch(16, 100)
+    b = p.next_batch(16, 10000)
+    assert [b.read(i) for i in range(b.n_reads)] == [b'A' * 5000, b'GG']
+    assert p.next_batch(16, 10000).n_reads == 0
+
+
+def test_missing_file():
+    with pytest.raises(N.GoetiaFileException):
+        FastxParser('/nonexistent/reads.fq')
+
+
+def test_batch_layout_feeds_dbg_calls():
+    """Batches use the ReadBatch layout of the batched dBG calls."""
+    b = FastxParser(os.path.join(DATA, 'test-fastq-reads.fq')).next_batch()
+    assert b.offsets[0] == 0 and np.all(np.diff(b.offsets) >= 0) and b.offsets[-1] == len(b.seq)
+    assert set(np.unique(b.seq).tolist()) <= set(b'ACGT')

@@ -1,0 +1,87 @@
+"""The chunk driver on the GPU: InserterProcessor.advance returns exactly where
+goetia's FileProcessor::advance does (after the read whose k-mers bring the
+IntervalCounter to the interval) with the same running totals, and the
+tables it builds equal inserting the parsed records directly and the
+sequential oracle's."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from goetia_amd import libgoetia as lg  # noqa: E402
+from goetia_amd import workload  # noqa: E402
+from goetia_amd.processors import IntervalCounter  # noqa: E402
+from oracle import fastx as ofx  # noqa: E402
+
+DATA = os.path.join(os.path.dirname(__file__), 'golden', 'data')
+
+
+def expected_advances(sequences, K, interval):
+    """FileProcessor::advance over the parsed records with an IntervalCounter."""
+    timer = IntervalCounter(interval)
+    out, n = [], 0
+    for s in sequences:
+        n += 1
+        if timer.poll(max(0, len(s) - K + 1)):
+            out.append((n, timer.total(), True))
+    out.append((n, timer.total(), False))
+    return out
+
+
+def run(proc, parser):
+    got = []
+    while True:
+        r = proc.advance(parser)
+        got.append(r)
+        if not r[2]:
+            return got
+
+
+def _write_fastq(path, batch):
+    with gzip.open(path, 'wb') as f:
+        for i in range(batch.n_reads):
+            s = batch.read(i)
+            f.write(b'@r%d\n%s\n+\n%s\n' % (i, s, b'I' * len(s)))
+
+
+@pytest.mark.parametrize('name,interval', [('test-fastq-reads.fq', 777), ('random-20-a.fa', 1000),
+                                           ('fastx/mixed.fq', 50), ('fastx/random_wrapped.fa', 5000)])
+def test_advance_matches_interval_semantics(oracle, name, interval):
+    path = os.path.join(DATA, name)
+    K = 21
+    seqs = ofx.parse(path)['sequences']
+    sizes = oracle.primes_near_x(4, 100_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    proc = lg.InserterProcessor.build(g, interval)
+    got = run(proc, lg.FastxParser.build(path))
+    assert got == expected_advances(seqs, K, interval)
+    assert proc.n_sequences() == len(seqs) and proc.interval() == interval
+    o = oracle.Storage(oracle.BYTE, sizes)
+    for s in seqs:
+        if len(s) >= K:
+            o.insert_sequence(s, K, True)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+
+
+def test_process_file_equals_direct_insert(tmp_path):
+    """A 50k-read gzip FASTQ through process() with the default interval and
+    small host batches builds the same Bloom tables as one batched insert."""
+    genome = workload.genome(61, 200_000)
+    batch = workload.reads(62, genome, 50_000, 80, 220, 0.01, 0.0)
+    path = str(tmp_path / 'reads.fq.gz')
+    _write_fastq(path, batch)
+    K = 31
+    g1 = lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage.build(2_000_000, 4), K)
+    proc = lg.InserterProcessor(g1, batch_bases=1 << 20)
+    n, t = proc.process(path)
+    assert n == batch.n_reads and t == int(batch.kmer_counts(K).sum())
+    g2 = lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage.build(2_000_000, 4), K)
+    g2.insert_sequences(batch)
+    for a, b in zip(g1.get_raw(), g2.get_raw()):
+        np.testing.assert_array_equal(a, b)
+    assert g1.n_unique() == g2.n_unique()
