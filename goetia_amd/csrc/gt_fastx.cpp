@@ -12,15 +12,37 @@
 // record or, strict, stops the parse; records shorter than min_length are
 // skipped; a quality of another length is an invalid read.  n_parsed and
 // n_skipped count as goetia's do.  Host code only: no device is needed.
+//
+// Plain (not gzip) files are read in large blocks and each block is parsed by
+// several threads when it is well formed -- FASTA whose lines do not start
+// with '@' or '+', or FASTQ of 4-line records with equal sequence and quality
+// lengths, without '\r' -- because there record boundaries can be found from
+// any point (a FASTA header is a line starting with '>'; a FASTQ header is a
+// line starting with '@' two lines before a line starting with '+', which a
+// quality line starting with '@' never is, since the next sequence line cannot
+// start with '+').  On those inputs the kseq reading above gives exactly the
+// same records.  A block that is not well formed, a gzip file or a strict
+// parser is read record by record from that block on, as is everything when
+// only one thread is available.  The next block is parsed in the background
+// while the current one is handed out.
 #include "goetia_amd.h"
 
 #include <zlib.h>
 
 #include <algorithm>
 #include <cctype>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <future>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 namespace gt {
 int set_error(int code, const std::string& msg);
@@ -114,6 +136,40 @@ enum Next { kRecord, kSkipped, kDone, kInvalidRead, kIoError, kInvalidChar };
 
 }  // namespace
 
+namespace {
+
+// Valid records of one parsed block, in file order.
+struct Block {
+    std::string seq, names;
+    std::vector<int64_t> off{0}, noff{0};
+    std::vector<uint64_t> cum_parsed, cum_skipped;   // counts up to and including each record
+    uint64_t tot_parsed = 0, tot_skipped = 0;
+    size_t next = 0;
+
+    size_t count() const { return off.size() - 1; }
+    void clear() {
+        seq.clear();
+        names.clear();
+        off.assign(1, 0);
+        noff.assign(1, 0);
+        cum_parsed.clear();
+        cum_skipped.clear();
+        tot_parsed = tot_skipped = 0;
+        next = 0;
+    }
+};
+
+// One block of a mapped file parsed by several threads: the valid records of
+// each thread's part in order, or ok = false when the block is not well formed.
+struct ParsedBlock {
+    std::vector<Block> parts;
+    uint64_t start = 0, end = 0;           // file offsets
+    uint64_t tot_parsed = 0, tot_skipped = 0;
+    bool ok = false;
+};
+
+}  // namespace
+
 struct gt_fastx {
     Stream ks;
     int last_char = 0;
@@ -125,6 +181,19 @@ struct gt_fastx {
     bool pending = false;                  // seq / name hold a record not yet handed out
     std::string path;
     std::string last_bad;                  // sequence of the record that stopped a strict parse
+    // block-parallel reading of plain files (the file mapped read-only)
+    bool par = false;
+    const char* map = nullptr;
+    size_t map_len = 0;
+    char fmt = 0;                          // '>' or '@'
+    int threads = 1;
+    size_t block = 64u << 20;
+    uint64_t next_pos = 0;                 // file offset where the next block starts
+    ParsedBlock cur;                       // block being handed out
+    size_t cur_part = 0, cur_rec = 0;
+    uint64_t part_parsed = 0, part_skipped = 0;   // counts of cur's parts before cur_part
+    std::future<ParsedBlock> ahead;        // the block after cur, parsed in the background
+    uint64_t base_parsed = 0, base_skipped = 0;   // counts of the blocks before cur
 };
 
 namespace {
@@ -205,6 +274,237 @@ Next next_record(gt_fastx& p) {
     return kRecord;
 }
 
+// ------------------------------------------------------------ block-parallel reading
+
+inline size_t line_end(const char* b, size_t i, size_t n) {
+    const void* q = i < n ? std::memchr(b + i, '\n', n - i) : nullptr;
+    return q ? (size_t)((const char*)q - b) : n;
+}
+
+inline bool has_cr(const char* b, size_t i, size_t e) { return e > i && std::memchr(b + i, '\r', e - i); }
+
+// A well-formed FASTQ record starts at line start i: '@' line, a non-empty
+// sequence line not starting with '>', '+' or '@', a '+' line and a quality
+// line of the same length; every line ends in '\n' unless the input ends.
+bool fq_header_at(const char* b, size_t i, size_t n, bool eof) {
+    if (i >= n || b[i] != '@') return false;
+    const size_t e0 = line_end(b, i, n);
+    if (e0 >= n) return false;
+    const size_t s1 = e0 + 1, e1 = line_end(b, s1, n);
+    if (e1 >= n || e1 == s1 || b[s1] == '>' || b[s1] == '+' || b[s1] == '@') return false;
+    const size_t s2 = e1 + 1;
+    if (s2 >= n || b[s2] != '+') return false;
+    const size_t e2 = line_end(b, s2, n);
+    if (e2 >= n) return false;
+    const size_t s3 = e2 + 1, e3 = line_end(b, s3, n);
+    if (e3 >= n && !eof) return false;
+    return e3 - s3 == e1 - s1;
+}
+
+// First record start at or after `from` (n if none): a line starting with
+// '>' (FASTA) or a well-formed FASTQ record.
+size_t next_header(const char* b, size_t from, size_t n, char fmt, bool eof) {
+    size_t i = from;
+    if (i > 0 && b[i - 1] != '\n') i = line_end(b, i, n) + 1;
+    while (i < n) {
+        if (fmt == '>' ? b[i] == '>' : fq_header_at(b, i, n, eof)) return i;
+        i = line_end(b, i, n) + 1;
+    }
+    return n;
+}
+
+// Start of the last record whose header line is complete and verifiable
+// (0 if none beyond the first): the block is parsed up to there and the rest
+// kept for the next block.
+size_t last_header(const char* b, size_t n, char fmt) {
+    size_t i = n;
+    while (i > 1) {
+        const void* q = memrchr(b, '\n', i - 1);
+        if (!q) return 0;
+        const size_t ls = (size_t)((const char*)q - b) + 1;
+        if (fmt == '>' ? b[ls] == '>' : fq_header_at(b, ls, n, false)) return ls;
+        i = ls - 1;
+    }
+    return 0;
+}
+
+// Validated (uppercased) copy of [i, e) appended to out; false if a byte is
+// outside A/C/G/T/a/c/g/t.
+inline bool append_valid(std::string& out, const char* b, size_t i, size_t e) {
+    const size_t o = out.size();
+    out.resize(o + (e - i));
+    char* d = &out[o];
+    unsigned char ok = 0xFF;
+    for (size_t k = i; k < e; ++k) {
+        const unsigned char v = kValidator.map[(unsigned char)b[k]];
+        ok &= (unsigned char)(v ? 0xFF : 0);
+        *d++ = (char)v;
+    }
+    return ok != 0;
+}
+
+// FastxParser::next's bookkeeping for one record: keep it, or count it as
+// skipped (twice when it is both invalid and short, as goetia counts).
+inline void finish_record(Block& out, size_t seq_start, bool valid, uint32_t min_length, const char* name,
+                          size_t name_len) {
+    const size_t len = out.seq.size() - seq_start;
+    out.tot_parsed++;
+    if (!valid) out.tot_skipped++;
+    if (len < min_length) out.tot_skipped++;
+    if (!valid || len < min_length) {
+        out.seq.resize(seq_start);
+        return;
+    }
+    out.off.push_back((int64_t)out.seq.size());
+    out.names.append(name, name_len);
+    out.noff.push_back((int64_t)out.names.size());
+    out.cum_parsed.push_back(out.tot_parsed);
+    out.cum_skipped.push_back(out.tot_skipped);
+}
+
+inline size_t name_end(const char* b, size_t i, size_t e) {
+    while (i < e && !std::isspace((unsigned char)b[i])) ++i;
+    return i;
+}
+
+// Records whose headers lie in [a, end) of a well-formed block; false when the
+// range is not well formed.
+bool parse_range(const char* b, size_t a, size_t end, size_t n, bool eof, char fmt, uint32_t min_length,
+                 Block& out) {
+    size_t pos = a;
+    while (pos < end) {
+        if (b[pos] != fmt) return false;
+        const size_t he = line_end(b, pos, n);
+        if (has_cr(b, pos, he)) return false;
+        const char* name = b + pos + 1;
+        const size_t name_len = name_end(b, pos + 1, he) - (pos + 1);
+        const size_t start = out.seq.size();
+        bool valid = true;
+        if (fmt == '>') {
+            pos = he + 1;
+            while (pos < n && b[pos] != '>') {
+                const size_t le = line_end(b, pos, n);
+                if (le > pos) {
+                    if (b[pos] == '@' || b[pos] == '+' || has_cr(b, pos, le)) return false;
+                    valid &= append_valid(out.seq, b, pos, le);
+                }
+                pos = le + 1;
+            }
+        } else {
+            if (!fq_header_at(b, pos, n, eof)) return false;
+            const size_t s1 = he + 1, e1 = line_end(b, s1, n);
+            const size_t e2 = line_end(b, e1 + 1, n);
+            const size_t e3 = line_end(b, e2 + 1, n);
+            if (has_cr(b, s1, e3)) return false;
+            valid = append_valid(out.seq, b, s1, e1);
+            pos = e3 + 1;
+        }
+        finish_record(out, start, valid, min_length, name, name_len);
+    }
+    return pos == end || (eof && end == n && pos >= n);
+}
+
+int block_threads() {
+    if (const char* v = std::getenv("GOETIA_AMD_FASTX_THREADS")) {
+        const int t = std::atoi(v);
+        if (t > 0) return std::min(t, 64);
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hc ? hc : 1u));
+}
+
+size_t block_bytes() {
+    if (const char* v = std::getenv("GOETIA_AMD_FASTX_BLOCK")) {
+        const unsigned long long x = std::strtoull(v, nullptr, 10);
+        if (x >= 64) return (size_t)x;
+    }
+    return 64u << 20;
+}
+
+// Parses the block of the mapping starting at `pos` (a record start): up to
+// the last verifiable record start within `block` bytes, growing the block
+// while it holds less than one record.
+ParsedBlock parse_block_at(const char* map, size_t map_len, uint64_t pos, size_t block, int T, char fmt,
+                           uint32_t min_length) {
+    ParsedBlock out;
+    out.start = pos;
+    const char* b = map + pos;
+    const size_t avail = map_len - pos;
+    if (avail == 0 || b[0] != fmt) return out;
+    size_t n = 0, cut = 0;
+    for (;;) {
+        n = std::min(block, avail);
+        cut = n == avail ? n : last_header(b, n, fmt);
+        if (cut > 0) break;
+        block *= 2;
+    }
+    const bool eof = n == avail && cut == n;
+    std::vector<size_t> st(T + 1);
+    st[0] = 0;
+    st[T] = cut;
+    for (int t = 1; t < T; ++t) st[t] = std::max(st[t - 1], next_header(b, cut * (size_t)t / (size_t)T, cut, fmt, eof));
+    out.parts.resize(T);
+    std::vector<char> ok(T, 1);
+    auto work = [&](int t) { ok[t] = parse_range(b, st[t], st[t + 1], n, eof, fmt, min_length, out.parts[t]) ? 1 : 0; };
+    if (T == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+    }
+    for (int t = 0; t < T; ++t) {
+        if (!ok[t]) return out;
+        out.tot_parsed += out.parts[t].tot_parsed;
+        out.tot_skipped += out.parts[t].tot_skipped;
+    }
+    out.end = pos + cut;
+    out.ok = true;
+    return out;
+}
+
+void launch_ahead(gt_fastx& p) {
+    p.ahead = std::async(std::launch::async, parse_block_at, p.map, p.map_len, p.next_pos, p.block, p.threads, p.fmt,
+                         p.min_length);
+}
+
+// Switches a block-parallel parser to record-by-record reading from file
+// offset pos (a record start, so kseq resumes exactly there).
+int fall_back(gt_fastx& p, uint64_t pos) {
+    p.par = false;
+    p.cur = ParsedBlock();
+    p.n_parsed = p.base_parsed;
+    p.n_skipped = p.base_skipped;
+    if (gzseek(p.ks.f, (z_off_t)pos, SEEK_SET) < 0) return -3;
+    p.ks.begin = p.ks.end = 0;
+    p.ks.is_eof = false;
+    p.last_char = 0;
+    return -1;
+}
+
+// Makes the next block current: 1 ready, 0 end of input, -1 switched to
+// record-by-record reading, -3 read error.
+int next_block(gt_fastx& p) {
+    p.base_parsed += p.cur.tot_parsed;
+    p.base_skipped += p.cur.tot_skipped;
+    p.n_parsed = p.base_parsed;
+    p.n_skipped = p.base_skipped;
+    p.cur = ParsedBlock();
+    p.cur_part = p.cur_rec = 0;
+    p.part_parsed = p.part_skipped = 0;
+    if (!p.ahead.valid()) {
+        if (p.next_pos >= p.map_len) return 0;
+        launch_ahead(p);
+    }
+    ParsedBlock blk = p.ahead.get();
+    if (!blk.ok) return fall_back(p, blk.start);
+    p.cur = std::move(blk);
+    p.next_pos = p.cur.end;
+    if (p.next_pos < p.map_len) launch_ahead(p);
+    return 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -224,6 +524,30 @@ int gt_fastx_open(const char* path, int strict, uint32_t min_length, gt_fastx** 
     p->strict = strict != 0;
     p->min_length = min_length;
     p->path = path;
+    // block-parallel reading for plain FASTA / FASTQ files (not gzip, not strict)
+    if (!p->strict && block_threads() > 1) {
+        const int fd = ::open(path, O_RDONLY);
+        struct stat sb;
+        if (fd >= 0 && ::fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size >= 2) {
+            void* m = ::mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m != MAP_FAILED) {
+                const unsigned char* h = (const unsigned char*)m;
+                const bool gz = h[0] == 0x1f && h[1] == 0x8b;
+                if (!gz && (h[0] == '>' || h[0] == '@')) {
+                    ::madvise(m, (size_t)sb.st_size, MADV_SEQUENTIAL);
+                    p->map = (const char*)m;
+                    p->map_len = (size_t)sb.st_size;
+                    p->par = true;
+                    p->fmt = (char)h[0];
+                    p->threads = block_threads();
+                    p->block = block_bytes();
+                } else {
+                    ::munmap(m, (size_t)sb.st_size);
+                }
+            }
+        }
+        if (fd >= 0) ::close(fd);
+    }
     *out = p;
     return GT_OK;
 }
@@ -237,6 +561,54 @@ int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uin
     if (names) name_offsets[0] = 0;
     *n_reads = 0;
     for (;;) {
+        if (p->par) {
+            if (p->cur_part >= p->cur.parts.size()) {
+                const int r = next_block(*p);
+                if (r == 0) {
+                    p->complete = true;
+                    break;
+                }
+                if (r == -3) {
+                    *n_reads = n;
+                    return set_error(GT_ERR_IO, "error reading " + p->path);
+                }
+                continue;
+            }
+            const Block& B = p->cur.parts[p->cur_part];
+            if (p->cur_rec >= B.count()) {
+                p->part_parsed += B.tot_parsed;
+                p->part_skipped += B.tot_skipped;
+                p->n_parsed = p->base_parsed + p->part_parsed;
+                p->n_skipped = p->base_skipped + p->part_skipped;
+                ++p->cur_part;
+                p->cur_rec = 0;
+                continue;
+            }
+            const size_t i = p->cur_rec;
+            const uint64_t len = (uint64_t)(B.off[i + 1] - B.off[i]);
+            const uint64_t nlen = (uint64_t)(B.noff[i + 1] - B.noff[i]);
+            const bool fits = n < max_reads && bases + len <= max_bases && (!names || nbytes + nlen <= names_cap);
+            p->n_parsed = p->base_parsed + p->part_parsed + B.cum_parsed[i];
+            p->n_skipped = p->base_skipped + p->part_skipped + B.cum_skipped[i];
+            if (!fits) {
+                if (n == 0) {
+                    return set_error(GT_ERR_INVALID_ARG, "record of " + std::to_string(len) +
+                                                             " bases does not fit the batch buffers");
+                }
+                break;
+            }
+            std::memcpy(seq + bases, B.seq.data() + B.off[i], len);
+            bases += len;
+            offsets[n + 1] = (int64_t)bases;
+            if (names) {
+                std::memcpy(names + nbytes, B.names.data() + B.noff[i], nlen);
+                nbytes += nlen;
+                name_offsets[n + 1] = (int64_t)nbytes;
+            }
+            ++n;
+            ++p->cur_rec;
+            continue;
+        }
         if (!p->pending) {
             const Next r = next_record(*p);
             if (r == kDone) break;
@@ -288,6 +660,8 @@ int gt_fastx_stats(const gt_fastx* p, uint64_t* n_parsed, uint64_t* n_skipped, u
 
 int gt_fastx_close(gt_fastx* p) {
     if (!p) return GT_OK;
+    if (p->ahead.valid()) p->ahead.wait();
+    if (p->map) ::munmap((void*)p->map, p->map_len);
     if (p->ks.f) gzclose(p->ks.f);
     delete p;
     return GT_OK;

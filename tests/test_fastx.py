@@ -138,3 +138,46 @@ def test_batch_layout_feeds_dbg_calls():
     b = FastxParser(os.path.join(DATA, 'test-fastq-reads.fq')).next_batch()
     assert b.offsets[0] == 0 and np.all(np.diff(b.offsets) >= 0) and b.offsets[-1] == len(b.seq)
     assert set(np.unique(b.seq).tolist()) <= set(b'ACGT')
+
+
+def _clean_file(path, seed, fastq, n=400, tail=b''):
+    """Well-formed input for the block-parallel reader: 4-line FASTQ whose
+    quality lines often start with '@' or '+', or multi-line FASTA with empty
+    lines and empty records; lowercase and invalid bytes inside lines."""
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        L = rng.choice([1, 5, 21, 40, 150, 151, 600])
+        s = ''.join(rng.choice('ACGTacgtACGT') for _ in range(L))
+        if rng.random() < 0.08:
+            j = rng.randrange(L)
+            s = s[:j] + rng.choice('N.-* ') + s[j + 1:]
+        name = f'c{i}' + (f' d{i}' if rng.random() < 0.5 else '')
+        if fastq:
+            q = ''.join(rng.choice('@+I#5') for _ in range(L))
+            out.append(f'@{name}\n{s}\n+{name if rng.random() < 0.3 else ""}\n{q}\n')
+        else:
+            if L > 150 and rng.random() < 0.05:
+                s = ''
+            w = rng.choice([60, 61, 80])
+            body = ''.join(s[k:k + w] + '\n' + ('\n' if rng.random() < 0.03 else '') for k in range(0, len(s), w))
+            out.append(f'>{name}\n{body}')
+    data = ''.join(out).encode() + tail
+    if rng.random() < 0.5 and not tail:
+        data = data.rstrip(b'\n')            # no final newline
+    with open(path, 'wb') as f:
+        f.write(data)
+
+
+@pytest.mark.parametrize('fastq', [False, True])
+@pytest.mark.parametrize('block,threads', [(64, 1), (200, 3), (1000, 8), (1 << 20, 16)])
+def test_block_parallel_reader_vs_oracle(tmp_path, monkeypatch, fastq, block, threads):
+    monkeypatch.setenv('GOETIA_AMD_FASTX_BLOCK', str(block))
+    monkeypatch.setenv('GOETIA_AMD_FASTX_THREADS', str(threads))
+    for k, tail in enumerate([b'', b'\n@odd\nACGT\n+\nII\n' if fastq else b'\n+odd line\nACGT\n']):
+        path = str(tmp_path / f'c{k}.{"fq" if fastq else "fa"}')
+        _clean_file(path, block + threads + k, fastq, tail=tail)
+        for min_length in (0, 30):
+            exp = ofx.parse(path, False, min_length)
+            got = product_parse(path, 0, min_length, max_reads=13, max_bases=5000)
+            assert got == exp, (k, min_length)
