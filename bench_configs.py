@@ -11,6 +11,7 @@ neighbors_5m  repeats + SNP bubbles, 5M x 150 bp, K=31, ByteStorage(2^28, 4):
 serving_64    64 sample streams, see goetia_amd/serving.py.
 """
 import json
+import os
 import sys
 import time
 
@@ -216,6 +217,84 @@ def serving(args):
     print(json.dumps(out), flush=True)
 
 
+def fastx(args):
+    """FASTQ file -> FastxParser -> InserterProcessor -> ByteStorage(2^28, 4):
+    5M x 150 bp reads of the count-min workload's genome written as a plain
+    FASTQ file, parsed alone, then inserted end to end with a large interval
+    and with goetia's default one; tables checked against inserting the same
+    reads from memory."""
+    import ctypes as C
+    import tempfile
+    import torch
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    from goetia_amd.parsing import FastxParser
+    n_reads = max(1000, int(5_000_000 * args.scale))
+    g = workload.genome(2, 50_000_000)
+    batch = workload.reads(2, g, n_reads, 150, 150, 0.005, 0.0)
+    K = 31
+    tmp = tempfile.mkdtemp(prefix='goetia_amd_fastx_')
+    path = os.path.join(tmp, 'reads.fq')
+    t0 = time.perf_counter()
+    lens = batch.lengths()
+    with open(path, 'wb') as f:
+        chunk = []
+        for i in range(batch.n_reads):
+            s = batch.read(i)
+            chunk.append(b'@r%d\n%s\n+\n%s\n' % (i, s, b'I' * int(lens[i])))
+            if len(chunk) == 100_000:
+                f.write(b''.join(chunk))
+                chunk = []
+        f.write(b''.join(chunk))
+    size = os.path.getsize(path)
+    log(f'wrote {size / 1e9:.2f} GB FASTQ in {time.perf_counter() - t0:.1f}s')
+    nk = int(batch.kmer_counts(K).sum())
+    # parse only (native batches into preallocated host buffers)
+    lib = N.load()
+    mr, mb = 1 << 16, 64 << 20
+    seq = np.empty(mb, np.uint8)
+    off = np.zeros(mr + 1, np.int64)
+    h = N.vp()
+    N.call('gt_fastx_open', path.encode(), 0, 0, C.byref(h))
+    got = C.c_uint64()
+    t0 = time.perf_counter()
+    total = 0
+    while True:
+        N.check(lib.gt_fastx_next_batch(h, mr, mb, N.ptr(seq), N.ptr(off), None, 0, None, C.byref(got)))
+        if got.value == 0:
+            break
+        total += got.value
+    t_parse = time.perf_counter() - t0
+    lib.gt_fastx_close(h)
+    assert total == batch.n_reads
+    out = {'metric': 'canonical k-mers/s, FASTQ file to count-min tables (1 GPU)', 'workload': 'fastx',
+           'reads': batch.n_reads, 'kmers': nk, 'file_bytes': size,
+           'parse_gbps': round(size / t_parse / 1e9, 2), 'parse_kmers_per_s': round(nk / t_parse, 1)}
+    ref = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(2 ** 28, 4), K)
+    ref.insert_device(batch.to_device())
+    torch.cuda.synchronize()
+    ref_tables = ref.get_raw()
+    del ref
+    for label, interval in (('large_interval', 1 << 28), ('default_interval', 500_000)):
+        gr = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(2 ** 28, 4), K)
+        proc = lg.InserterProcessor(gr, interval)
+        t0 = time.perf_counter()
+        n, t = proc.process(path)
+        dt = time.perf_counter() - t0
+        same = all(np.array_equal(a, b) for a, b in zip(gr.get_raw(), ref_tables))
+        out[label] = {'interval': interval, 'kmers_per_s': round(nk / dt, 1), 'seconds': round(dt, 3),
+                      'n_sequences': n, 'kmers': t, 'tables_equal_in_memory_insert': same}
+        del gr, proc
+    out['value'] = out['large_interval']['kmers_per_s']
+    out['unit'] = 'kmers/s'
+    out['note'] = ('plain FASTQ read by the block-parallel parser (host threads); each advance() inserts one '
+                   'interval of reads through gt_dbg_insert (host buffers, PCIe included)')
+    os.remove(path)
+    os.rmdir(tmp)
+    print(json.dumps(out), flush=True)
+
+
 def run(args):
     return {'bloom_100k': bloom, 'exact_mixed': exact, 'neighbors_5m': neighbors,
-            'serving_64': serving}[args.workload](args)
+            'serving_64': serving, 'fastx': fastx}[args.workload](args)
