@@ -102,8 +102,9 @@ def cpu_baseline(cfg, batch, sample_reads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build targets one GPU; multi-GPU sharding is a design '
-                                   'described in DESIGN.md and is not run', 'n_gpus': args.gpus}))
+        print(json.dumps({'error': 'this build runs on one GPU; sharding across GPUs (reads per GPU, '
+                                   'tables merged by OR / saturating-add all-reduce) is not implemented',
+                          'n_gpus': args.gpus}))
         sys.exit(2)
     if args.workload != 'cms_20m':
         import bench_configs
