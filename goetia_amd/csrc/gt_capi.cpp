@@ -225,6 +225,7 @@ struct gt_storage {
     uint64_t ecount_ub = 0;            // host upper bound on the set size
     Scratch own;                       // default scratch of calls on this storage
     DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
+    DevBuf solid_counts;               // insert_and_query counts of a solid-filter batch
     uint64_t gen_issued = 0;           // generation bumps enqueued so far (device kGen follows it)
     uint64_t epoch = 0;                // bumped when device table pointers change (exact-set growth)
     std::vector<struct gt_pipeline*> pipelines;   // attached pipelines
@@ -1335,6 +1336,46 @@ int gt_dbg_insert_and_query(gt_storage* st, uint16_t K, int canonical, int mode,
     GT_TRY(gt_dbg_insert_and_query_dev(st, K, canonical, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads,
                                        n_bases, dout.as<int16_t>(), s));
     GT_TRY(stage_out(counts, dout, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return check_kernel_errors(st, s);
+}
+
+int gt_solid_filter_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                        uint64_t n_reads, uint64_t n_bases, float min_prop_solid, uint32_t solid_threshold,
+                        uint8_t* passed, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0) return GT_OK;
+    if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    GT_TRY(st->solid_counts.ensure(std::max<uint64_t>(n_bases, 1) * sizeof(int16_t)));
+    int16_t* counts = st->solid_counts.as<int16_t>();
+    GT_TRY(gt_dbg_insert_and_query_dev(st, K, canonical, mode, seq, offsets, n_reads, n_bases, counts, stream));
+    // insert_and_query left this batch's packed k-mer offsets in the storage's scratch
+    launch_solid_reads(offsets, st->own.kmer_off.as<int64_t>(), n_reads, K, counts, solid_threshold, min_prop_solid,
+                       passed, s);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+int gt_solid_filter(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                    uint64_t n_reads, float min_prop_solid, uint32_t solid_threshold, uint8_t* passed) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0) return GT_OK;
+    DevBuf dseq, doff, dpass;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    GT_TRY(dpass.ensure(n_reads));
+    GT_TRY(gt_solid_filter_dev(st, K, canonical, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads, n_bases,
+                               min_prop_solid, solid_threshold, dpass.as<uint8_t>(), s));
+    GT_TRY(stage_out(passed, dpass, n_reads, s));
     GT_HIP(hipStreamSynchronize(s));
     return check_kernel_errors(st, s);
 }

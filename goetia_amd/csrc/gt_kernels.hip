@@ -1582,6 +1582,29 @@ __global__ void bin_iq_combine_kernel(const uint32_t* vq, const int64_t* nk, uin
     add_counter(counters + kNUnique, n_new);
 }
 
+// StreamingSolidFilter::Filter::filter_sequence (solidifier.hh:58-73) per read
+// from its insert_and_query counts, with the reference's types: count_t
+// against a uint32_t threshold, a float ratio compared in double with
+// 1.0 - min_prop_solid.  Reads shorter than K do not pass.
+__global__ void solid_reads_kernel(const int64_t* off, const int64_t* kmer_off, uint64_t n_reads, int K,
+                                   const int16_t* counts, uint32_t threshold, float min_prop_solid,
+                                   uint8_t* passed) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n_reads;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t len = off[r + 1] - off[r];
+        if (len < K) {
+            passed[r] = 0;
+            continue;
+        }
+        uint32_t n_not_solid = 0;
+        for (int64_t k = kmer_off[r]; k < kmer_off[r + 1]; ++k)
+            n_not_solid += (uint32_t)(int)counts[k] < threshold ? 1u : 0u;
+        const uint32_t n_kmers = (uint32_t)(len - K + 1);
+        const bool fails = (double)((float)n_not_solid / (float)n_kmers) >= (1.0 - (double)min_prop_solid);
+        passed[r] = fails ? 0 : 1;
+    }
+}
+
 // Zeroes p[0, *n) (32-bit words).
 __global__ void zero_words_kernel(uint32_t* p, const int64_t* n) {
     const uint64_t len = (uint64_t)*n;
@@ -1847,6 +1870,13 @@ void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n
 void launch_zero_words(uint32_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {
     const uint64_t g = (max_n + 255) / 256;
     zero_words_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(p, n);
+}
+
+void launch_solid_reads(const int64_t* off, const int64_t* kmer_off, uint64_t n_reads, int K, const int16_t* counts,
+                        uint32_t threshold, float min_prop_solid, uint8_t* passed, hipStream_t s) {
+    const uint64_t g = (n_reads + 255) / 256;
+    solid_reads_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(off, kmer_off, n_reads, K, counts,
+                                                                                threshold, min_prop_solid, passed);
 }
 
 uint64_t scan_u32_block_count(uint64_t n) { return (n + kScanU32Chunk - 1) / kScanU32Chunk; }

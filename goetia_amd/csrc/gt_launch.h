@@ -65,6 +65,8 @@ void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_
                          uint64_t* counters, hipStream_t s);
 void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
 void launch_zero_words(uint32_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
+void launch_solid_reads(const int64_t* off, const int64_t* kmer_off, uint64_t n_reads, int K, const int16_t* counts,
+                        uint32_t threshold, float min_prop_solid, uint8_t* passed, hipStream_t s);
 void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,
                            uint64_t* counters, hipStream_t s);
 uint64_t scan_u32_block_count(uint64_t n);

@@ -1,0 +1,46 @@
+"""Consumers of the dBG path: goetia's StreamingSolidFilter
+(solidifier.hh:20-80), batched on the GPU with exact per-read results."""
+import numpy as np
+
+from . import _native as N
+from .dbg import _as_batch, _stream_ptr
+
+
+class StreamingSolidFilter:
+    """goetia::StreamingSolidFilter<dBG>::Filter: a read passes unless the
+    share of its k-mers whose count (just after their own insert) is below
+    solid_threshold reaches 1 - min_prop_solid."""
+
+    def __init__(self, graph, min_prop_solid=0.75, solid_threshold=1):
+        self.dbg = graph
+        self.K = graph.K
+        self.min_prop_solid = float(min_prop_solid)
+        self.solid_threshold = int(solid_threshold)
+
+    @classmethod
+    def build(cls, graph, min_prop_solid=0.75, solid_threshold=1):
+        return cls(graph, min_prop_solid, solid_threshold)
+
+    def filter_sequence(self, sequence):
+        """solidifier.hh:58-73: (passed, n_kmers); SequenceLengthException
+        below K, as the reference's insert_and_query_sequence raises."""
+        s = sequence.encode() if isinstance(sequence, str) else bytes(sequence)
+        if len(s) < self.K:
+            raise N.SequenceLengthException('Sequence must have length >= K')
+        return bool(self.filter_sequences([s])[0]), len(s) - self.K + 1
+
+    def filter_sequences(self, reads):
+        """One pass over a batch: passed flag per read (reads shorter than K
+        do not pass), identical to filtering the reads one after another."""
+        batch = _as_batch(reads)
+        passed = np.zeros(max(batch.n_reads, 1), np.uint8)
+        N.call('gt_solid_filter', self.dbg.S.handle, self.K, int(self.dbg.canonical), self.dbg.mode,
+               N.ptr(batch.seq), N.ptr(batch.offsets), batch.n_reads, self.min_prop_solid, self.solid_threshold,
+               N.ptr(passed))
+        return passed[:batch.n_reads].astype(bool)
+
+    def filter_device(self, reads, passed, stream=None):
+        """Device batch (DeviceReads) into a uint8 device tensor, asynchronous."""
+        N.call('gt_solid_filter_dev', self.dbg.S.handle, self.K, int(self.dbg.canonical), self.dbg.mode,
+               N.ptr(reads.seq), N.ptr(reads.offsets), reads.n_reads, reads.n_bases, self.min_prop_solid,
+               self.solid_threshold, N.ptr(passed), _stream_ptr(stream))

@@ -11,10 +11,12 @@ from ._native import (GoetiaException, GoetiaFileException, InvalidCharacterExce
                       SequenceLengthException, UninitializedShifterException)
 from .parsing import FastxParser, NoMoreReadsAvailable, Record
 from .processors import InserterProcessor, IntervalCounter
+from .filters import StreamingSolidFilter
 
 __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Canonical', 'Hash',
            'DNA_SIMPLE', 'KmerIterator', 'BitStorage', 'ByteStorage', 'SparseppSetStorage',
            'PHMapStorage', 'BTreeStorage', 'StorageTraits', 'count_t', 'get_n_primes_near_x',
            'GoetiaException', 'InvalidSequenceException', 'SequenceLengthException',
            'UninitializedShifterException', 'GoetiaFileException', 'InvalidCharacterException',
-           'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter']
+           'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter',
+           'StreamingSolidFilter']

@@ -326,6 +326,24 @@ int gt_profile_enable(int on);
 int gt_profile_reset(void);
 int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
 
+/* ------------------------------------------------------------ consumers */
+
+/* StreamingSolidFilter<dBG<S, Sh>>::Filter::filter_sequence (solidifier.hh:58-73)
+ * over a batch: each read's k-mers are inserted and queried in read order,
+ * exactly as insert_and_query_sequence (dbg.hh:327-340) would one read after
+ * the other, and read r passes (passed[r] = 1) unless
+ * (float)n_not_solid / (float)n_kmers >= 1.0 - min_prop_solid, where a
+ * k-mer is not solid when its count is below solid_threshold.  Reads shorter
+ * than K do not pass (FilterProcessor::process_sequence skips them,
+ * processors.hh:389-401).  goetia's defaults: min_prop_solid 0.75,
+ * solid_threshold 1. */
+int gt_solid_filter_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                        const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, float min_prop_solid,
+                        uint32_t solid_threshold, uint8_t* passed, void* stream);
+int gt_solid_filter(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                    const int64_t* offsets, uint64_t n_reads, float min_prop_solid, uint32_t solid_threshold,
+                    uint8_t* passed);
+
 /* ------------------------------------------------------------ FASTX input */
 
 /* goetia's FastxParser<DNA_SIMPLE> (readers.hh:95-232; construction

@@ -10,6 +10,7 @@
 #include "goetia/dbg.hh"
 #include "goetia/parsing/readers.hh"
 #include "goetia/processors.hh"
+#include "goetia/solidifier.hh"
 
 using namespace goetia;
 
@@ -126,6 +127,18 @@ int main(int argc, char** argv) {
         auto p3 = FastxParser<>::build(argv[1]);
         for (auto b = p3->next_batch(); b.size(); b = p3->next_batch()) g5->insert_sequences(b);
         CHECK(g4->n_unique() == g5->n_unique() && g4->n_occupied() == g5->n_occupied());
+        // solid filter: every read of a fresh graph is new, so with threshold 2 none passes,
+        // and the second time through every read passes
+        auto g6 = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(100000, 4), 21);
+        auto sf = StreamingSolidFilter<dBG<ByteStorage, CanLemireShifter>>::Filter::build(g6, 0.75, 2);
+        auto p4 = FastxParser<>::build(argv[1]);
+        auto b4 = p4->next_batch();
+        auto first = sf->filter_sequences(b4);
+        auto second = sf->filter_sequences(b4);
+        uint64_t n1 = 0, n2 = 0;
+        for (auto v : first) n1 += v;
+        for (auto v : second) n2 += v;
+        CHECK(n2 == b4.size() && n1 < n2);
     }
     std::printf("FACADE OK\n");
     return 0;
