@@ -539,11 +539,13 @@ struct ExactResolveOp {
 // hash value.
 // Counts are staged in LDS per tile and written out as one contiguous run
 // (the tile's windows have consecutive packed indices).
+template <int TILE>
 struct QueryStage {
-    int16_t out[kTileMax];
+    int16_t out[TILE];
     int64_t k0, k1;                   // packed indices of the tile's windows
 };
 
+template <int TILE>
 struct QueryOp {
     Tables T;
     ExactSet E;
@@ -551,7 +553,7 @@ struct QueryOp {
     int tile_len;
     int16_t* counts;
     uint64_t* hashes;
-    QueryStage* st;
+    QueryStage<TILE>* st;
     int64_t k0 = 0;
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
         const int64_t k = R.kmer_off[r] + (s - rstart);
@@ -607,27 +609,40 @@ struct HashOp {
 // (equal to hashextender.hh:352-364,386-396 shift-and-undo), counted as present
 // when their query is >= min_count (unitig_walker.hh:361-371 uses != 0, which
 // is min_count 1).  Windows not selected by `only` get 0xFF.
+// The `only` mask is read and the degrees are written per tile through LDS,
+// one contiguous run each (the tile's windows have consecutive packed
+// indices).
+template <int TILE>
+struct DegreeStage {
+    uint8_t only[TILE];
+    uint8_t in[TILE];
+    uint8_t out[TILE];
+    int64_t k0, k1;
+};
+
+template <int TILE>
 struct DegreeOp {
     Tables T;
     ExactSet E;
-    int canonical;
-    int K;
+    Reads R;
+    int tile_len;
     int min_count;
-    const int64_t* kmer_off;
     const uint8_t* only;
     uint8_t* in_deg;
     uint8_t* out_deg;
     const uint8_t* bytes;   // LDS tile
-    int64_t t0;
     const uint64_t* tf;
     const uint64_t* tr;
+    DegreeStage<TILE>* st;
+    int64_t t0 = 0, k0 = 0;
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
-        const int64_t k = kmer_off[r] + (s - rstart);
-        if (!valid || (only && !only[k])) {
-            in_deg[k] = 0xFF;
-            out_deg[k] = 0xFF;
+        const int64_t i = R.kmer_off[r] + (s - rstart) - k0;
+        if (!valid || (only && !st->only[i])) {
+            st->in[i] = 0xFF;
+            st->out[i] = 0xFF;
             return;
         }
+        const int K = R.K;
         const unsigned Kr = (unsigned)K & 63u;
         const uint8_t first = bytes[s - t0];
         const uint8_t last = bytes[s - t0 + K - 1];
@@ -640,17 +655,43 @@ struct DegreeOp {
             // right extension: shift_right(first, x)
             const uint64_t F1 = rf ^ tf[x];
             const uint64_t R1 = gt_rotr(rc ^ gt_rotl(tr[x], Kr) ^ tr[first], 1);
-            nout += storage_query(T, E, kmer_value(F1, R1, canonical)) >= min_count;
+            nout += storage_query(T, E, kmer_value(F1, R1, R.canonical)) >= min_count;
             // left extension: shift_left(x, last)
             const uint64_t F2 = gt_rotr(fw ^ gt_rotl(tf[x], Kr) ^ tf[last], 1);
             const uint64_t R2 = lr ^ tr[x];
-            nin += storage_query(T, E, kmer_value(F2, R2, canonical)) >= min_count;
+            nin += storage_query(T, E, kmer_value(F2, R2, R.canonical)) >= min_count;
         }
-        in_deg[k] = nin;
-        out_deg[k] = nout;
+        st->in[i] = nin;
+        st->out[i] = nout;
     }
-    __device__ void begin_tile(int64_t tile_base) { t0 = tile_base; }
+    __device__ void begin_tile(int64_t tile_base) {
+        t0 = tile_base;
+        if (threadIdx.x == 0) {
+            uint64_t n_reads;
+            int64_t n_bases;
+            batch_sizes(R, n_reads, n_bases);
+            int64_t t1 = t0 + tile_len;
+            if (t1 > R.b1) t1 = R.b1;
+            st->k0 = first_kidx(R, n_reads, n_bases, t0);
+            st->k1 = first_kidx(R, n_reads, n_bases, t1);
+        }
+        __syncthreads();
+        k0 = st->k0;
+        if (only) {
+            const int n = (int)(st->k1 - k0);
+            for (int j = threadIdx.x; j < n; j += kBlock) st->only[j] = only[k0 + j];
+            __syncthreads();
+        }
+    }
     __device__ void flush() {}
+    __device__ void end_tile(int64_t) {
+        __syncthreads();
+        const int n = (int)(st->k1 - k0);
+        for (int j = threadIdx.x; j < n; j += kBlock) {
+            in_deg[k0 + j] = st->in[j];
+            out_deg[k0 + j] = st->out[j];
+        }
+    }
 };
 
 // ------------------------------------------------------------ kernels
@@ -704,9 +745,9 @@ __global__ __launch_bounds__(kBlock) void exact_resolve_kernel(Reads R, ExactSet
 template <int PER>
 __global__ __launch_bounds__(kBlock) void query_kernel(Reads R, Tables T, ExactSet E, int16_t* counts,
                                                      uint64_t* hashes) {
-    __shared__ TileShared sh;
-    __shared__ QueryStage st;
-    QueryOp op{T, E, R, kBlock * PER, counts, hashes, &st};
+    __shared__ TileSharedT<kBlock * PER> sh;
+    __shared__ QueryStage<kBlock * PER> st;
+    QueryOp<kBlock * PER> op{T, E, R, kBlock * PER, counts, hashes, &st};
     walk_tiles<PER>(R, sh, op);
 }
 
@@ -720,9 +761,9 @@ __global__ __launch_bounds__(kBlock) void hash_kernel(Reads R, uint64_t* fw, uin
 template <int PER>
 __global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, ExactSet E, int min_count,
                                                       const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
-    __shared__ TileShared sh;
-    DegreeOp op{T, E, R.canonical, R.K, min_count, R.kmer_off, only, in_deg, out_deg,
-                sh.bytes, 0, sh.tf, sh.tr};
+    __shared__ TileSharedT<kBlock * PER> sh;
+    __shared__ DegreeStage<kBlock * PER> st;
+    DegreeOp<kBlock * PER> op{T, E, R, kBlock * PER, min_count, only, in_deg, out_deg, sh.bytes, sh.tf, sh.tr, &st};
     walk_tiles<PER>(R, sh, op);
 }
 
