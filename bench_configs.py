@@ -288,8 +288,9 @@ def fastx(args):
         del gr, proc
     out['value'] = out['large_interval']['kmers_per_s']
     out['unit'] = 'kmers/s'
-    out['note'] = ('plain FASTQ read by the block-parallel parser (host threads); each advance() inserts one '
-                   'interval of reads through gt_dbg_insert (host buffers, PCIe included)')
+    out['note'] = ('plain FASTQ read by the block-parallel parser (host threads) into pinned double buffers; '
+                   'the next batch is parsed while the GPU inserts the current one (PCIe included); '
+                   'advance() waits for its inserts at every interval')
     os.remove(path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)

@@ -27,6 +27,7 @@
 // while the current one is handed out.
 #include "goetia_amd.h"
 
+#include <hip/hip_runtime.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -671,6 +672,21 @@ int gt_fastx_close(gt_fastx* p) {
 
 // ------------------------------------------------------------ chunk driver
 
+namespace {
+
+// One batch slot: parsed reads in pinned host memory and their device copy.
+struct Slot {
+    uint8_t* h_seq = nullptr;
+    int64_t* h_off = nullptr;              // offsets as parsed
+    int64_t* h_sub = nullptr;              // offsets of the segment being inserted, rebased
+    uint8_t* d_seq = nullptr;
+    int64_t* d_off = nullptr;
+    hipEvent_t copied = nullptr;           // the host buffers may be refilled after this
+    uint64_t n = 0, cur = 0;               // parsed reads, next one to insert
+};
+
+}  // namespace
+
 struct gt_inserter {
     gt_storage* st = nullptr;
     uint16_t K = 0;
@@ -678,10 +694,52 @@ struct gt_inserter {
     uint64_t interval = 500000;
     uint64_t counter = 0, total = 0, n_sequences = 0;
     uint64_t max_reads = 0, max_bases = 0;
-    std::vector<uint8_t> seq;
-    std::vector<int64_t> off, sub_off;
-    uint64_t n = 0, cur = 0;              // parsed reads in the buffer, next one to insert
+    Slot slot[2];
+    int active = 0;                        // slot holding the reads being inserted
+    hipStream_t stream = nullptr;
+    bool pending_work = false;             // inserts enqueued since the last synchronisation
+
+    ~gt_inserter() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (Slot& sl : slot) {
+            if (sl.h_seq) (void)hipHostFree(sl.h_seq);
+            if (sl.h_off) (void)hipHostFree(sl.h_off);
+            if (sl.h_sub) (void)hipHostFree(sl.h_sub);
+            if (sl.d_seq) (void)hipFree(sl.d_seq);
+            if (sl.d_off) (void)hipFree(sl.d_off);
+            if (sl.copied) (void)hipEventDestroy(sl.copied);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
+
+namespace {
+
+#define GT_TRY_FX(expr)                 \
+    do {                                \
+        const int rc_ = (expr);         \
+        if (rc_ != GT_OK) return rc_;   \
+    } while (0)
+
+int hip_fail(hipError_t e, const char* what) {
+    return set_error(GT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GT_FX_HIP(call)                                       \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);     \
+    } while (0)
+
+int sync_inserts(gt_inserter* ins) {
+    if (ins->pending_work) {
+        GT_FX_HIP(hipStreamSynchronize(ins->stream));
+        ins->pending_work = false;
+    }
+    return GT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -699,13 +757,18 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     ins->interval = interval;
     ins->max_bases = batch_bases ? batch_bases : (64ULL << 20);
     ins->max_reads = std::max<uint64_t>(ins->max_bases / 64, 1024);
-    try {
-        ins->seq.resize(ins->max_bases);
-        ins->off.resize(ins->max_reads + 1);
-        ins->sub_off.resize(ins->max_reads + 1);
-    } catch (...) {
+    hipError_t e = hipStreamCreate(&ins->stream);
+    for (Slot& sl : ins->slot) {
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_seq, ins->max_bases);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_off, (ins->max_reads + 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_sub, (ins->max_reads + 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_seq, ins->max_bases);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_off, (ins->max_reads + 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
         delete ins;
-        return set_error(GT_ERR_OUT_OF_MEMORY, "gt_inserter_new buffers");
+        return set_error(GT_ERR_OUT_OF_MEMORY, std::string("gt_inserter_new: ") + hipGetErrorString(e));
     }
     *out = ins;
     return GT_OK;
@@ -715,32 +778,38 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
 // with IntervalCounter::poll (metrics.hh:126-135): reads are inserted until
 // the k-mers of the reads taken since the last interval reach `interval`
 // (each read counts len - K + 1, 0 when shorter than K); the call returns
-// right after the read that reaches it, or when the input is exhausted.
+// right after the read that reaches it, or when the input is exhausted, with
+// every insert finished.  In between, the next batch is parsed into the other
+// slot while the GPU inserts the current one.
 int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequences, uint64_t* time_total,
                         int* remaining) {
     if (!ins || !parser) return set_error(GT_ERR_INVALID_ARG, "null inserter or parser");
     for (;;) {
-        if (ins->cur >= ins->n) {
-            ins->n = ins->cur = 0;
+        Slot& sl = ins->slot[ins->active];
+        if (sl.cur >= sl.n) {
+            // refill this slot once its previous copy to the device is done
+            GT_FX_HIP(hipEventSynchronize(sl.copied));
+            sl.n = sl.cur = 0;
             uint64_t got = 0;
-            const int rc = gt_fastx_next_batch(parser, ins->max_reads, ins->max_bases, ins->seq.data(),
-                                               ins->off.data(), nullptr, 0, nullptr, &got);
+            const int rc = gt_fastx_next_batch(parser, ins->max_reads, ins->max_bases, sl.h_seq, sl.h_off, nullptr, 0,
+                                               nullptr, &got);
             // FileProcessor::handle_next (processors.hh:149-171) skips a record
             // with an invalid character even from a strict parser
             if (rc != GT_OK && rc != GT_ERR_INVALID_CHARACTER) return rc;
-            ins->n = got;
+            sl.n = got;
             if (got == 0 && rc == GT_ERR_INVALID_CHARACTER) continue;
             if (got == 0) {
+                GT_TRY_FX(sync_inserts(ins));
                 if (n_sequences) *n_sequences = ins->n_sequences;
                 if (time_total) *time_total = ins->total;
                 if (remaining) *remaining = 0;
                 return GT_OK;
             }
         }
-        uint64_t end = ins->cur;
+        uint64_t end = sl.cur;
         bool crossed = false;
-        while (end < ins->n) {
-            const int64_t len = ins->off[end + 1] - ins->off[end];
+        while (end < sl.n) {
+            const int64_t len = sl.h_off[end + 1] - sl.h_off[end];
             const uint64_t t = len >= ins->K ? (uint64_t)(len - ins->K + 1) : 0;
             ++end;
             ++ins->n_sequences;
@@ -752,14 +821,23 @@ int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequence
                 break;
             }
         }
-        const uint64_t nr = end - ins->cur;
-        const int64_t b0 = ins->off[ins->cur];
-        for (uint64_t i = 0; i <= nr; ++i) ins->sub_off[i] = ins->off[ins->cur + i] - b0;
-        const int rc = gt_dbg_insert(ins->st, ins->K, ins->canonical, ins->mode, ins->seq.data() + b0,
-                                     ins->sub_off.data(), nr, nullptr, nullptr, nullptr);
+        const uint64_t nr = end - sl.cur;
+        const int64_t b0 = sl.h_off[sl.cur];
+        const uint64_t nb = (uint64_t)(sl.h_off[end] - b0);
+        // the rebased offsets of this slot's previous segment may still be in flight
+        GT_FX_HIP(hipEventSynchronize(sl.copied));
+        for (uint64_t i = 0; i <= nr; ++i) sl.h_sub[i] = sl.h_off[sl.cur + i] - b0;
+        if (nb) GT_FX_HIP(hipMemcpyAsync(sl.d_seq, sl.h_seq + b0, nb, hipMemcpyHostToDevice, ins->stream));
+        GT_FX_HIP(hipMemcpyAsync(sl.d_off, sl.h_sub, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ins->stream));
+        GT_FX_HIP(hipEventRecord(sl.copied, ins->stream));
+        const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
+                                         nullptr, nullptr, ins->stream);
         if (rc != GT_OK) return rc;
-        ins->cur = end;
+        ins->pending_work = true;
+        sl.cur = end;
+        if (sl.cur >= sl.n) ins->active ^= 1;   // parse into the other slot while this one is inserted
         if (crossed) {
+            GT_TRY_FX(sync_inserts(ins));
             if (n_sequences) *n_sequences = ins->n_sequences;
             if (time_total) *time_total = ins->total;
             if (remaining) *remaining = 1;
