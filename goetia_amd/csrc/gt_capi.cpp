@@ -647,12 +647,13 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     const uint64_t groups_max = (tiles_max + kBinGroup - 1) / kBinGroup;
     const uint64_t n_ent = tiles_max * nt * kBinTile;
     const uint64_t cnt_max = ((uint64_t)P.n_d1 << P.shift2) * groups_max;
-    GT_TRY(sc.bin_ent1.ensure(n_ent * sizeof(uint64_t)));
-    GT_TRY(sc.bin_ent2.ensure(n_ent * sizeof(uint64_t)));
-    GT_TRY(sc.bin_off1.ensure((kDigits1 + 1) * tiles_max * sizeof(uint16_t)));
-    GT_TRY(sc.bin_cnt2.ensure((cnt_max + 1) * sizeof(uint32_t)));
-    GT_TRY(sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t)));
-    GT_TRY(sc.bin_kbase.ensure(sizeof(uint64_t)));
+    int rc = sc.bin_ent1.ensure(n_ent * sizeof(uint64_t));
+    if (rc == GT_OK) rc = sc.bin_ent2.ensure(n_ent * sizeof(uint64_t));
+    if (rc == GT_OK) rc = sc.bin_off1.ensure((kDigits1 + 1) * tiles_max * sizeof(uint16_t));
+    if (rc == GT_OK) rc = sc.bin_cnt2.ensure((cnt_max + 1) * sizeof(uint32_t));
+    if (rc == GT_OK) rc = sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t));
+    if (rc == GT_OK) rc = sc.bin_kbase.ensure(sizeof(uint64_t));
+    if (rc != GT_OK) return rc;
     const bool iq = iq_counts != nullptr;
     uint8_t* flags = new_flags;
     if (iq) {
@@ -726,7 +727,16 @@ int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8
     const bool binned = use_binned(st, sc, R);
     GT_TRY(prepare_reads(sc, R, new_flags != nullptr || binned, s));
     if (n_new) launch_fill_u64(n_new, R.n_reads, 0, s);
-    if (binned) return binned_insert_impl(st, sc, R, n_new, new_flags, nullptr, s);
+    if (binned) {
+        const int rc = binned_insert_impl(st, sc, R, n_new, new_flags, nullptr, s);
+        if (rc != GT_ERR_OUT_OF_MEMORY) return rc;
+        // not enough device memory for the event buffers: release them and
+        // insert with atomics on the tables instead (same results)
+        sc.bin_ent1.release();
+        sc.bin_ent2.release();
+        sc.bin_flags.release();
+        sc.bin_vq.release();
+    }
     uint64_t sub = 0;
     GT_TRY(prepare_claims(sc, st->kind, st->n_tables, (uint64_t)R.n_bases, &sub, s));
     const Launch L = make_launch(st, s);
@@ -1288,7 +1298,11 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         // of the same bins: the binned replay gives each event its bins'
         // counts just before it.
         GT_TRY(prepare_reads(st, R, true, s));
-        return binned_insert_impl(st, st->own, R, nullptr, nullptr, counts, s);
+        const int rc = binned_insert_impl(st, st->own, R, nullptr, nullptr, counts, s);
+        if (rc != GT_ERR_OUT_OF_MEMORY) return rc;
+        st->own.bin_ent1.release();     // no room for the event buffers: strict order below
+        st->own.bin_ent2.release();
+        st->own.bin_vq.release();
     }
     if (st->kind == kByte) {
         // More than four tables or 2^30 bins: hash the batch, then apply it in
