@@ -43,8 +43,8 @@ BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64,
 # HBM bytes per k-mer (read + write) measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload and scaled by a 1 GiB copy (the table and the
 # commands are in profiles/cms_20m_hbm_traffic.md).
-MEASURED_BYTES_PER_KMER = {'bin_hash': 36.2, 'bin_count': 37.1, 'bin_route': 83.7, 'bin_apply': 64.0, 'bin_new': 1.2,
-                           'query': 515.3}
+MEASURED_BYTES_PER_KMER = {'bin_hash': 36.2, 'bin_count': 37.1, 'bin_route': 91.1, 'bin_apply': 64.0, 'bin_new': 1.2,
+                           'query': 515.5}
 
 
 def log(*a):
