@@ -149,6 +149,8 @@ SIGNATURES = {
     'gt_gen_reads': [u64, vp, u64, u64, vp, u32, u32, C.c_double, C.c_double, vp, vp, u64p],
     'gt_solid_filter_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, C.c_float, u32, vp, vp],
     'gt_solid_filter': [st_p, u16, cint, cint, vp, vp, u64, C.c_float, u32, vp],
+    'gt_diginorm_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, u32, vp, vp],
+    'gt_diginorm': [st_p, u16, cint, cint, vp, vp, u64, u32, vp],
     'gt_fastx_open': [C.c_char_p, cint, u32, C.POINTER(vp)],
     'gt_fastx_next_batch': [vp, u64, u64, vp, vp, vp, u64, vp, u64p],
     'gt_fastx_stats': [vp, u64p, u64p, u64p, C.POINTER(cint)],

@@ -226,6 +226,9 @@ struct gt_storage {
     Scratch own;                       // default scratch of calls on this storage
     DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
     DevBuf solid_counts;               // insert_and_query counts of a solid-filter batch
+    // batched diginorm: read of every routed event and window, windows with
+    // events, windows failing the cutoff, per-read state, iteration control
+    DevBuf dn_rid, dn_wread, dn_hasev, dn_fails, dn_state, dn_ctl;
     uint64_t gen_issued = 0;           // generation bumps enqueued so far (device kGen follows it)
     uint64_t epoch = 0;                // bumped when device table pointers change (exact-set growth)
     std::vector<struct gt_pipeline*> pipelines;   // attached pipelines
@@ -620,14 +623,11 @@ uint64_t bin_sub_batch(uint64_t n_bases) {
     return std::min<uint64_t>(sb, (n_bases + kTileMax - 1) / kTileMax * kTileMax);
 }
 
-// Binned insert of prepared reads (tile_read and kmer_off set): per
-// sub-batch, partition the (window, table) events by bin and replay each
-// bucket in LDS; then per-read new counts from the window flags, or, with
-// iq_counts (count-min only), the insert_and_query count of every window.
-int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_new, uint8_t* new_flags,
-                       int16_t* iq_counts, hipStream_t s) {
+// Plan of the binned passes over this storage's tables (buckets, digits) and
+// scratch for sub-batches of up to tiles_max first-pass tiles.
+int bin_prepare(gt_storage* st, Scratch& sc, uint64_t tiles_max, BinPlan& P) {
     const uint32_t nt = st->n_tables;
-    BinPlan P{};
+    P = BinPlan{};
     P.n_tables = nt;
     uint32_t nb = 0;
     for (uint32_t i = 0; i < nt; ++i) {
@@ -642,8 +642,6 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     P.shift2 = bits > (uint32_t)kDigitBits1 ? bits - (uint32_t)kDigitBits1 : 0u;
     P.n_d1 = (nb + (1u << P.shift2) - 1) >> P.shift2;
 
-    const uint64_t sub = bin_sub_batch((uint64_t)R.n_bases);
-    const uint64_t tiles_max = sub / kBinTile;
     const uint64_t groups_max = (tiles_max + kBinGroup - 1) / kBinGroup;
     const uint64_t n_ent = tiles_max * nt * kBinTile;
     const uint64_t cnt_max = ((uint64_t)P.n_d1 << P.shift2) * groups_max;
@@ -654,6 +652,54 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     if (rc == GT_OK) rc = sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t));
     if (rc == GT_OK) rc = sc.bin_kbase.ensure(sizeof(uint64_t));
     if (rc != GT_OK) return rc;
+    P.ent1 = sc.bin_ent1.as<uint64_t>();
+    P.ent2 = sc.bin_ent2.as<uint64_t>();
+    P.off1 = sc.bin_off1.as<uint16_t>();
+    P.cnt2 = sc.bin_cnt2.as<uint32_t>();
+    P.kbase = sc.bin_kbase.as<uint64_t>();
+    return GT_OK;
+}
+
+// Routes the (window, table) events of the windows starting in [b0, b1) of
+// prepared reads (at most the prepared tiles_max tiles): by bucket, in
+// window order within a bucket (first pass, second-pass count and scan,
+// second-pass scatter).
+void bin_route(const Reads& R, const Tables& T, BinPlan& P, Scratch& sc, int64_t b0, int64_t b1, hipStream_t s) {
+    Reads Rs = R;
+    Rs.b0 = b0;
+    Rs.b1 = b1;
+    P.tile0 = b0 / kBinTile;
+    P.tiles1 = (uint32_t)((b1 + kBinTile - 1) / kBinTile - P.tile0);
+    P.n_groups = (P.tiles1 + kBinGroup - 1) / kBinGroup;
+    const uint64_t cnt_len = ((uint64_t)P.n_d1 << P.shift2) * P.n_groups;
+    const int grid_l1 = std::max(1, g_dev.cus * 2);     // 70 KB of LDS per workgroup
+    const int grid = std::max(1, g_dev.cus * 8);
+    launch_bin_kbase(R, b0, sc.bin_kbase.as<uint64_t>(), s);
+    {
+        ProfScope ps("bin_hash", s);
+        launch_bin_l1(Rs, T, P, grid_l1, s);
+    }
+    {
+        ProfScope ps("bin_count", s);
+        launch_bin_l2_count(P, grid, s);
+        launch_scan_u32(P.cnt2, cnt_len, sc.bin_sums.as<uint32_t>(), s);
+    }
+    {
+        ProfScope ps("bin_route", s);
+        launch_bin_l2_scatter(P, grid, s);
+    }
+}
+
+// Binned insert of prepared reads (tile_read and kmer_off set): per
+// sub-batch, route the (window, table) events by bin and replay each bucket
+// in LDS; then per-read new counts from the window flags, or, with iq_counts
+// (count-min only), the insert_and_query count of every window.
+int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_new, uint8_t* new_flags,
+                       int16_t* iq_counts, hipStream_t s) {
+    const uint32_t nt = st->n_tables;
+    const uint64_t sub = bin_sub_batch((uint64_t)R.n_bases);
+    BinPlan P;
+    GT_TRY(bin_prepare(st, sc, sub / kBinTile, P));
     const bool iq = iq_counts != nullptr;
     uint8_t* flags = new_flags;
     if (iq) {
@@ -667,40 +713,14 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
         }
         launch_zero_bytes(flags, R.kmer_off + R.n_reads, (uint64_t)R.n_bases, s);
     }
-    P.ent1 = sc.bin_ent1.as<uint64_t>();
-    P.ent2 = sc.bin_ent2.as<uint64_t>();
-    P.off1 = sc.bin_off1.as<uint16_t>();
-    P.cnt2 = sc.bin_cnt2.as<uint32_t>();
-    P.kbase = sc.bin_kbase.as<uint64_t>();
     P.flags = flags;
     P.vq = iq ? sc.bin_vq.as<uint8_t>() : nullptr;
 
     const Tables T = st->tables_view();
     uint64_t* ctr = st->counters.as<uint64_t>();
-    const int grid_l1 = std::max(1, g_dev.cus * 2);     // 70 KB of LDS per workgroup
     const int grid = std::max(1, g_dev.cus * 8);
     for (int64_t b0 = 0; b0 < R.n_bases; b0 += (int64_t)sub) {
-        Reads Rs = R;
-        Rs.b0 = b0;
-        Rs.b1 = std::min<int64_t>(R.n_bases, b0 + (int64_t)sub);
-        P.tile0 = b0 / kBinTile;
-        P.tiles1 = (uint32_t)((Rs.b1 - b0 + kBinTile - 1) / kBinTile);
-        P.n_groups = (P.tiles1 + kBinGroup - 1) / kBinGroup;
-        const uint64_t cnt_len = ((uint64_t)P.n_d1 << P.shift2) * P.n_groups;
-        launch_bin_kbase(R, b0, sc.bin_kbase.as<uint64_t>(), s);
-        {
-            ProfScope ps("bin_hash", s);
-            launch_bin_l1(Rs, T, P, grid_l1, s);
-        }
-        {
-            ProfScope ps("bin_count", s);
-            launch_bin_l2_count(P, grid, s);
-            launch_scan_u32(P.cnt2, cnt_len, sc.bin_sums.as<uint32_t>(), s);
-        }
-        {
-            ProfScope ps("bin_route", s);
-            launch_bin_l2_scatter(P, grid, s);
-        }
+        bin_route(R, T, P, sc, b0, std::min<int64_t>(R.n_bases, b0 + (int64_t)sub), s);
         {
             ProfScope ps("bin_apply", s);
             launch_bin_apply(P, T, ctr, grid, s);
@@ -1389,6 +1409,154 @@ int gt_solid_filter(gt_storage* st, uint16_t K, int canonical, int mode, const u
     GT_TRY(dpass.ensure(n_reads));
     GT_TRY(gt_solid_filter_dev(st, K, canonical, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads, n_bases,
                                min_prop_solid, solid_threshold, dpass.as<uint8_t>(), s));
+    GT_TRY(stage_out(passed, dpass, n_reads, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return check_kernel_errors(st, s);
+}
+
+namespace {
+
+// Start positions per diginorm chunk (GOETIA_AMD_DIGINORM_CHUNK, default
+// 2^25, at least one first-pass tile) and the number of guesses after which
+// a chunk is cut at its first undecided read (GOETIA_AMD_DIGINORM_GUESSES,
+// default 32, at least 2).
+uint64_t dn_chunk_starts() {
+    uint64_t v = 1ULL << 25;
+    if (const char* e = std::getenv("GOETIA_AMD_DIGINORM_CHUNK")) {
+        const unsigned long long x = std::strtoull(e, nullptr, 10);
+        if (x) v = std::min<uint64_t>(x, 1ULL << 28);
+    }
+    return std::max<uint64_t>(v, kBinTile);
+}
+
+int dn_guess_cap() {
+    int v = 32;
+    if (const char* e = std::getenv("GOETIA_AMD_DIGINORM_GUESSES")) v = std::atoi(e);
+    return std::max(v, 2);
+}
+
+}  // namespace
+
+int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                    uint64_t n_reads, uint64_t n_bases, uint32_t cutoff, uint8_t* passed, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (st->kind != kByte) return fail(GT_ERR_UNSUPPORTED, "DiginormFilter needs a counting (ByteStorage) graph");
+    if (!bin_buckets(st))
+        return fail(GT_ERR_UNSUPPORTED, "DiginormFilter needs at most 4 tables of at most 2^30 bins in all");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0) return GT_OK;
+    if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    hipStream_t s = (hipStream_t)stream;
+    // chunks end at read boundaries: the offsets come to the host once
+    std::vector<int64_t> off(n_reads + 1);
+    GT_HIP(hipMemcpyAsync(off.data(), offsets, (n_reads + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    GT_TRY(check_host_offsets(off.data(), n_reads));
+    if ((uint64_t)off[n_reads] != n_bases) return fail(GT_ERR_INVALID_ARG, "offsets[n_reads] must equal n_bases");
+    std::vector<int64_t> kof(n_reads + 1, 0);      // packed windows before each read (as kmer_off)
+    for (uint64_t r = 0; r < n_reads; ++r) kof[r + 1] = kof[r] + std::max<int64_t>(off[r + 1] - off[r] - K + 1, 0);
+    const int64_t Tl = kBinTile;
+    auto tiles = [&](uint64_t a, uint64_t b) { return (uint64_t)((off[b] + Tl - 1) / Tl - off[a] / Tl); };
+    uint64_t tiles_max = dn_chunk_starts() / kBinTile;
+    for (uint64_t r = 0; r < n_reads; ++r) tiles_max = std::max(tiles_max, tiles(r, r + 1));
+
+    Scratch& sc = st->own;
+    Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    GT_TRY(prepare_reads(sc, R, true, s));
+    BinPlan P;
+    GT_TRY(bin_prepare(st, sc, tiles_max, P));
+    const uint64_t ev_max = tiles_max * st->n_tables * kBinTile;
+    const uint64_t win_max = tiles_max * kBinTile;
+    GT_TRY(st->dn_rid.ensure(ev_max * sizeof(uint32_t)));
+    GT_TRY(st->dn_wread.ensure(win_max * sizeof(uint32_t)));
+    GT_TRY(st->dn_hasev.ensure(win_max));
+    GT_TRY(st->dn_fails.ensure(win_max));
+    GT_TRY(st->dn_state.ensure(n_reads));
+    GT_TRY(st->dn_ctl.ensure(2 * sizeof(uint32_t)));
+    GT_TRY(sc.bin_flags.ensure(std::max<uint64_t>((uint64_t)kof[n_reads], 1)));
+    uint32_t* rid = st->dn_rid.as<uint32_t>();
+    uint32_t* wread = st->dn_wread.as<uint32_t>();
+    uint8_t* hasev = st->dn_hasev.as<uint8_t>();
+    uint8_t* fails = st->dn_fails.as<uint8_t>();
+    uint8_t* state = st->dn_state.as<uint8_t>();
+    uint32_t* ctl = st->dn_ctl.as<uint32_t>();
+    uint8_t* flags = sc.bin_flags.as<uint8_t>();
+    P.flags = flags;
+    P.vq = nullptr;
+    P.rid = rid;
+    P.keep = state;
+    const Tables T = st->tables_view();
+    uint64_t* ctr = st->counters.as<uint64_t>();
+    const int grid = std::max(1, g_dev.cus * 8);
+    const int cap = dn_guess_cap();
+
+    uint64_t r0 = 0;
+    while (r0 < n_reads) {
+        uint64_t r1 = r0 + 1;
+        while (r1 < n_reads && tiles(r0, r1 + 1) <= tiles_max) ++r1;
+        launch_dn_init(R.off, R.kmer_off, (uint32_t)r0, (uint32_t)r1, K, wread, state, s);
+        uint64_t r_end = r1;
+        const uint64_t n_win = (uint64_t)(kof[r1] - kof[r0]);
+        if (n_win) {
+            bin_route(R, T, P, sc, off[r0], off[r1], s);
+            launch_fill_u8(hasev, n_win, 0, s);
+            launch_dn_event_reads(P, wread, rid, hasev, n_win * st->n_tables, s);
+            bool settled = false;
+            uint32_t h_ctl[2] = {0, 0};
+            for (int g = 1; g <= cap && !settled; ++g) {
+                {
+                    ProfScope ps("dn_replay", s);
+                    launch_fill_u8(fails, n_win, 0, s);
+                    launch_dn_replay(P, T, rid, state, cutoff, fails, grid, s);
+                }
+                {
+                    ProfScope ps("dn_decide", s);
+                    launch_dn_ctl_reset(ctl, s);
+                    launch_dn_decide(R.off, R.kmer_off, (uint32_t)r0, (uint32_t)r1, K, hasev, fails, g & 1, state,
+                                     ctl, s);
+                }
+                GT_HIP(hipGetLastError());
+                GT_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(h_ctl), hipMemcpyDeviceToHost, s));
+                GT_HIP(hipStreamSynchronize(s));
+                settled = h_ctl[0] == 0;
+            }
+            if (!settled) {
+                // reads before the first undecided one are final; the rest
+                // start the next chunk
+                if (h_ctl[1] <= r0 || h_ctl[1] > r1) return fail(GT_ERR_INTERNAL, "diginorm: no read was decided");
+                r_end = h_ctl[1];
+            }
+            // insert the kept reads of [r0, r_end) in read order
+            ProfScope ps("dn_apply", s);
+            launch_fill_u8(flags + kof[r0], (uint64_t)(kof[r_end] - kof[r0]), 0, s);
+            P.r_end = (uint32_t)r_end;
+            launch_dn_apply(P, T, ctr, grid, s);
+            launch_bin_read_new(R.kmer_off + r0, r_end - r0, flags, nullptr, ctr, s);
+        }
+        launch_dn_passed(state, (uint32_t)r0, (uint32_t)r_end, passed, s);
+        GT_HIP(hipGetLastError());
+        r0 = r_end;
+    }
+    return GT_OK;
+}
+
+int gt_diginorm(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                uint64_t n_reads, uint32_t cutoff, uint8_t* passed) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    if (n_reads == 0) return GT_OK;
+    DevBuf dseq, doff, dpass;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    GT_TRY(dpass.ensure(n_reads));
+    GT_TRY(gt_diginorm_dev(st, K, canonical, mode, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads, n_bases, cutoff,
+                           dpass.as<uint8_t>(), s));
     GT_TRY(stage_out(passed, dpass, n_reads, s));
     GT_HIP(hipStreamSynchronize(s));
     return check_kernel_errors(st, s);

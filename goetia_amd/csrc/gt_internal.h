@@ -110,6 +110,11 @@ struct BinPlan {
     uint8_t* flags;                       // per packed window: 1 = new k-mer
     uint8_t* vq;                          // insert_and_query: per packed window, kBinMaxTables
                                           // counts just after the event (flags unused then)
+    // masked replay (batched diginorm): only events of reads r < r_end with
+    // keep[r] & 1 are applied; rid[i] is the read of ent2 event i
+    const uint32_t* rid;
+    const uint8_t* keep;
+    uint32_t r_end;
 };
 
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the

@@ -1165,6 +1165,7 @@ struct BinL1Op {
     uint64_t kbase;
     BinL1Shared* sh;
     uint64_t* slots;          // 4 x kDigits1 match slots (the walker's read-offset cache, idle here)
+    int64_t b_lo;             // first start of the sub-batch (tiles may begin before it)
     int64_t t0 = 0;
 
     __device__ void begin_tile(int64_t tile_base) {
@@ -1175,7 +1176,7 @@ struct BinL1Op {
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
-        if (!valid) return;
+        if (!valid || s < b_lo) return;
         const int w = (int)(s - t0);
         const uint64_t h = kmer_value(fw, rc, canonical);
         sh->ord[w] = (uint32_t)((uint64_t)(kmer_off[r] + (s - rstart)) - kbase);
@@ -1253,7 +1254,7 @@ struct BinL1Op {
 __global__ __launch_bounds__(kBlock) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
     __shared__ TileSharedT<kBinTile> sh;
     __shared__ BinL1Shared bs;
-    BinL1Op op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff)};
+    BinL1Op op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
     walk_tiles<kBinPer>(R, sh, op);
 }
 
@@ -1456,10 +1457,63 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
     }
 }
 
-// Replay: one wavefront per bucket (four per workgroup).
-__global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, uint64_t* counters) {
-    __shared__ uint32_t slice[4][kBucketBins / 4];
-    __shared__ uint64_t slots[4][256];
+// Lanes of the wavefront on the same bin of a bucket (key < kBucketBins):
+// the low 8 bits through LDS slots, the rest by ballot.
+__device__ __forceinline__ uint64_t bin_match(uint64_t* slots, uint32_t key, bool active) {
+    uint64_t m = match_lds(slots, key & 255u, active);
+#pragma unroll
+    for (int k = 8; k < kBinBits; ++k) {
+        const bool bit = (key >> k) & 1u;
+        const uint64_t b = __ballot(bit);
+        m &= bit ? b : ~b;
+    }
+    return m;
+}
+
+// Stages the bins of bucket b of byte tables in LDS (sw: kBucketBins / 4
+// words); returns the table of the bucket and the staged words' source.
+__device__ __forceinline__ uint32_t* bin_stage_slice(const BinPlan& P, const Tables& T, uint32_t b, uint32_t* sw,
+                                                     uint32_t* table, uint32_t* n_words) {
+    const int lane = threadIdx.x & 63;
+    uint32_t t = 0;
+    uint64_t pt = T.p[0];
+    uint8_t* base = T.base[0];
+#pragma unroll
+    for (uint32_t i = 1; i < kBinMaxTables; ++i) {
+        if (i < P.n_tables && b >= P.bfirst[i]) {
+            t = i;
+            pt = T.p[i];
+            base = T.base[i];
+        }
+    }
+    const uint64_t bin0 = (uint64_t)(b - P.bfirst[t]) << kBinBits;
+    const uint64_t left = pt - bin0;
+    const uint32_t nb = (uint32_t)(left < (uint64_t)kBucketBins ? left : (uint64_t)kBucketBins);
+    const uint32_t nw = (nb + 3u) / 4u;
+    uint32_t* src = reinterpret_cast<uint32_t*>(base + bin0);
+    for (uint32_t w0 = 0; w0 < nw; w0 += 8 * 64) {
+        uint32_t x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t w = w0 + (uint32_t)q * 64 + (uint32_t)lane;
+            x[q] = w < nw ? src[w] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t w = w0 + (uint32_t)q * 64 + (uint32_t)lane;
+            if (w < nw) sw[w] = x[q];
+        }
+    }
+    *table = t;
+    *n_words = nw;
+    return src;
+}
+
+// Replay: one wavefront per bucket (four per workgroup).  Masked: only the
+// events of kept reads before P.r_end take part (batched diginorm).
+template <bool kMasked>
+__device__ __forceinline__ void bin_apply_body(const BinPlan& P, const Tables& T, uint64_t* counters,
+                                               uint32_t (*slice)[kBucketBins / 4], uint64_t (*slots)[256]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lanes_below();
     const uint64_t kb = *P.kbase;
@@ -1511,7 +1565,11 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
         }
         for (uint32_t c = lo_e; c < hi_e; c += 64) {
             const uint32_t i = c + (uint32_t)lane;
-            const bool v = i < hi_e;
+            bool v = i < hi_e;
+            if (kMasked && v) {
+                const uint32_t r = P.rid[i];
+                v = r < P.r_end && (P.keep[r] & 1u);
+            }
             const uint64_t e = pf[0];
 #pragma unroll
             for (int q = 0; q + 1 < kAhead; ++q) pf[q] = pf[q + 1];
@@ -1520,14 +1578,7 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
                 pf[kAhead - 1] = nx < hi_e ? P.ent2[nx] : 0;
             }
             const uint32_t key = v ? ((uint32_t)(e >> 32) & (kBucketBins - 1)) : 0u;
-            // lanes on the same bin: low 8 bits through LDS slots, the rest by ballot
-            uint64_t m = match_lds(slots[wave], key & 255u, v);
-#pragma unroll
-            for (int k = 8; k < kBinBits; ++k) {
-                const bool bit = (key >> k) & 1u;
-                const uint64_t b = __ballot(bit);
-                m &= bit ? b : ~b;
-            }
+            const uint64_t m = bin_match(slots[wave], key, v);
             const uint32_t rank = (uint32_t)__popcll(m & lt);
             if (v) {
                 if (bytes) {
@@ -1561,6 +1612,18 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
         __builtin_amdgcn_wave_barrier();
     }
     add_counter(counters + kNOccupied, occ);
+}
+
+__global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, uint64_t* counters) {
+    __shared__ uint32_t slice[4][kBucketBins / 4];
+    __shared__ uint64_t slots[4][256];
+    bin_apply_body<false>(P, T, counters, slice, slots);
+}
+
+__global__ __launch_bounds__(kBlock) void dn_apply_kernel(BinPlan P, Tables T, uint64_t* counters) {
+    __shared__ uint32_t slice[4][kBucketBins / 4];
+    __shared__ uint64_t slots[4][256];
+    bin_apply_body<true>(P, T, counters, slice, slots);
 }
 
 // Per-read new k-mers from the window flags; adds the total to n_unique.  A
@@ -1644,6 +1707,206 @@ __global__ void solid_reads_kernel(const int64_t* off, const int64_t* kmer_off, 
         const bool fails = (double)((float)n_not_solid / (float)n_kmers) >= (1.0 - (double)min_prop_solid);
         passed[r] = fails ? 0 : 1;
     }
+}
+
+// ------------------------------------------------------------ batched diginorm
+//
+// DiginormFilter::Filter::filter_sequence (diginorm.hh:105-119) drops a read
+// when median_count_at_least holds (diginorm.hh:39-76): at least
+// min_req = (unsigned)(0.5 + float(n_kmers) / 2) of its k-mers have a count of
+// at least the cutoff; every other read is inserted before the next read is
+// looked at.  A chunk of reads is resolved together: a state byte per read
+// holds the current guess of the kept set (kDnKept) and whether the read's
+// decision is final (kDnSettled).  One replay of the chunk's events gives every
+// k-mer of every unsettled read the count it would see under the guess (the
+// tables before the chunk plus the guessed-kept reads before it), and the
+// decision pass makes the next guess from those counts.
+
+constexpr uint8_t kDnKept = 1, kDnSettled = 2;
+
+// Read of every window of reads [r0, r1) (wread, indexed by window - the
+// chunk's first window) and the reads' starting state: reads shorter than K
+// are final and not kept (the reference's k-mer iterator raises
+// SequenceLengthException, kmeriterator.hh:72-74, and FilterProcessor skips
+// the read, processors.hh:394-401); the others start not kept.  One wavefront
+// per read.
+__global__ void dn_init_kernel(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
+                               uint32_t* wread, uint8_t* state) {
+    const int64_t kb = kmer_off[r0];
+    const int lane = threadIdx.x & 63;
+    const uint64_t wpb = blockDim.x / 64;
+    for (uint64_t r = r0 + (uint64_t)blockIdx.x * wpb + threadIdx.x / 64; r < r1; r += (uint64_t)gridDim.x * wpb) {
+        const int64_t a = kmer_off[r] - kb, b = kmer_off[r + 1] - kb;
+        for (int64_t w = a + lane; w < b; w += 64) wread[w] = (uint32_t)r;
+        if (lane == 0) state[r] = off[r + 1] - off[r] < K ? kDnSettled : 0;
+    }
+}
+
+// Read of every event of the chunk (rid, aligned with ent2), and which
+// windows have events (in skip-N mode windows holding a non-ACGT byte have
+// none).
+__global__ void dn_event_reads_kernel(BinPlan P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev) {
+    const uint32_t n = P.cnt2[(size_t)P.n_buckets * P.n_groups];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t ord = (uint32_t)P.ent2[i];
+        rid[i] = wread[ord];
+        hasev[ord] = 1;
+    }
+}
+
+// The events [pos, ...) of one read rr that runs past a 64-event step: each
+// sees its bin as it was before the read; a kept read then raises its bins.
+// Returns the end of the read's events.
+__device__ uint32_t dn_long_read(const BinPlan& P, const uint32_t* rid, uint8_t st, uint32_t cutoff, uint8_t* fails,
+                                 uint8_t* sb, uint64_t* sl, uint32_t pos, uint32_t hi_e, uint32_t rr) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below();
+    uint32_t q = pos;
+    for (;;) {
+        const uint32_t i = q + (uint32_t)lane;
+        const bool in = i < hi_e && rid[i] == rr;          // a prefix of the lanes
+        const uint32_t cnt = (uint32_t)__popcll(__ballot(in));
+        if (in && !(st & kDnSettled)) {
+            const uint64_t e = P.ent2[i];
+            if ((uint32_t)sb[(uint32_t)(e >> 32) & (kBucketBins - 1)] < cutoff) fails[(uint32_t)e] = 1;
+        }
+        q += cnt;
+        if (cnt < 64) break;
+    }
+    if (st & kDnKept) {
+        for (uint32_t c = pos; c < q; c += 64) {
+            const uint32_t i = c + (uint32_t)lane;
+            const bool act = i < q;
+            const uint32_t key = act ? ((uint32_t)(P.ent2[i] >> 32) & (kBucketBins - 1)) : 0u;
+            const uint64_t m = bin_match(sl, key, act);
+            if (act && !(m & lt)) {
+                const uint32_t post = sb[key] + (uint32_t)__popcll(m);
+                sb[key] = (uint8_t)(post < 255u ? post : 255u);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return q;
+}
+
+// One replay of the chunk (one wavefront per bucket, the bucket's slice of the
+// tables as they were before the chunk staged in LDS, never written back).
+// Events are taken in read order, whole reads at a time: all k-mers of a read
+// are queried before any of them is inserted (diginorm.hh:105-116), so an
+// event sees its bin's count before its read; an event of an unsettled read
+// whose count is below the cutoff marks its window in `fails`; the events of a
+// guessed-kept read then raise their bins, saturating at 255
+// (bytestorage.cc:60-113).  The events of one read are consecutive within a
+// bucket; a step takes the whole reads among the next 64 events.
+__global__ __launch_bounds__(kBlock) void dn_replay_kernel(BinPlan P, Tables T, const uint32_t* rid,
+                                                           const uint8_t* state, uint32_t cutoff, uint8_t* fails) {
+    __shared__ uint32_t slice[4][kBucketBins / 4];
+    __shared__ uint64_t slots[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanes_below();
+    const uint64_t le = lt | (1ULL << lane);
+    uint32_t* sw = slice[wave];
+    uint8_t* sb = reinterpret_cast<uint8_t*>(sw);
+    for (uint32_t b = blockIdx.x * 4u + (uint32_t)wave; b < P.n_buckets; b += gridDim.x * 4u) {
+        const uint32_t lo_e = P.cnt2[(size_t)b * P.n_groups];
+        const uint32_t hi_e = P.cnt2[(size_t)(b + 1) * P.n_groups];
+        if (lo_e == hi_e) continue;
+        uint32_t t, nw;
+        bin_stage_slice(P, T, b, sw, &t, &nw);
+        __builtin_amdgcn_wave_barrier();
+        uint32_t pos = lo_e;
+        while (pos < hi_e) {
+            const uint32_t i = pos + (uint32_t)lane;
+            const bool v = i < hi_e;
+            const uint64_t e = v ? P.ent2[i] : 0;
+            const uint32_t r = v ? rid[i] : 0xFFFFFFFFu;
+            const uint32_t rn = pos + 64u < hi_e ? rid[pos + 64u] : 0xFFFFFFFFu;
+            const uint32_t rp = __shfl_up(r, 1);
+            const uint64_t bmask = __ballot(v && (lane == 0 || r != rp));   // first event of each read
+            uint32_t np = hi_e - pos < 64u ? hi_e - pos : 64u;
+            if (np == 64u && __shfl(r, 63) == rn) {
+                // the last read of the step goes on past it
+                const uint32_t s63 = 63u - (uint32_t)__clzll(bmask);
+                if (s63 == 0) {
+                    pos = dn_long_read(P, rid, state[__shfl(r, 0)], cutoff, fails, sb, slots[wave], pos, hi_e,
+                                       __shfl(r, 0));
+                    continue;
+                }
+                np = s63;
+            }
+            const bool act = (uint32_t)lane < np;
+            const uint32_t key = act ? ((uint32_t)(e >> 32) & (kBucketBins - 1)) : 0u;
+            const uint64_t m = bin_match(slots[wave], key, act);
+            const uint8_t st = act ? state[r] : (uint8_t)0;
+            const uint64_t km = __ballot(act && (st & kDnKept));
+            const uint32_t first = 63u - (uint32_t)__clzll(bmask & le);      // first lane of my read
+            if (act) {
+                uint32_t seen = sb[key] + (uint32_t)__popcll(m & km & ((1ULL << first) - 1ULL));
+                seen = seen < 255u ? seen : 255u;
+                if (!(st & kDnSettled) && seen < cutoff) fails[(uint32_t)e] = 1;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (act && (st & kDnKept) && !(m & km & lt)) {
+                const uint32_t post = sb[key] + (uint32_t)__popcll(m & km);
+                sb[key] = (uint8_t)(post < 255u ? post : 255u);
+            }
+            __builtin_amdgcn_wave_barrier();
+            pos += np;
+        }
+    }
+}
+
+// The next guess for every unsettled read of [r0, r1): C = its windows that
+// have events and no mark in `fails`; the read is dropped when C >= min_req,
+// computed as the reference does (diginorm.hh:43).  With `upper` set the new
+// guess is an upper bound of the kept set (the one replayed was a lower
+// bound), so its drops are final; otherwise its keeps are.  ctl[0] counts
+// reads whose guess changed, ctl[1] is the first read not yet final.  One
+// wavefront per read.
+__global__ void dn_decide_kernel(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
+                                 const uint8_t* hasev, const uint8_t* fails, int upper, uint8_t* state,
+                                 uint32_t* ctl) {
+    const int64_t kb = kmer_off[r0];
+    const int lane = threadIdx.x & 63;
+    const uint64_t wpb = blockDim.x / 64;
+    uint32_t changed = 0;
+    for (uint64_t r = r0 + (uint64_t)blockIdx.x * wpb + threadIdx.x / 64; r < r1; r += (uint64_t)gridDim.x * wpb) {
+        const uint8_t st = state[r];
+        if (st & kDnSettled) continue;
+        const int64_t a = kmer_off[r] - kb, b = kmer_off[r + 1] - kb;
+        uint32_t c = 0;
+        for (int64_t w = a + lane; w < b; w += 64) c += (hasev[w] && !fails[w]) ? 1u : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
+        const uint64_t n_kmers = (uint64_t)(off[r + 1] - off[r]) - (uint64_t)K + 1u;
+        const uint32_t min_req = (uint32_t)(0.5 + (double)((float)n_kmers / 2));
+        const bool keep = c < min_req;
+        if (lane == 0) {
+            const bool settled = upper ? !keep : keep;
+            changed += keep != ((st & kDnKept) != 0);
+            state[r] = (uint8_t)((keep ? kDnKept : 0) | (settled ? kDnSettled : 0));
+            if (!settled) atomicMin(&ctl[1], (uint32_t)r);
+        }
+    }
+    if (lane == 0 && changed) atomicAdd(&ctl[0], changed);
+}
+
+__global__ void dn_ctl_reset_kernel(uint32_t* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[0] = 0;
+        ctl[1] = 0xFFFFFFFFu;
+    }
+}
+
+__global__ void dn_passed_kernel(const uint8_t* state, uint32_t r0, uint32_t r1, uint8_t* passed) {
+    for (uint64_t r = r0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < r1; r += (uint64_t)gridDim.x * blockDim.x)
+        passed[r] = state[r] & kDnKept;
+}
+
+// p[0, n) = v.
+__global__ void fill_u8_kernel(uint8_t* p, uint64_t n, uint8_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // Zeroes p[0, *n) (32-bit words).
@@ -1894,6 +2157,49 @@ void launch_bin_read_new(const int64_t* kmer_off, uint64_t n_reads, const uint8_
     const uint64_t g = (n_reads + kBlock - 1) / kBlock;
     bin_read_new_kernel<<<(unsigned)(g < 4096 ? (g ? g : 1) : 4096), kBlock, 0, s>>>(kmer_off, n_reads, flags, n_new,
                                                                                   counters);
+}
+
+static unsigned grid_of(uint64_t units, uint64_t per_block, uint64_t cap) {
+    const uint64_t g = (units + per_block - 1) / per_block;
+    return (unsigned)(g < cap ? (g ? g : 1) : cap);
+}
+
+void launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s) {
+    if (n) fill_u8_kernel<<<grid_of(n, 256 * 16, 16384), 256, 0, s>>>(p, n, v);
+}
+
+void launch_dn_init(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K, uint32_t* wread,
+                    uint8_t* state, hipStream_t s) {
+    dn_init_kernel<<<grid_of(r1 - r0, 4, 65535), 256, 0, s>>>(off, kmer_off, r0, r1, K, wread, state);
+}
+
+void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev, uint64_t max_events,
+                           hipStream_t s) {
+    dn_event_reads_kernel<<<grid_of(max_events, 256 * 4, 16384), 256, 0, s>>>(P, wread, rid, hasev);
+}
+
+void launch_dn_replay(const BinPlan& P, const Tables& T, const uint32_t* rid, const uint8_t* state, uint32_t cutoff,
+                      uint8_t* fails, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((P.n_buckets + 3) / 4, 1), grid);
+    dn_replay_kernel<<<g, kBlock, 0, s>>>(P, T, rid, state, cutoff, fails);
+}
+
+void launch_dn_decide(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
+                      const uint8_t* hasev, const uint8_t* fails, int upper, uint8_t* state, uint32_t* ctl,
+                      hipStream_t s) {
+    dn_decide_kernel<<<grid_of(r1 - r0, 4, 65535), 256, 0, s>>>(off, kmer_off, r0, r1, K, hasev, fails, upper, state,
+                                                                  ctl);
+}
+
+void launch_dn_ctl_reset(uint32_t* ctl, hipStream_t s) { dn_ctl_reset_kernel<<<1, 64, 0, s>>>(ctl); }
+
+void launch_dn_passed(const uint8_t* state, uint32_t r0, uint32_t r1, uint8_t* passed, hipStream_t s) {
+    if (r1 > r0) dn_passed_kernel<<<grid_of(r1 - r0, 256, 16384), 256, 0, s>>>(state, r0, r1, passed);
+}
+
+void launch_dn_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int grid, hipStream_t s) {
+    const int g = (int)std::min<int64_t>(std::max<int64_t>((P.n_buckets + 3) / 4, 1), grid);
+    dn_apply_kernel<<<g, kBlock, 0, s>>>(P, T, counters);
 }
 
 void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {

@@ -69,6 +69,23 @@ void launch_solid_reads(const int64_t* off, const int64_t* kmer_off, uint64_t n_
                         uint32_t threshold, float min_prop_solid, uint8_t* passed, hipStream_t s);
 void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,
                            uint64_t* counters, hipStream_t s);
+void launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
+
+// Batched diginorm over one chunk of reads [r0, r1) whose events are routed
+// (init, event_reads once; replay, ctl_reset, decide per guess; apply of the
+// kept reads and passed at the end).
+void launch_dn_init(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K, uint32_t* wread,
+                    uint8_t* state, hipStream_t s);
+void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev, uint64_t max_events,
+                           hipStream_t s);
+void launch_dn_replay(const BinPlan& P, const Tables& T, const uint32_t* rid, const uint8_t* state, uint32_t cutoff,
+                      uint8_t* fails, int grid, hipStream_t s);
+void launch_dn_decide(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
+                      const uint8_t* hasev, const uint8_t* fails, int upper, uint8_t* state, uint32_t* ctl,
+                      hipStream_t s);
+void launch_dn_ctl_reset(uint32_t* ctl, hipStream_t s);
+void launch_dn_passed(const uint8_t* state, uint32_t r0, uint32_t r1, uint8_t* passed, hipStream_t s);
+void launch_dn_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int grid, hipStream_t s);
 uint64_t scan_u32_block_count(uint64_t n);
 void launch_scan_u32(uint32_t* a, uint64_t n, uint32_t* sums, hipStream_t s);
 

@@ -11,7 +11,7 @@ from ._native import (GoetiaException, GoetiaFileException, InvalidCharacterExce
                       SequenceLengthException, UninitializedShifterException)
 from .parsing import FastxParser, NoMoreReadsAvailable, Record
 from .processors import InserterProcessor, IntervalCounter
-from .filters import StreamingSolidFilter
+from .filters import DiginormFilter, StreamingSolidFilter
 
 __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Canonical', 'Hash',
            'DNA_SIMPLE', 'KmerIterator', 'BitStorage', 'ByteStorage', 'SparseppSetStorage',
@@ -19,4 +19,4 @@ __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Cano
            'GoetiaException', 'InvalidSequenceException', 'SequenceLengthException',
            'UninitializedShifterException', 'GoetiaFileException', 'InvalidCharacterException',
            'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter',
-           'StreamingSolidFilter']
+           'StreamingSolidFilter', 'DiginormFilter']

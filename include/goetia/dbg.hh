@@ -37,6 +37,7 @@ struct ReadBatch {
         offsets.push_back((int64_t)seq.size());
     }
     uint64_t size() const { return offsets.size() - 1; }
+    std::string read(uint64_t i) const { return seq.substr(offsets[i], offsets[i + 1] - offsets[i]); }
     uint64_t n_kmers(uint16_t K) const {
         uint64_t n = 0;
         for (size_t i = 0; i + 1 < offsets.size(); ++i) {

@@ -344,6 +344,24 @@ int gt_solid_filter(gt_storage* st, uint16_t K, int canonical, int mode, const u
                     const int64_t* offsets, uint64_t n_reads, float min_prop_solid, uint32_t solid_threshold,
                     uint8_t* passed);
 
+/* goetia's DiginormFilter<dBG<ByteStorage, Sh>>::Filter::filter_sequence
+ * (diginorm.hh:105-119) over a batch, in read order: a read is dropped
+ * (passed = 0) when median_count_at_least (diginorm.hh:39-76) holds -- at
+ * least min_req = (unsigned)(0.5 + float(len - K + 1) / 2) of its k-mers have
+ * a count >= cutoff -- and otherwise inserted (passed = 1) before the next read
+ * is looked at.  Results, tables and counters equal filtering the reads one
+ * after another.  Reads shorter than K do not pass and are not inserted
+ * (FilterProcessor::process_sequence skips them, processors.hh:389-401).
+ * Skip-N mode: windows holding a non-ACGT byte are neither counted nor
+ * inserted (min_req still counts them).  Count-min tables only, at most 4
+ * tables and 2^30 bins in all (GT_ERR_UNSUPPORTED otherwise).  The device
+ * variant synchronises `stream` (the offsets come to the host once, and each
+ * chunk of reads is resolved in a few passes). */
+int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                    uint64_t n_reads, uint64_t n_bases, uint32_t cutoff, uint8_t* passed, void* stream);
+int gt_diginorm(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq, const int64_t* offsets,
+                uint64_t n_reads, uint32_t cutoff, uint8_t* passed);
+
 /* ------------------------------------------------------------ FASTX input */
 
 /* goetia's FastxParser<DNA_SIMPLE> (readers.hh:95-232; construction

@@ -605,6 +605,60 @@ uint64_t or_insert_reads(or_storage* st, int K, int canonical, int mode,
     return total;
 }
 
+/* DiginormFilter<dBG<ByteStorage, Sh>>::Filter::filter_sequence
+ * (diginorm.hh:105-119) for one read: -1 when the read is shorter than K (the
+ * k-mer iterator raises SequenceLengthException, kmeriterator.hh:72-74), 0
+ * when median_count_at_least (diginorm.hh:39-76) holds, else 1 after the read
+ * is inserted (dbg.hh:249-265).  median_count_at_least queries the first
+ * min_req k-mers, min_req = (unsigned)(0.5 + float(n_kmers) / 2)
+ * (diginorm.hh:43), counting those with count >= cutoff; it holds if all of
+ * them did, else it goes on through the remaining k-mers until the count
+ * reaches min_req (diginorm.hh:47-73).  Under skip-N, windows holding a
+ * non-ACGT byte are never counted. */
+int or_diginorm_filter(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
+                       const char* s, size_t len) {
+    if (len < (size_t)K) return -1;
+    or_windows w;
+    windows_of(s, len, K, mode, &w);
+    const unsigned min_req = (unsigned)(0.5 + (double)((float)(len - (size_t)K + 1) / 2));
+    unsigned num = 0;
+    int at_least = 0;
+    size_t i = 0;
+    for (; i < min_req; ++i)
+        if (w.valid[i] && (unsigned)or_query(st, wval(&w, i, canonical)) >= cutoff) ++num;
+    if (num >= min_req) {
+        at_least = 1;
+    } else {
+        for (; i < w.n; ++i) {
+            if (w.valid[i] && (unsigned)or_query(st, wval(&w, i, canonical)) >= cutoff) {
+                ++num;
+                if (num >= min_req) {
+                    at_least = 1;
+                    break;
+                }
+            }
+        }
+    }
+    windows_free(&w);
+    if (at_least) return 0;
+    or_insert_sequence(st, K, canonical, mode, s, len, NULL, NULL);
+    return 1;
+}
+
+/* The filter over reads in order: passed[r] = 1 for inserted reads, 0 for
+ * dropped and short ones.  Returns the reads that passed. */
+uint64_t or_diginorm_reads(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
+                           const char* seq, const int64_t* offsets, uint64_t n_reads, uint8_t* passed) {
+    uint64_t n = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        int p = or_diginorm_filter(st, K, canonical, mode, cutoff, seq + offsets[r],
+                                   (size_t)(offsets[r + 1] - offsets[r]));
+        passed[r] = (uint8_t)(p == 1);
+        n += p == 1;
+    }
+    return n;
+}
+
 /* Counts are packed per read: read r owns max(0, len_r - K + 1) entries. */
 uint64_t or_query_reads(const or_storage* st, int K, int canonical, int mode,
                         const char* seq, const int64_t* offsets, uint64_t n_reads,

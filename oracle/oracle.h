@@ -22,6 +22,10 @@
  *   or_degrees                  : neighbour hashes vs direct hashing of the
  *       neighbour strings, linear-path and fork degrees, golden decision sets
  *       from oracle/_ref.
+ *   or_diginorm_*               : golden pass flags, tables and counters written
+ *       by oracle/_ref's DiginormFilter (tests/golden/diginorm_ref.npz); identical
+ *       reads pass exactly until cutoff copies are in; the early return of
+ *       median_count_at_least equals the plain count of k-mers at the cutoff.
  */
 #ifndef GOETIA_ORACLE_H
 #define GOETIA_ORACLE_H
@@ -102,6 +106,12 @@ uint64_t or_query_reads(const or_storage* st, int K, int canonical, int mode,
                         int16_t* counts);
 
 uint64_t or_exact_keys(const or_storage* st, uint64_t* out, uint64_t cap);
+
+/* DiginormFilter (count-min graphs): one read, then reads in order. */
+int      or_diginorm_filter(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
+                            const char* s, size_t len);
+uint64_t or_diginorm_reads(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
+                           const char* seq, const int64_t* offsets, uint64_t n_reads, uint8_t* passed);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,10 @@ def lib():
             'or_query_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
                                             C.c_uint64, i16p]),
             'or_exact_keys': (C.c_uint64, [sp, u64p, C.c_uint64]),
+            'or_diginorm_filter': (C.c_int, [sp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_char_p,
+                                             C.c_size_t]),
+            'or_diginorm_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_char_p,
+                                               i64p, C.c_uint64, u8p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -230,6 +234,21 @@ class Storage:
                                       seq.ctypes.data_as(C.c_char_p), _p(offsets, C.c_int64),
                                       n_reads, _p(nn, C.c_uint64))
         return total, nn[:n_reads]
+
+    def diginorm_filter(self, seq, K, cutoff, canonical=True, mode=MODE_RAW):
+        """DiginormFilter::Filter::filter_sequence: 1 inserted, 0 dropped,
+        -1 shorter than K."""
+        s = seq.encode() if isinstance(seq, str) else bytes(seq)
+        return lib().or_diginorm_filter(self.ref, K, int(canonical), mode, cutoff, s, len(s))
+
+    def diginorm_reads(self, seq, offsets, K, cutoff, canonical=True, mode=MODE_RAW):
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        n_reads = len(offsets) - 1
+        passed = np.zeros(max(n_reads, 1), np.uint8)
+        lib().or_diginorm_reads(self.ref, K, int(canonical), mode, cutoff, seq.ctypes.data_as(C.c_char_p),
+                                _p(offsets, C.c_int64), n_reads, _p(passed, C.c_uint8))
+        return passed[:n_reads].astype(bool)
 
     def query_reads(self, seq, offsets, K, canonical=True, mode=MODE_RAW):
         seq = np.ascontiguousarray(seq, np.uint8)

@@ -12,6 +12,7 @@
 #include "goetia/hashing/hashshifter.hh"
 #include "goetia/hashing/kmeriterator.hh"
 #include "goetia/parsing/readers.hh"
+#include "goetia/diginorm.hh"
 
 #include <cstdint>
 #include <cstring>
@@ -80,6 +81,27 @@ void* make_graph(std::shared_ptr<S> store, int K) {
     auto* holder = new Holder<Graph<S, H>>();
     holder->g = Graph<S, H>::build(store, (uint16_t)K);
     return holder;
+}
+
+// DiginormFilter<G>::Filter::filter_sequence over reads in order; a read
+// shorter than K raises SequenceLengthException and does not pass, as in
+// FilterProcessor::process_sequence (processors.hh:389-401).
+template <class G>
+uint64_t diginorm_run(void* graph, unsigned cutoff, const char* seq, const int64_t* off, uint64_t n_reads,
+                      uint8_t* passed) {
+    auto f = DiginormFilter<G>::Filter::build(static_cast<Holder<G>*>(graph)->g, cutoff);
+    uint64_t n = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        std::string s(seq + off[r], (size_t)(off[r + 1] - off[r]));
+        passed[r] = 0;
+        try {
+            bool p = std::get<0>(f->filter_sequence(s));
+            passed[r] = p ? 1 : 0;
+            n += p;
+        } catch (SequenceLengthException&) {
+        }
+    }
+    return n;
 }
 
 }  // namespace
@@ -283,6 +305,17 @@ uint64_t ref_fastx_parse(const char* path, int strict, uint32_t min_length, char
     counts[0] = parser->n_parsed();
     counts[1] = parser->n_skipped();
     return n;
+}
+
+// DiginormFilter over reads in order (count-min graphs only; returns the
+// reads that passed, or UINT64_MAX for another storage).
+uint64_t ref_diginorm_reads(void* hv, unsigned cutoff, const char* seq, const int64_t* off, uint64_t n_reads,
+                            uint8_t* passed) {
+    Handle* h = (Handle*)hv;
+    if (h->kind != 2) return UINT64_MAX;
+    if (h->canonical) return diginorm_run<Graph<ByteStorage, CanLemireShifter>>(h->graph, cutoff, seq, off, n_reads,
+                                                                             passed);
+    return diginorm_run<Graph<ByteStorage, FwdLemireShifter>>(h->graph, cutoff, seq, off, n_reads, passed);
 }
 
 void ref_free(void* hv) {

@@ -11,6 +11,7 @@
 #include "goetia/parsing/readers.hh"
 #include "goetia/processors.hh"
 #include "goetia/solidifier.hh"
+#include "goetia/diginorm.hh"
 
 using namespace goetia;
 
@@ -139,6 +140,20 @@ int main(int argc, char** argv) {
         for (auto v : first) n1 += v;
         for (auto v : second) n2 += v;
         CHECK(n2 == b4.size() && n1 < n2);
+        // diginorm with cutoff 1: every read of the batch passes the first
+        // time (its k-mers are new), and a second pass over the same reads
+        // drops every read (all its k-mers now have count >= 1)
+        auto g7 = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(1000000, 4), 21);
+        auto dn = DiginormFilter<dBG<ByteStorage, CanLemireShifter>>::Filter::build(g7, 1);
+        auto p5 = FastxParser<>::build(argv[1]);
+        auto b5 = p5->next_batch();
+        uint64_t d1 = 0, d2 = 0;
+        for (auto v : dn->filter_sequences(b5)) d1 += v;
+        for (auto v : dn->filter_sequences(b5)) d2 += v;
+        CHECK(d1 > 0 && d2 == 0);
+        using DF = DiginormFilter<dBG<ByteStorage, CanLemireShifter>>;
+        CHECK(DF::median_count_at_least(b5.read(0), 1, *g7));
+        CHECK(!std::get<0>(dn->filter_sequence(b5.read(0))));
     }
     std::printf("FACADE OK\n");
     return 0;
