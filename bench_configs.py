@@ -9,6 +9,9 @@ neighbors_5m  repeats + SNP bubbles, 5M x 150 bp, K=31, ByteStorage(2^28, 4):
               insert, then degrees (count >= 2) of every new k-mer and the
               decision nodes.
 serving_64    64 sample streams, see goetia_amd/serving.py.
+diginorm      the count-min workload's 20M x 150 bp reads (50 Mbp genome, about
+              60x) through DiginormFilter, cutoff 20, K=31, ByteStorage(2^28, 4):
+              reads/s with the reference's per-read decisions.
 """
 import json
 import os
@@ -296,6 +299,63 @@ def fastx(args):
     print(json.dumps(out), flush=True)
 
 
+def diginorm(args):
+    """DiginormFilter over the count-min workload's reads (raw hashing, as
+    the reference's k-mer iterator), cutoff 20: one call over all reads into
+    fresh tables; the oracle (sequential C, 1 thread) on the first reads."""
+    import ctypes as C
+
+    import torch
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    cfg, batch = workload.make('cms_20m', args.scale)
+    K, cutoff = cfg['K'], 20
+    d = batch.to_device()
+    nk = int(batch.kmer_counts(K).sum())
+    passed = torch.zeros(batch.n_reads, dtype=torch.uint8, device='cuda')
+    store = lg.ByteStorage.build(cfg['max_table'], cfg['n_tables'])
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(store, K)
+    f = lg.DiginormFilter.build(g, cutoff)
+    stream = torch.cuda.current_stream()
+    # warm-up on a small prefix (allocations), then a fresh run over all reads
+    small = workload.make('cms_20m', min(args.scale, 0.005))[1].to_device()
+    f.filter_device(small, torch.zeros(small.n_reads, dtype=torch.uint8, device='cuda'), stream=stream)
+    store.reset()
+    names = ['bin_hash', 'bin_count', 'bin_route', 'dn_replay', 'dn_decide', 'dn_apply']
+    N.call('gt_profile_enable', 1)
+    N.call('gt_profile_reset')
+    _sync()
+    t0 = time.perf_counter()
+    f.filter_device(d, passed, stream=stream)
+    _sync()
+    dt = time.perf_counter() - t0
+    N.call('gt_profile_enable', 0)
+    phases = {}
+    for n in names:
+        cnt, ms = C.c_uint64(), C.c_double()
+        N.call('gt_profile_get', n.encode(), C.byref(cnt), C.byref(ms))
+        phases[n] = {'calls': cnt.value, 'ms': round(ms.value, 2)}
+    n_passed = int(passed.sum().item())
+    out = {'metric': 'reads/s through DiginormFilter (1 GPU)', 'workload': 'diginorm',
+           'value': round(batch.n_reads / dt, 1), 'unit': 'reads/s', 'kmers_per_s': round(nk / dt, 1),
+           'seconds': round(dt, 3), 'reads': batch.n_reads, 'kmers': nk, 'cutoff': cutoff,
+           'passed': n_passed, 'n_unique': g.n_unique(), 'phases': phases,
+           'chunks': phases['dn_apply']['calls'], 'guesses': phases['dn_replay']['calls']}
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        n = min(args.cpu_sample_reads, batch.n_reads)
+        st = po.Storage(po.BYTE, po.primes_near_x(cfg['n_tables'], cfg['max_table']))
+        t0 = time.perf_counter()
+        exp = st.diginorm_reads(batch.seq[:batch.offsets[n]], batch.offsets[:n + 1], K, cutoff)
+        dtc = time.perf_counter() - t0
+        out['cpu_baseline'] = {'value': round(n / dtc, 1), 'unit': 'reads/s', 'cores': 1, 'kind': 'port',
+                               'sample': f'first {n} reads (below the cutoff: nearly every read is queried '
+                                         f'and inserted), oracle/oracle.c single-threaded, {dtc:.1f} s'}
+        out['cpu_sample_same_decisions'] = bool(np.array_equal(exp, passed[:n].cpu().numpy().astype(bool)))
+    print(json.dumps(out), flush=True)
+
+
 def run(args):
     return {'bloom_100k': bloom, 'exact_mixed': exact, 'neighbors_5m': neighbors,
-            'serving_64': serving, 'fastx': fastx}[args.workload](args)
+            'serving_64': serving, 'fastx': fastx, 'diginorm': diginorm}[args.workload](args)

@@ -1500,9 +1500,10 @@ int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const u
         uint64_t r_end = r1;
         const uint64_t n_win = (uint64_t)(kof[r1] - kof[r0]);
         if (n_win) {
+            P.hasev = hasev;
             bin_route(R, T, P, sc, off[r0], off[r1], s);
-            launch_fill_u8(hasev, n_win, 0, s);
-            launch_dn_event_reads(P, wread, rid, hasev, n_win * st->n_tables, s);
+            P.hasev = nullptr;
+            launch_dn_event_reads(P, wread, rid, n_win * st->n_tables, s);
             bool settled = false;
             uint32_t h_ctl[2] = {0, 0};
             for (int g = 1; g <= cap && !settled; ++g) {

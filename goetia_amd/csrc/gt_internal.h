@@ -115,6 +115,8 @@ struct BinPlan {
     const uint32_t* rid;
     const uint8_t* keep;
     uint32_t r_end;
+    uint8_t* hasev;                       // optional, per window of the sub-batch: the first pass
+                                          // writes 1 for windows with events, 0 for skipped ones
 };
 
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the

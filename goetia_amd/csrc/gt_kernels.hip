@@ -1176,10 +1176,13 @@ struct BinL1Op {
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
-        if (!valid || s < b_lo) return;
+        if (s < b_lo) return;
+        const uint32_t ord = (uint32_t)((uint64_t)(kmer_off[r] + (s - rstart)) - kbase);
+        if (P.hasev) P.hasev[ord] = valid ? 1 : 0;
+        if (!valid) return;
         const int w = (int)(s - t0);
         const uint64_t h = kmer_value(fw, rc, canonical);
-        sh->ord[w] = (uint32_t)((uint64_t)(kmer_off[r] + (s - rstart)) - kbase);
+        sh->ord[w] = ord;
 #pragma unroll
         for (uint32_t i = 0; i < kBinMaxTables; ++i)
             if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + (uint32_t)gt_fastmod(h, T.p[i], T.m[i]);
@@ -1742,16 +1745,11 @@ __global__ void dn_init_kernel(const int64_t* off, const int64_t* kmer_off, uint
     }
 }
 
-// Read of every event of the chunk (rid, aligned with ent2), and which
-// windows have events (in skip-N mode windows holding a non-ACGT byte have
-// none).
-__global__ void dn_event_reads_kernel(BinPlan P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev) {
+// Read of every event of the chunk (rid, aligned with ent2).
+__global__ void dn_event_reads_kernel(BinPlan P, const uint32_t* wread, uint32_t* rid) {
     const uint32_t n = P.cnt2[(size_t)P.n_buckets * P.n_groups];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t ord = (uint32_t)P.ent2[i];
-        rid[i] = wread[ord];
-        hasev[ord] = 1;
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        rid[i] = wread[(uint32_t)P.ent2[i]];
 }
 
 // The events [pos, ...) of one read rr that runs past a 64-event step: each
@@ -1814,13 +1812,24 @@ __global__ __launch_bounds__(kBlock) void dn_replay_kernel(BinPlan P, Tables T, 
         uint32_t t, nw;
         bin_stage_slice(P, T, b, sw, &t, &nw);
         __builtin_amdgcn_wave_barrier();
+        // the events of the next two steps are loaded ahead: [pos, pos + 64)
+        // in (e, r), [pos + 64, pos + 128) in (eb, rb)
+        auto load = [&](uint32_t at, uint64_t& ex, uint32_t& rx) {
+            const uint32_t i = at + (uint32_t)lane;
+            ex = i < hi_e ? P.ent2[i] : 0;
+            rx = i < hi_e ? rid[i] : 0xFFFFFFFFu;
+        };
         uint32_t pos = lo_e;
+        uint64_t e, eb;
+        uint32_t r, rb;
+        load(pos, e, r);
+        load(pos + 64u, eb, rb);
         while (pos < hi_e) {
-            const uint32_t i = pos + (uint32_t)lane;
-            const bool v = i < hi_e;
-            const uint64_t e = v ? P.ent2[i] : 0;
-            const uint32_t r = v ? rid[i] : 0xFFFFFFFFu;
-            const uint32_t rn = pos + 64u < hi_e ? rid[pos + 64u] : 0xFFFFFFFFu;
+            uint64_t ec;
+            uint32_t rc;
+            load(pos + 128u, ec, rc);
+            const bool v = pos + (uint32_t)lane < hi_e;
+            const uint32_t rn = __shfl(rb, 0);                              // read of event pos + 64
             const uint32_t rp = __shfl_up(r, 1);
             const uint64_t bmask = __ballot(v && (lane == 0 || r != rp));   // first event of each read
             uint32_t np = hi_e - pos < 64u ? hi_e - pos : 64u;
@@ -1830,6 +1839,8 @@ __global__ __launch_bounds__(kBlock) void dn_replay_kernel(BinPlan P, Tables T, 
                 if (s63 == 0) {
                     pos = dn_long_read(P, rid, state[__shfl(r, 0)], cutoff, fails, sb, slots[wave], pos, hi_e,
                                        __shfl(r, 0));
+                    load(pos, e, r);
+                    load(pos + 64u, eb, rb);
                     continue;
                 }
                 np = s63;
@@ -1852,6 +1863,15 @@ __global__ __launch_bounds__(kBlock) void dn_replay_kernel(BinPlan P, Tables T, 
             }
             __builtin_amdgcn_wave_barrier();
             pos += np;
+            if (np == 64u) {
+                e = eb;
+                r = rb;
+                eb = ec;
+                rb = rc;
+            } else {
+                load(pos, e, r);
+                load(pos + 64u, eb, rb);
+            }
         }
     }
 }
@@ -1861,34 +1881,54 @@ __global__ __launch_bounds__(kBlock) void dn_replay_kernel(BinPlan P, Tables T, 
 // computed as the reference does (diginorm.hh:43).  With `upper` set the new
 // guess is an upper bound of the kept set (the one replayed was a lower
 // bound), so its drops are final; otherwise its keeps are.  ctl[0] counts
-// reads whose guess changed, ctl[1] is the first read not yet final.  One
-// wavefront per read.
-__global__ void dn_decide_kernel(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
-                                 const uint8_t* hasev, const uint8_t* fails, int upper, uint8_t* state,
-                                 uint32_t* ctl) {
+// reads whose guess changed, ctl[1] is the first read not yet final.  A
+// thread per read; a workgroup's kBlock reads have one contiguous window
+// range, staged in LDS when it fits.
+__global__ __launch_bounds__(kBlock) void dn_decide_kernel(const int64_t* off, const int64_t* kmer_off, uint32_t r0,
+                                                           uint32_t r1, int K, const uint8_t* hasev,
+                                                           const uint8_t* fails, int upper, uint8_t* state,
+                                                           uint32_t* ctl) {
+    __shared__ uint8_t ok[kReadNewStage];
     const int64_t kb = kmer_off[r0];
-    const int lane = threadIdx.x & 63;
-    const uint64_t wpb = blockDim.x / 64;
-    uint32_t changed = 0;
-    for (uint64_t r = r0 + (uint64_t)blockIdx.x * wpb + threadIdx.x / 64; r < r1; r += (uint64_t)gridDim.x * wpb) {
+    uint32_t changed = 0, first_open = 0xFFFFFFFFu;
+    for (uint64_t q0 = r0 + (uint64_t)blockIdx.x * kBlock; q0 < r1; q0 += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t q1 = q0 + kBlock < r1 ? q0 + kBlock : r1;
+        const int64_t w0 = kmer_off[q0] - kb, w1 = kmer_off[q1] - kb;
+        const bool staged = w1 - w0 <= kReadNewStage;
+        __syncthreads();
+        if (staged)
+            for (int64_t i = threadIdx.x; i < w1 - w0; i += kBlock) ok[i] = hasev[w0 + i] && !fails[w0 + i];
+        __syncthreads();
+        const uint64_t r = q0 + threadIdx.x;
+        if (r >= q1) continue;
         const uint8_t st = state[r];
         if (st & kDnSettled) continue;
         const int64_t a = kmer_off[r] - kb, b = kmer_off[r + 1] - kb;
         uint32_t c = 0;
-        for (int64_t w = a + lane; w < b; w += 64) c += (hasev[w] && !fails[w]) ? 1u : 0u;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
+        if (staged) {
+            for (int64_t w = a; w < b; ++w) c += ok[w - w0];
+        } else {
+            for (int64_t w = a; w < b; ++w) c += (hasev[w] && !fails[w]) ? 1u : 0u;
+        }
         const uint64_t n_kmers = (uint64_t)(off[r + 1] - off[r]) - (uint64_t)K + 1u;
         const uint32_t min_req = (uint32_t)(0.5 + (double)((float)n_kmers / 2));
         const bool keep = c < min_req;
-        if (lane == 0) {
-            const bool settled = upper ? !keep : keep;
-            changed += keep != ((st & kDnKept) != 0);
-            state[r] = (uint8_t)((keep ? kDnKept : 0) | (settled ? kDnSettled : 0));
-            if (!settled) atomicMin(&ctl[1], (uint32_t)r);
-        }
+        const bool settled = upper ? !keep : keep;
+        changed += keep != ((st & kDnKept) != 0);
+        state[r] = (uint8_t)((keep ? kDnKept : 0) | (settled ? kDnSettled : 0));
+        if (!settled && (uint32_t)r < first_open) first_open = (uint32_t)r;
     }
-    if (lane == 0 && changed) atomicAdd(&ctl[0], changed);
+    // one atomic per wavefront
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        changed += __shfl_xor(changed, d);
+        const uint32_t o = __shfl_xor(first_open, d);
+        first_open = o < first_open ? o : first_open;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (changed) atomicAdd(&ctl[0], changed);
+        if (first_open != 0xFFFFFFFFu) atomicMin(&ctl[1], first_open);
+    }
 }
 
 __global__ void dn_ctl_reset_kernel(uint32_t* ctl) {
@@ -2173,9 +2213,9 @@ void launch_dn_init(const int64_t* off, const int64_t* kmer_off, uint32_t r0, ui
     dn_init_kernel<<<grid_of(r1 - r0, 4, 65535), 256, 0, s>>>(off, kmer_off, r0, r1, K, wread, state);
 }
 
-void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev, uint64_t max_events,
+void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint64_t max_events,
                            hipStream_t s) {
-    dn_event_reads_kernel<<<grid_of(max_events, 256 * 4, 16384), 256, 0, s>>>(P, wread, rid, hasev);
+    dn_event_reads_kernel<<<grid_of(max_events, 256 * 4, 16384), 256, 0, s>>>(P, wread, rid);
 }
 
 void launch_dn_replay(const BinPlan& P, const Tables& T, const uint32_t* rid, const uint8_t* state, uint32_t cutoff,
@@ -2187,8 +2227,8 @@ void launch_dn_replay(const BinPlan& P, const Tables& T, const uint32_t* rid, co
 void launch_dn_decide(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K,
                       const uint8_t* hasev, const uint8_t* fails, int upper, uint8_t* state, uint32_t* ctl,
                       hipStream_t s) {
-    dn_decide_kernel<<<grid_of(r1 - r0, 4, 65535), 256, 0, s>>>(off, kmer_off, r0, r1, K, hasev, fails, upper, state,
-                                                                  ctl);
+    dn_decide_kernel<<<grid_of(r1 - r0, kBlock, 16384), kBlock, 0, s>>>(off, kmer_off, r0, r1, K, hasev, fails,
+                                                                          upper, state, ctl);
 }
 
 void launch_dn_ctl_reset(uint32_t* ctl, hipStream_t s) { dn_ctl_reset_kernel<<<1, 64, 0, s>>>(ctl); }

@@ -76,7 +76,7 @@ void launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
 // kept reads and passed at the end).
 void launch_dn_init(const int64_t* off, const int64_t* kmer_off, uint32_t r0, uint32_t r1, int K, uint32_t* wread,
                     uint8_t* state, hipStream_t s);
-void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint8_t* hasev, uint64_t max_events,
+void launch_dn_event_reads(const BinPlan& P, const uint32_t* wread, uint32_t* rid, uint64_t max_events,
                            hipStream_t s);
 void launch_dn_replay(const BinPlan& P, const Tables& T, const uint32_t* rid, const uint8_t* state, uint32_t cutoff,
                       uint8_t* fails, int grid, hipStream_t s);

@@ -73,6 +73,24 @@ def test_vs_oracle_high_coverage(oracle, cutoff):
     assert 0.05 < exp.mean() < 0.95
 
 
+def test_skip_n_mode_vs_oracle(oracle):
+    """Skip-N input policy: windows holding a non-ACGT byte are neither
+    counted nor inserted (min_req still counts them)."""
+    genome = workload.genome(97, 30_000)
+    batch = workload.reads(98, genome, 8_000, 30, 150, 0.01, 0.02)
+    K = 21
+    sizes = oracle.primes_near_x(4, 60_000)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp = o.diginorm_reads(batch.seq, batch.offsets, K, 3, True, oracle.MODE_SKIP_N)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter](lg.ByteStorage(sizes), K, mode=N.MODE_SKIP_N)
+    got = lg.DiginormFilter.build(g, 3).filter_sequences(batch)
+    np.testing.assert_array_equal(got, exp)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    assert 0.05 < exp.mean() < 0.95
+
+
 def test_device_reads_and_one_at_a_time(oracle):
     import torch
     batch = _coverage_reads(90, 5_000, 20_000)
