@@ -153,6 +153,7 @@ SIGNATURES = {
     'gt_diginorm': [st_p, u16, cint, cint, vp, vp, u64, u32, vp],
     'gt_fastx_open': [C.c_char_p, cint, u32, C.POINTER(vp)],
     'gt_fastx_next_batch': [vp, u64, u64, vp, vp, vp, u64, vp, u64p],
+    'gt_fastx_next_batch_qual': [vp, u64, u64, vp, vp, vp, vp, vp, u64, vp, u64p],
     'gt_fastx_stats': [vp, u64p, u64p, u64p, C.POINTER(cint)],
     'gt_fastx_close': [vp],
     'gt_inserter_new': [st_p, C.c_uint16, cint, cint, u64, u64, C.POINTER(vp)],

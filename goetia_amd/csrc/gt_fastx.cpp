@@ -141,8 +141,8 @@ namespace {
 
 // Valid records of one parsed block, in file order.
 struct Block {
-    std::string seq, names;
-    std::vector<int64_t> off{0}, noff{0};
+    std::string seq, names, qual;
+    std::vector<int64_t> off{0}, noff{0}, qoff{0};     // qualities: empty for FASTA records
     std::vector<uint64_t> cum_parsed, cum_skipped;   // counts up to and including each record
     uint64_t tot_parsed = 0, tot_skipped = 0;
     size_t next = 0;
@@ -151,8 +151,10 @@ struct Block {
     void clear() {
         seq.clear();
         names.clear();
+        qual.clear();
         off.assign(1, 0);
         noff.assign(1, 0);
+        qoff.assign(1, 0);
         cum_parsed.clear();
         cum_skipped.clear();
         tot_parsed = tot_skipped = 0;
@@ -346,17 +348,19 @@ inline bool append_valid(std::string& out, const char* b, size_t i, size_t e) {
 
 // FastxParser::next's bookkeeping for one record: keep it, or count it as
 // skipped (twice when it is both invalid and short, as goetia counts).
-inline void finish_record(Block& out, size_t seq_start, bool valid, uint32_t min_length, const char* name,
-                          size_t name_len) {
+inline void finish_record(Block& out, size_t seq_start, size_t qual_start, bool valid, uint32_t min_length,
+                          const char* name, size_t name_len) {
     const size_t len = out.seq.size() - seq_start;
     out.tot_parsed++;
     if (!valid) out.tot_skipped++;
     if (len < min_length) out.tot_skipped++;
     if (!valid || len < min_length) {
         out.seq.resize(seq_start);
+        out.qual.resize(qual_start);
         return;
     }
     out.off.push_back((int64_t)out.seq.size());
+    out.qoff.push_back((int64_t)out.qual.size());
     out.names.append(name, name_len);
     out.noff.push_back((int64_t)out.names.size());
     out.cum_parsed.push_back(out.tot_parsed);
@@ -379,7 +383,7 @@ bool parse_range(const char* b, size_t a, size_t end, size_t n, bool eof, char f
         if (has_cr(b, pos, he)) return false;
         const char* name = b + pos + 1;
         const size_t name_len = name_end(b, pos + 1, he) - (pos + 1);
-        const size_t start = out.seq.size();
+        const size_t start = out.seq.size(), qstart = out.qual.size();
         bool valid = true;
         if (fmt == '>') {
             pos = he + 1;
@@ -398,9 +402,10 @@ bool parse_range(const char* b, size_t a, size_t end, size_t n, bool eof, char f
             const size_t e3 = line_end(b, e2 + 1, n);
             if (has_cr(b, s1, e3)) return false;
             valid = append_valid(out.seq, b, s1, e1);
+            out.qual.append(b + e2 + 1, e3 - (e2 + 1));
             pos = e3 + 1;
         }
-        finish_record(out, start, valid, min_length, name, name_len);
+        finish_record(out, start, qstart, valid, min_length, name, name_len);
     }
     return pos == end || (eof && end == n && pos >= n);
 }
@@ -555,7 +560,15 @@ int gt_fastx_open(const char* path, int strict, uint32_t min_length, gt_fastx** 
 
 int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uint8_t* seq, int64_t* offsets,
                         char* names, uint64_t names_cap, int64_t* name_offsets, uint64_t* n_reads) {
+    return gt_fastx_next_batch_qual(p, max_reads, max_bases, seq, offsets, nullptr, nullptr, names, names_cap,
+                                    name_offsets, n_reads);
+}
+
+int gt_fastx_next_batch_qual(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uint8_t* seq, int64_t* offsets,
+                             uint8_t* qual, uint8_t* has_qual, char* names, uint64_t names_cap,
+                             int64_t* name_offsets, uint64_t* n_reads) {
     if (!p || !n_reads || !offsets || (!seq && max_bases)) return set_error(GT_ERR_INVALID_ARG, "null argument");
+    if ((qual == nullptr) != (has_qual == nullptr)) return set_error(GT_ERR_INVALID_ARG, "qual needs has_qual");
     if (names && !name_offsets) return set_error(GT_ERR_INVALID_ARG, "names need name_offsets");
     uint64_t n = 0, bases = 0, nbytes = 0;
     offsets[0] = 0;
@@ -599,6 +612,11 @@ int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uin
                 break;
             }
             std::memcpy(seq + bases, B.seq.data() + B.off[i], len);
+            if (qual) {
+                const bool q = B.qoff[i + 1] > B.qoff[i];
+                if (q) std::memcpy(qual + bases, B.qual.data() + B.qoff[i], len);
+                has_qual[n] = q ? 1 : 0;
+            }
             bases += len;
             offsets[n + 1] = (int64_t)bases;
             if (names) {
@@ -635,6 +653,11 @@ int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uin
             break;
         }
         std::memcpy(seq + bases, p->seq.data(), len);
+        if (qual) {
+            const bool q = !p->qual.empty();
+            if (q) std::memcpy(qual + bases, p->qual.data(), len);
+            has_qual[n] = q ? 1 : 0;
+        }
         bases += len;
         offsets[n + 1] = (int64_t)bases;
         if (names) {

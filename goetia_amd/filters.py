@@ -7,7 +7,18 @@ from . import _native as N
 from .dbg import _as_batch, _stream_ptr
 
 
-class StreamingSolidFilter:
+class _FilterFamily(type):
+    """Filter[graph_type] -> a namespace with .Filter and .Processor, as
+    goetia's cppyy classes (DiginormFilter<G>::Filter, ::Processor)."""
+
+    def __getitem__(cls, graph_type):
+        from .processors import FilterProcessor
+        name = getattr(graph_type, '__name__', str(graph_type))
+        return type(f'{cls.__name__}[{name}]', (), {'Filter': cls, 'Processor': FilterProcessor,
+                                                    'graph_type': graph_type})
+
+
+class StreamingSolidFilter(metaclass=_FilterFamily):
     """goetia::StreamingSolidFilter<dBG>::Filter: a read passes unless the
     share of its k-mers whose count (just after their own insert) is below
     solid_threshold reaches 1 - min_prop_solid."""
@@ -21,6 +32,9 @@ class StreamingSolidFilter:
     @classmethod
     def build(cls, graph, min_prop_solid=0.75, solid_threshold=1):
         return cls(graph, min_prop_solid, solid_threshold)
+
+    def __smartptr__(self):
+        return self
 
     def filter_sequence(self, sequence):
         """solidifier.hh:58-73: (passed, n_kmers); SequenceLengthException
@@ -47,7 +61,7 @@ class StreamingSolidFilter:
                self.solid_threshold, N.ptr(passed), _stream_ptr(stream))
 
 
-class DiginormFilter:
+class DiginormFilter(metaclass=_FilterFamily):
     """goetia::DiginormFilter<dBG<ByteStorage, Sh>>::Filter (diginorm.hh:93-119):
     a read is dropped when at least half of its k-mers (min_req =
     int(0.5 + float32(n_kmers) / 2)) already have a count of at least the
@@ -63,6 +77,9 @@ class DiginormFilter:
     @classmethod
     def build(cls, graph, cutoff):
         return cls(graph, cutoff)
+
+    def __smartptr__(self):
+        return self
 
     def filter_sequence(self, sequence):
         """diginorm.hh:105-119: (passed, n_kmers); SequenceLengthException

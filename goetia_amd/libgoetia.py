@@ -10,7 +10,7 @@ from .storage import (BitStorage, BTreeStorage, ByteStorage, PHMapStorage, Spars
 from ._native import (GoetiaException, GoetiaFileException, InvalidCharacterException, InvalidSequenceException,
                       SequenceLengthException, UninitializedShifterException)
 from .parsing import FastxParser, NoMoreReadsAvailable, Record
-from .processors import InserterProcessor, IntervalCounter
+from .processors import FilterProcessor, InserterProcessor, IntervalCounter
 from .filters import DiginormFilter, StreamingSolidFilter
 
 __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Canonical', 'Hash',
@@ -19,4 +19,5 @@ __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Cano
            'GoetiaException', 'InvalidSequenceException', 'SequenceLengthException',
            'UninitializedShifterException', 'GoetiaFileException', 'InvalidCharacterException',
            'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter',
+           'FilterProcessor',
            'StreamingSolidFilter', 'DiginormFilter']

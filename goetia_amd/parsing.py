@@ -18,7 +18,8 @@ class NoMoreReadsAvailable(N.GoetiaException):
 
 
 class Record:
-    """goetia::Record (readers.hh:60-68): name and validated sequence."""
+    """goetia::Record (parsing.hh:25-57): name, validated sequence and
+    quality (None for FASTA)."""
     __slots__ = ('name', 'sequence', 'quality')
 
     def __init__(self, name, sequence, quality=None):
@@ -26,8 +27,19 @@ class Record:
         self.sequence = sequence
         self.quality = quality
 
+    def write_fastx(self, out):
+        """parsing.hh:38-49: FASTQ when the record has a quality, else FASTA."""
+        out.write(fastx_text(self.name, self.sequence, self.quality))
+
     def __repr__(self):
         return f'Record(name={self.name!r}, sequence={self.sequence!r})'
+
+
+def fastx_text(name, sequence, quality=None):
+    """The text Record::write_fastx writes (parsing.hh:38-49)."""
+    if quality:
+        return f'@{name}\n{sequence}\n+\n{quality}\n'
+    return f'>{name}\n{sequence}\n'
 
 
 class FastxParser:
@@ -70,20 +82,26 @@ class FastxParser:
         self.close()
 
     # ------------------------------------------------------------ batches
-    def next_batch(self, max_reads=1 << 16, max_bases=64 << 20, names=False):
+    def next_batch(self, max_reads=1 << 16, max_bases=64 << 20, names=False, quality=False):
         """Up to max_reads records / max_bases bases as a ReadBatch (empty at
-        the end of the input), with their names when names is True."""
+        the end of the input); with names=True also their names, with
+        quality=True their names and qualities (bytes, or None for a record
+        without one)."""
         if self._error is not None:
             err, self._error = self._error, None
             raise err
+        names = names or quality
         seq = np.empty(max(max_bases, 1), np.uint8)
         off = np.zeros(max_reads + 1, np.int64)
         cap = max_reads * 256 if names else 0
         nb = C.create_string_buffer(cap) if names else None
         noff = np.zeros(max_reads + 1, np.int64) if names else None
+        qual = np.empty(max(max_bases, 1), np.uint8) if quality else None
+        hq = np.zeros(max(max_reads, 1), np.uint8) if quality else None
         n = C.c_uint64()
-        rc = N.load().gt_fastx_next_batch(self._h, max_reads, max_bases, N.ptr(seq), N.ptr(off), nb, cap,
-                                          N.ptr(noff), C.byref(n))
+        rc = N.load().gt_fastx_next_batch_qual(self._h, max_reads, max_bases, N.ptr(seq), N.ptr(off),
+                                               N.ptr(qual) if quality else None, N.ptr(hq) if quality else None,
+                                               nb, cap, N.ptr(noff), C.byref(n))
         k = n.value
         if rc != N.GT_OK:
             if k == 0:
@@ -95,7 +113,11 @@ class FastxParser:
         batch = ReadBatch(seq[:int(off[k])].copy(), off[:k + 1].copy())
         if names:
             raw = nb.raw
-            return batch, [raw[noff[i]:noff[i + 1]].decode() for i in range(k)]
+            nm = [raw[noff[i]:noff[i + 1]].decode() for i in range(k)]
+            if quality:
+                qs = [qual[off[i]:off[i + 1]].tobytes() if hq[i] else None for i in range(k)]
+                return batch, nm, qs
+            return batch, nm
         return batch
 
     # ------------------------------------------------------------ records
@@ -105,11 +127,12 @@ class FastxParser:
         if self._pos >= len(self._records):
             if self._done:
                 raise NoMoreReadsAvailable('No more reads available')
-            batch, nm = self.next_batch(self._batch_reads, self._batch_bases, names=True)
+            batch, nm, qs = self.next_batch(self._batch_reads, self._batch_bases, quality=True)
             if batch.n_reads == 0:
                 self._done = True
                 return None
-            self._records = [Record(nm[i], batch.read(i).decode()) for i in range(batch.n_reads)]
+            self._records = [Record(nm[i], batch.read(i).decode(), qs[i].decode() if qs[i] else None)
+                             for i in range(batch.n_reads)]
             self._pos = 0
         rec = self._records[self._pos]
         self._pos += 1

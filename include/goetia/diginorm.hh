@@ -19,6 +19,7 @@
 
 #include "goetia/dbg.hh"
 #include "goetia/goetia_amd.hh"
+#include "goetia/processors.hh"
 
 namespace goetia {
 
@@ -83,6 +84,8 @@ struct DiginormFilter {
             return {filter_sequences(b)[0] != 0, sequence.length() - K + 1};
         }
     };
+
+    using Processor = FilterProcessor<Filter>;
 };
 
 }  // namespace goetia

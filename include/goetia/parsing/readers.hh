@@ -11,6 +11,7 @@
 
 #include <cstdint>
 #include <deque>
+#include <ostream>
 #include <memory>
 #include <optional>
 #include <string>
@@ -27,11 +28,19 @@ class NoMoreReadsAvailable : public GoetiaException {
     NoMoreReadsAvailable() : GoetiaException("No more reads available") {}
 };
 
-// readers.hh:60-68
+// parsing.hh:25-57
 struct Record {
     std::string name;
     std::string sequence;
     std::string quality;
+
+    // parsing.hh:38-49: FASTQ when the record has a quality, else FASTA.
+    void write_fastx(std::ostream& output) const {
+        if (quality.length() != 0)
+            output << "@" << name << '\n' << sequence << '\n' << "+" << '\n' << quality << '\n';
+        else
+            output << ">" << name << '\n' << sequence << '\n';
+    }
 };
 
 template <class Alphabet = DNA_SIMPLE>
@@ -72,21 +81,41 @@ class FastxParser {
         return b;
     }
 
+    // The same with each record's name and quality (empty for FASTA).
+    ReadBatch next_batch(std::vector<std::string>& names, std::vector<std::string>& qualities,
+                         uint64_t max_reads = 1 << 16, uint64_t max_bases = 64ULL << 20) {
+        ReadBatch b;
+        b.seq.resize(max_bases);
+        b.offsets.assign(max_reads + 1, 0);
+        const uint64_t names_cap = max_reads * 256;
+        std::string nm(names_cap, '\0'), q(max_bases, '\0');
+        std::vector<int64_t> noff(max_reads + 1, 0);
+        std::vector<uint8_t> hq(max_reads, 0);
+        uint64_t n = 0;
+        const int rc = gt_fastx_next_batch_qual(_h, max_reads, max_bases, reinterpret_cast<uint8_t*>(&b.seq[0]),
+                                                b.offsets.data(), reinterpret_cast<uint8_t*>(&q[0]), hq.data(),
+                                                &nm[0], names_cap, noff.data(), &n);
+        b.offsets.resize(n + 1);
+        b.seq.resize((size_t)b.offsets[n]);
+        names.clear();
+        qualities.clear();
+        for (uint64_t i = 0; i < n; ++i) {
+            names.push_back(nm.substr(noff[i], noff[i + 1] - noff[i]));
+            qualities.push_back(hq[i] ? q.substr(b.offsets[i], b.offsets[i + 1] - b.offsets[i]) : std::string());
+        }
+        if (rc != GT_OK && n == 0) detail::check(rc, "FastxParser::next_batch");
+        return b;
+    }
+
     // FastxParser::next: the next record; empty once the input is exhausted,
     // NoMoreReadsAvailable after that.
     std::optional<Record> next() {
         if (_buffered.empty()) {
             if (_done) throw NoMoreReadsAvailable();
-            const uint64_t max_reads = 4096, max_bases = 1 << 20, names_cap = max_reads * 256;
-            std::string seq(max_bases, '\0'), names(names_cap, '\0');
-            std::vector<int64_t> off(max_reads + 1), noff(max_reads + 1);
-            uint64_t n = 0;
-            const int rc = gt_fastx_next_batch(_h, max_reads, max_bases, reinterpret_cast<uint8_t*>(&seq[0]),
-                                               off.data(), &names[0], names_cap, noff.data(), &n);
-            for (uint64_t i = 0; i < n; ++i)
-                _buffered.push_back(Record{names.substr(noff[i], noff[i + 1] - noff[i]),
-                                           seq.substr(off[i], off[i + 1] - off[i]), std::string()});
-            if (rc != GT_OK && n == 0) detail::check(rc, "FastxParser::next");
+            std::vector<std::string> names, quals;
+            ReadBatch b = next_batch(names, quals, 4096, 1 << 20);
+            for (uint64_t i = 0; i < b.size(); ++i)
+                _buffered.push_back(Record{names[i], b.read(i), quals[i]});
             if (_buffered.empty()) {
                 _done = true;
                 return {};

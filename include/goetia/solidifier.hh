@@ -15,6 +15,7 @@
 
 #include "goetia/dbg.hh"
 #include "goetia/goetia_amd.hh"
+#include "goetia/processors.hh"
 
 namespace goetia {
 
@@ -56,6 +57,8 @@ struct StreamingSolidFilter {
             return {filter_sequences(b)[0] != 0, sequence.length() - K + 1};
         }
     };
+
+    using Processor = FilterProcessor<Filter>;
 };
 
 }  // namespace goetia

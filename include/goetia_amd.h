@@ -384,6 +384,14 @@ int gt_fastx_open(const char* path, int strict, uint32_t min_length, gt_fastx** 
  * readers.hh:148-219, one batch at a time). */
 int gt_fastx_next_batch(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uint8_t* seq, int64_t* offsets,
                         char* names, uint64_t names_cap, int64_t* name_offsets, uint64_t* n_reads);
+/* gt_fastx_next_batch with each record's quality (Record::quality,
+ * readers.hh:186-191): qual has the layout of seq (a FASTQ record's quality
+ * has its sequence's length), has_qual[i] = 1 when record i has one (FASTQ
+ * with a non-empty sequence), 0 otherwise (FASTA).  qual and has_qual are
+ * both given or both null. */
+int gt_fastx_next_batch_qual(gt_fastx* p, uint64_t max_reads, uint64_t max_bases, uint8_t* seq, int64_t* offsets,
+                             uint8_t* qual, uint8_t* has_qual, char* names, uint64_t names_cap,
+                             int64_t* name_offsets, uint64_t* n_reads);
 /* n_parsed / n_skipped (readers.hh:221-227), invalid reads, is_complete. */
 int gt_fastx_stats(const gt_fastx* p, uint64_t* n_parsed, uint64_t* n_skipped, uint64_t* n_invalid_reads,
                    int* complete);

@@ -61,13 +61,15 @@ class _Stream:
         return len(s), dret, s
 
 
-def parse(path, strict=False, min_length=0):
+def parse(path, strict=False, min_length=0, with_quality=False):
     data = open(path, 'rb').read()
     if data[:2] == b'\x1f\x8b':
         data = gzip.decompress(data)
     ks = _Stream(data)
     last_char = 0
     out = {'sequences': [], 'names': [], 'n_parsed': 0, 'n_skipped': 0, 'invalid_reads': 0, 'strict_stopped': 0}
+    if with_quality:
+        out['qualities'] = []     # Record::quality (readers.hh:186-191); None when empty
     while True:
         # kseq_read
         stat = None
@@ -95,6 +97,7 @@ def parse(path, strict=False, min_length=0):
                 c = ks.getc()
             if c in (ord('>'), ord('@')):
                 last_char = c
+            qual = bytearray()
             if c != ord('+'):
                 stat = len(seq)
             else:
@@ -126,6 +129,8 @@ def parse(path, strict=False, min_length=0):
             if stat >= 0:
                 out['sequences'].append(''.join(VALID[b] for b in seq))
                 out['names'].append(bytes(name).decode())
+                if with_quality:
+                    out['qualities'].append(bytes(qual).decode() if qual else None)
             out['n_parsed'] += 1
         if stat == -1:
             return out

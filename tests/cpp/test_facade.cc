@@ -3,6 +3,7 @@
 // include/ and goetia_amd/libgoetia_amd.so; prints "FACADE OK" on success.
 #include <cassert>
 #include <cstdio>
+#include <fstream>
 #include <set>
 #include <string>
 #include <vector>
@@ -102,7 +103,8 @@ int main(int argc, char** argv) {
     CHECK(c2->n_unique() == 0);
     CHECK(g->get_raw() != nullptr);
 
-    // FASTX input and the chunk driver (argv[1]: goetia's test-fastq-reads.fq)
+    // FASTX input and the chunk driver (argv[1]: goetia's test-fastq-reads.fq,
+    // argv[2]: a scratch output file)
     if (argc > 1) {
         auto parser = FastxParser<>::build(argv[1]);
         uint64_t n_rec = 0, kmers = 0, expected_stops = 0;
@@ -154,6 +156,32 @@ int main(int argc, char** argv) {
         using DF = DiginormFilter<dBG<ByteStorage, CanLemireShifter>>;
         CHECK(DF::median_count_at_least(b5.read(0), 1, *g7));
         CHECK(!std::get<0>(dn->filter_sequence(b5.read(0))));
+        // FilterProcessor: the reads the filter passes, written as FASTQ
+        auto g8 = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(1000000, 4), 21);
+        auto dn8 = DF::Filter::build(g8, 1);
+        const std::string out = argc > 2 ? std::string(argv[2]) : std::string(argv[1]) + ".facade_out.fq";
+        uint64_t n_proc = 0, n_pass = 0;
+        {
+            auto fp = DF::Processor::build(dn8, out, 1000);
+            auto p6 = FastxParser<>::build(argv[1]);
+            uint64_t t8 = 0, stops8 = 0;
+            bool rem = true;
+            while (rem) {
+                std::tie(n_proc, t8, rem) = fp->advance(p6);
+                stops8 += rem;
+            }
+            n_pass = fp->n_passed();
+            CHECK(n_proc == n_rec && t8 == kmers && stops8 == expected_stops);
+        }
+        uint64_t lines = 0, d3 = 0;
+        {
+            std::ifstream in(out);
+            std::string line;
+            while (std::getline(in, line)) ++lines;
+        }
+        for (auto v : dn8->filter_sequences(b5)) d3 += v;
+        CHECK(n_pass == d1 && lines == 4 * n_pass && d3 == 0);
+        std::remove(out.c_str());
     }
     std::printf("FACADE OK\n");
     return 0;

@@ -33,6 +33,6 @@ def test_facade_runs_on_gpu(tmp_path):
     exe = str(tmp_path / 'facade')
     _compile(exe)
     fq = os.path.join(ROOT, 'tests', 'golden', 'data', 'test-fastq-reads.fq')
-    res = subprocess.run([exe, fq], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([exe, fq, str(tmp_path / 'filtered.fq')], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     assert 'FACADE OK' in res.stdout

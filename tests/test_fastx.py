@@ -181,3 +181,42 @@ def test_block_parallel_reader_vs_oracle(tmp_path, monkeypatch, fastq, block, th
             exp = ofx.parse(path, False, min_length)
             got = product_parse(path, 0, min_length, max_reads=13, max_bases=5000)
             assert got == exp, (k, min_length)
+
+
+def _product_qualities(path, min_length, max_reads=13, max_bases=5000):
+    p = FastxParser(path, False, min_length)
+    seqs, quals = [], []
+    while True:
+        batch, _, qs = p.next_batch(max_reads, max_bases, quality=True)
+        if batch.n_reads == 0:
+            break
+        seqs += [batch.read(i).decode() for i in range(batch.n_reads)]
+        quals += qs
+    p.close()
+    return seqs, [q.decode() if q else None for q in quals]
+
+
+@pytest.mark.parametrize('fastq', [False, True])
+@pytest.mark.parametrize('block,threads', [(1 << 20, 1), (200, 3), (1 << 20, 16)])
+def test_qualities_vs_oracle(tmp_path, monkeypatch, fastq, block, threads):
+    """Record::quality through the batch reader (both the record-by-record
+    and the block-parallel path) equals the restated kseq reader's."""
+    monkeypatch.setenv('GOETIA_AMD_FASTX_BLOCK', str(block))
+    monkeypatch.setenv('GOETIA_AMD_FASTX_THREADS', str(threads))
+    for k, maker in enumerate([_clean_file, lambda p, s, fq: _random_file(p, s, fq, False)]):
+        path = str(tmp_path / f'q{k}.{"fq" if fastq else "fa"}')
+        maker(path, 40 + k + block % 7, fastq)
+        for min_length in (0, 30):
+            exp = ofx.parse(path, False, min_length, with_quality=True)
+            seqs, quals = _product_qualities(path, min_length)
+            assert seqs == exp['sequences'] and quals == exp['qualities'], (k, min_length)
+            if fastq:
+                assert any(q for q in quals)
+            else:
+                assert not any(quals)
+
+
+def test_record_iteration_has_qualities():
+    recs = list(FastxParser(os.path.join(DATA, 'test-fastq-reads.fq')))
+    exp = ofx.parse(os.path.join(DATA, 'test-fastq-reads.fq'), with_quality=True)
+    assert [r.quality for r in recs] == exp['qualities'] and all(len(r.quality) == len(r.sequence) for r in recs)

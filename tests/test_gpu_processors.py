@@ -85,3 +85,62 @@ def test_process_file_equals_direct_insert(tmp_path):
     for a, b in zip(g1.get_raw(), g2.get_raw()):
         np.testing.assert_array_equal(a, b)
     assert g1.n_unique() == g2.n_unique()
+
+
+def test_chunked_process_yields_advances(oracle):
+    """chunked_process (pythonize_processors.py:12-33) yields every advance
+    that consumed reads, with the parser's skip count."""
+    path = os.path.join(DATA, 'fastx', 'mixed.fq')
+    K = 21
+    seqs = ofx.parse(path)['sequences']
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(oracle.primes_near_x(4, 50_000)), K)
+    proc = lg.InserterProcessor.build(g, 50)
+    parser = lg.FastxParser.build(path)
+    got = list(proc.chunked_process(parser))
+    exp = [(n, t) for n, t, _ in expected_advances(seqs, K, 50)]
+    dedup = [exp[0]] + [b for a, b in zip(exp, exp[1:]) if b[0] > a[0]]
+    assert [(n, t) for n, t, _ in got] == dedup
+    assert all(sk == parser.n_skipped() for _, _, sk in got[-1:])
+
+
+def _solid_passes(o, s, K, min_prop, threshold):
+    counts = o.insert_and_query_sequence(s, K, True)
+    n_not = int(np.sum(counts.astype(np.int64) < threshold))
+    ratio = np.float32(n_not) / np.float32(len(s) - K + 1)
+    return not (float(ratio) >= 1.0 - float(np.float32(min_prop)))
+
+
+@pytest.mark.parametrize('kind', ['diginorm', 'solid'])
+@pytest.mark.parametrize('name,interval', [('test-fastq-reads.fq', 777), ('fastx/mixed.fq', 50),
+                                           ('fastx/random_wrapped.fa', 5000)])
+def test_filter_processor_vs_oracle(oracle, tmp_path, kind, name, interval):
+    """FilterProcessor (processors.hh:355-440): the same advances as goetia's
+    FileProcessor, the passed reads written as Record::write_fastx writes them,
+    and the tables of filtering the reads one after another."""
+    from goetia_amd.parsing import fastx_text
+    path = os.path.join(DATA, name)
+    K = 21
+    parsed = ofx.parse(path, with_quality=True)
+    sizes = oracle.primes_near_x(4, 50_000)
+    G = lg.dBG[lg.ByteStorage, lg.CanLemireShifter]
+    g = G.build(lg.ByteStorage(sizes), K)
+    family = lg.DiginormFilter[G] if kind == 'diginorm' else lg.StreamingSolidFilter[G]
+    f = family.Filter.build(g, 2) if kind == 'diginorm' else family.Filter.build(g, 0.75, 2)
+    out = str(tmp_path / 'passed.fx')
+    proc = family.Processor.build(f.__smartptr__(), out, interval)
+    got = run(proc, lg.FastxParser.build(path))
+    assert got == expected_advances(parsed['sequences'], K, interval)
+    proc.close()
+    o = oracle.Storage(oracle.BYTE, sizes)
+    text = []
+    for nm, s, q in zip(parsed['names'], parsed['sequences'], parsed['qualities']):
+        if len(s) < K:
+            continue
+        ok = o.diginorm_filter(s, K, 2) == 1 if kind == 'diginorm' else _solid_passes(o, s, K, 0.75, 2)
+        if ok:
+            text.append(fastx_text(nm, s, q))
+    assert open(out).read() == ''.join(text)
+    assert proc.n_passed() == len(text) and proc.n_sequences() == len(parsed['sequences'])
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+    assert len(text) > 0 or kind == 'solid'
