@@ -62,6 +62,8 @@ def parse():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-chunks', action='store_true')
     p.add_argument('--no-iq', action='store_true', help='skip the one-pass insert_and_query timing')
+    p.add_argument('--no-random-access', action='store_true',
+                   help='skip the random-access roofline microbenchmark (4 GiB buffer)')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
     p.add_argument('--workload', default='cms_20m',
@@ -80,6 +82,42 @@ def kernel_times(N, names):
         cnt, ms = C.c_uint64(), C.c_double()
         N.call('gt_profile_get', n.encode(), C.byref(cnt), C.byref(ms))
         out[n] = (cnt.value, ms.value)
+    return out
+
+
+def random_access(N, table_bytes, n_ops=1 << 30):
+    """The device's random-access roofline: uniform random 32-bit reads,
+    atomicOr and atomicCAS (saturating byte increment) over a 4 GiB buffer
+    from every CU (gt_membench_random), and random reads over a buffer of the
+    tables' size; best of three runs each, in ops/s."""
+    import ctypes as C
+
+    import torch
+    n_words = 1 << 30
+    buf = torch.zeros(n_words, dtype=torch.int32, device='cuda')
+    stream = torch.cuda.current_stream()
+
+    def rate(words, kind):
+        best = 0.0
+        for rep in range(3):
+            done = C.c_uint64()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            N.call('gt_membench_random', C.c_void_p(buf.data_ptr()), words, n_ops, kind, rep + 1,
+                   C.c_void_p(stream.cuda_stream), C.byref(done))
+            e1.record(stream)
+            e1.synchronize()
+            best = max(best, done.value / (e0.elapsed_time(e1) / 1e3))
+        return round(best, 1)
+
+    out = {'buffer_bytes': n_words * 4}
+    for kind, name in ((0, 'read'), (1, 'atomic_or'), (2, 'atomic_cas')):
+        out[f'{name}_ops_per_s'] = rate(n_words, kind)
+    tw = max(1, min(n_words, table_bytes // 4))
+    out['table_size_bytes'] = tw * 4
+    out['read_ops_per_s_table_size'] = rate(tw, 0)
+    del buf
+    torch.cuda.empty_cache()
     return out
 
 
@@ -214,6 +252,17 @@ def main():
                  'p99_ms': round(float(np.percentile(lat, 99)), 2)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(cfg, batch, args.cpu_sample_reads)
+
+    # the query against the measured random-read roofline: one random table
+    # byte per k-mer and table
+    if roofline is not None and not args.no_random_access:
+        ra = random_access(N, sum(store.table_bytes(i) for i in range(store.n_tables())))
+        q_ms = roofline['kernel_ms_per_step'].get('query')
+        if q_ms:
+            q_rate = n_kmers * store.n_tables() / (q_ms / 1e3)
+            ra['query_random_reads_per_s'] = round(q_rate, 1)
+            ra['query_frac_of_random_read'] = round(q_rate / ra['read_ops_per_s_table_size'], 4)
+        roofline['random_access'] = ra
 
     out = {
         'metric': 'canonical k-mers/s insert+query (1 GPU)',

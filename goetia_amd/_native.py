@@ -151,6 +151,7 @@ SIGNATURES = {
     'gt_solid_filter': [st_p, u16, cint, cint, vp, vp, u64, C.c_float, u32, vp],
     'gt_diginorm_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, u32, vp, vp],
     'gt_diginorm': [st_p, u16, cint, cint, vp, vp, u64, u32, vp],
+    'gt_membench_random': [vp, u64, u64, cint, u64, vp, u64p],
     'gt_fastx_open': [C.c_char_p, cint, u32, C.POINTER(vp)],
     'gt_fastx_next_batch': [vp, u64, u64, vp, vp, vp, u64, vp, u64p],
     'gt_fastx_next_batch_qual': [vp, u64, u64, vp, vp, vp, vp, vp, u64, vp, u64p],

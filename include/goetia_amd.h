@@ -319,12 +319,22 @@ int gt_server_free(gt_server* S);
  * library launches is bracketed by a pair of hipEvents recorded on the
  * kernel's own stream; gt_profile_get synchronizes and returns, for one
  * kernel name ("claim", "apply", "exact_lookup", "exact_insert",
- * "exact_resolve", "query", "hash", "degrees", "seq_insert_query"), the number
- * of launches and their summed duration in milliseconds since the last
- * reset. */
+ * "exact_resolve", "query", "hash", "degrees", "seq_insert_query", the binned
+ * insert's "bin_hash", "bin_count", "bin_route", "bin_apply", "bin_new",
+ * "bin_iq", diginorm's "dn_replay", "dn_decide", "dn_apply"), the number of
+ * launches and their summed duration in milliseconds since the last reset. */
 int gt_profile_enable(int on);
 int gt_profile_reset(void);
 int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
+
+/* Random-access roofline (no counterpart in goetia; measurement only):
+ * at least n_ops uniform random 32-bit accesses to the device buffer
+ * buf[0, n_words) from every CU, asynchronously on `stream`: kind 0 reads,
+ * 1 atomicOr of one bit, 2 atomicCAS saturating increment of the low byte.
+ * *ops_done receives the number issued.  The buffer's contents change for
+ * kinds 1 and 2. */
+int gt_membench_random(void* buf, uint64_t n_words, uint64_t n_ops, int kind, uint64_t seed, void* stream,
+                       uint64_t* ops_done);
 
 /* ------------------------------------------------------------ consumers */
 
