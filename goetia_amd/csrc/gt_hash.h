@@ -99,7 +99,13 @@ GT_HD unsigned char gt_complement(unsigned char c) {
     return c == 'A' ? 'T' : c == 'T' ? 'A' : c == 'C' ? 'G' : c == 'G' ? 'C' : 0;
 }
 
-GT_HD int gt_is_acgt(unsigned char c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+// Whether c is one of A, C, G, T, without branches: c in 64..127 and its bit
+// set in a mask over 64..127 ('A' = 64 + 1, 'C' = 64 + 3, 'G' = 64 + 7,
+// 'T' = 64 + 20).
+GT_HD int gt_is_acgt(unsigned char c) {
+    constexpr uint64_t kMask = (1ULL << 1) | (1ULL << 3) | (1ULL << 7) | (1ULL << 20);
+    return (int)(((c & 0xC0u) == 0x40u) & (unsigned)((kMask >> (c & 63u)) & 1u));
+}
 
 // Exact h mod p for any 64-bit h and p >= 1 without a hardware divide:
 // m = floor((2^64-1)/p) gives q = mulhi(h, m) in [floor(h/p)-2, floor(h/p)],
@@ -120,4 +126,15 @@ GT_HD uint64_t gt_fastmod(uint64_t h, uint64_t p, uint64_t m) {
     if (r >= p) r -= p;
     if (r >= p) r -= p;
     return r;
+}
+
+// The same for p < 2^30 (m = gt_mod_magic(p)): r = h - q*p is below 3p < 2^32,
+// so it is the low word of h - q*p, and each correction is min(r, r - p)
+// (r - p wraps above r when r < p).
+GT_HD uint32_t gt_min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+GT_HD uint32_t gt_fastmod30(uint64_t h, uint32_t p, uint64_t m) {
+    const uint64_t q = gt_mulhi64(h, m);
+    uint32_t r = (uint32_t)h - (uint32_t)q * p;
+    r = gt_min_u32(r, r - p);
+    return gt_min_u32(r, r - p);
 }

@@ -288,15 +288,29 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
         int64_t s_end = s0 + PER;
         if (s_end > b1) s_end = b1;
         if (s0 < s_end) {
-            auto OFF = [&](uint32_t r) -> int64_t { return cached ? sh.roff[r - rl] : R.off[r]; };
+            // read offsets from the LDS cache or from memory; the two cases are
+            // kept apart (uniform branch) so each compiles to its own loads
             uint32_t lo = rl, hi = rh;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (OFF(mid) <= s0) lo = mid; else hi = mid - 1;
+            if (cached) {
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (sh.roff[mid - rl] <= s0) lo = mid; else hi = mid - 1;
+                }
+            } else {
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (R.off[mid] <= s0) lo = mid; else hi = mid - 1;
+                }
             }
             uint32_t r = lo;
-            int64_t rstart = OFF(r);
-            int64_t rend = OFF(r + 1);
+            int64_t rstart, rend;
+            if (cached) {
+                rstart = sh.roff[r - rl];
+                rend = sh.roff[r + 1 - rl];
+            } else {
+                rstart = R.off[r];
+                rend = R.off[r + 1];
+            }
 
             const uint8_t* B = sh.bytes;
             const int li = (int)(s0 - t0);
@@ -319,10 +333,18 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
                     rc = gt_rotr(rc ^ gt_rotl(sh.tr[in], Kr) ^ sh.tr[out], 1);
                     bad += (int)!gt_is_acgt(in) - (int)!gt_is_acgt(out);
                 }
-                while (s >= rend && r < rh) {
-                    ++r;
-                    rstart = rend;
-                    rend = OFF(r + 1);
+                if (cached) {
+                    while (s >= rend && r < rh) {
+                        ++r;
+                        rstart = rend;
+                        rend = sh.roff[r + 1 - rl];
+                    }
+                } else {
+                    while (s >= rend && r < rh) {
+                        ++r;
+                        rstart = rend;
+                        rend = R.off[r + 1];
+                    }
                 }
                 if (s < rstart || s + K > rend) continue;
                 op(s, r, rstart, fw, rc, R.mode == kRaw || bad == 0);
@@ -1167,6 +1189,8 @@ struct BinL1Op {
     uint64_t* slots;          // 4 x kDigits1 match slots (the walker's read-offset cache, idle here)
     int64_t b_lo;             // first start of the sub-batch (tiles may begin before it)
     int64_t t0 = 0;
+    uint32_t kr = 0xFFFFFFFFu;  // read whose packed-index offset is cached in kd
+    int64_t kd = 0;             // kmer_off[kr] - start of read kr - kbase
 
     __device__ void begin_tile(int64_t tile_base) {
         t0 = tile_base;
@@ -1177,7 +1201,11 @@ struct BinL1Op {
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
         if (s < b_lo) return;
-        const uint32_t ord = (uint32_t)((uint64_t)(kmer_off[r] + (s - rstart)) - kbase);
+        if (r != kr) {
+            kr = r;
+            kd = kmer_off[r] - rstart - (int64_t)kbase;
+        }
+        const uint32_t ord = (uint32_t)(s + kd);
         if (P.hasev) P.hasev[ord] = valid ? 1 : 0;
         if (!valid) return;
         const int w = (int)(s - t0);
@@ -1185,7 +1213,7 @@ struct BinL1Op {
         sh->ord[w] = ord;
 #pragma unroll
         for (uint32_t i = 0; i < kBinMaxTables; ++i)
-            if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + (uint32_t)gt_fastmod(h, T.p[i], T.m[i]);
+            if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + gt_fastmod30(h, (uint32_t)T.p[i], T.m[i]);
     }
 
     __device__ void flush() {}
@@ -1311,30 +1339,19 @@ __device__ __forceinline__ uint32_t bin_run_of(const BinRuns& sh, uint32_t i) {
     return lo;
 }
 
-// Events [c0, c0 + m) of the loaded item, m <= kBinChunk: one binary search per
-// 64 events (into `first`, 64 entries), then each lane steps forward from its
-// block's run; lanes read consecutive events.  f(i - c0, event i).  Batched:
-// a thread's kBinChunk / kBlock loads are all issued before the first use
-// (more registers, fewer serialised memory latencies).
-template <bool kBatched, class F>
-__device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& sh, uint32_t* first, uint32_t c0,
-                                               uint32_t m, F&& f) {
+// Events [c0, c0 + m) of the loaded item, m <= kBinChunk, are found with one
+// binary search per 64 events (into `first`, by threads 0..63) and a forward
+// step from there per lane; lanes read consecutive events.
+__device__ __forceinline__ void bin_chunk_first(const BinRuns& sh, uint32_t* first, uint32_t c0, uint32_t m) {
     if (threadIdx.x < (unsigned)(kBinChunk / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
-    __syncthreads();
-    if (!kBatched) {
-        for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
-            const uint32_t i = c0 + j;
-            uint32_t k = first[j >> 6];
-            while (sh.rpre[k + 1] <= i) ++k;
-            f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
-        }
-        return;
-    }
-    // all of a thread's loads are issued before any is used
-    constexpr int Q = kBinChunk / kBlock;
-    uint64_t v[Q];
+}
+
+// Issues a thread's kBinChunk / kBlock loads of the chunk into v (first set
+// and visible); none is used here, so they stay in flight.
+__device__ __forceinline__ void bin_chunk_load(const BinPlan& P, const BinRuns& sh, const uint32_t* first, uint32_t c0,
+                                               uint32_t m, uint64_t* v) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    for (int q = 0; q < kBinChunk / kBlock; ++q) {
         const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
         v[q] = 0;
         if (j < m) {
@@ -1344,10 +1361,19 @@ __device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& 
             v[q] = P.ent1[sh.rstart[k] + (i - sh.rpre[k])];
         }
     }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
-        if (j < m) f(j, v[q]);
+}
+
+// f(i - c0, event i) for the events of a chunk, a lane per event.
+template <class F>
+__device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& sh, uint32_t* first, uint32_t c0,
+                                               uint32_t m, F&& f) {
+    bin_chunk_first(sh, first, c0, m);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
+        const uint32_t i = c0 + j;
+        uint32_t k = first[j >> 6];
+        while (sh.rpre[k + 1] <= i) ++k;
+        f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
     }
 }
 
@@ -1368,7 +1394,7 @@ __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
         const uint32_t n = bin_load_runs(P, d1, grp, sh);
         for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
             const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
-            bin_for_events<false>(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
+            bin_for_events(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
             __syncthreads();
         }
         for (uint32_t d2 = threadIdx.x; d2 <= mask2; d2 += kBlock)
@@ -1404,11 +1430,27 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
         for (uint32_t d2 = threadIdx.x; d2 < (uint32_t)kDigits2; d2 += kBlock)
             sh.cur[d2] = d2 <= mask2 ? P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] : 0u;
         const uint32_t n = bin_load_runs(P, d1, grp, sh.runs);
+        // chunk c's events are loaded while chunk c - 1 is sorted
+        uint64_t v[kBinChunk / kBlock];
+        {
+            const uint32_t m0 = n < (uint32_t)kBinChunk ? n : (uint32_t)kBinChunk;
+            bin_chunk_first(sh.runs, sh.first, 0, m0);
+            __syncthreads();
+            bin_chunk_load(P, sh.runs, sh.first, 0, m0, v);
+        }
         for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
             const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
-            bin_for_events<true>(P, sh.runs, sh.first, c0, m, [&](uint32_t j, uint64_t e) { sh.ev[j] = e; });
+#pragma unroll
+            for (int q = 0; q < kBinChunk / kBlock; ++q) {
+                const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+                if (j < m) sh.ev[j] = v[q];
+            }
             for (int d = lane; d <= kDigits2; d += 64) wc[d] = 0;
+            const uint32_t c1 = c0 + kBinChunk;
+            const uint32_t m1 = c1 < n ? (n - c1 < (uint32_t)kBinChunk ? n - c1 : (uint32_t)kBinChunk) : 0u;
+            bin_chunk_first(sh.runs, sh.first, c1, m1);
             __syncthreads();
+            bin_chunk_load(P, sh.runs, sh.first, c1, m1, v);
             const uint32_t per_wave = (m + 255u) / 256u * 64u;
             const uint32_t i0 = (uint32_t)wave * per_wave;
             for (uint32_t c = 0; c < per_wave; c += 64) {
