@@ -43,7 +43,7 @@ BYTES_PER_KMER = {'claim': 4 * 64, 'apply': 4 * 2 * 64, 'query': 4 * 64,
 # HBM bytes per k-mer (read + write) measured with rocprofv3 FETCH_SIZE /
 # WRITE_SIZE on this workload and scaled by a 1 GiB copy (the table and the
 # commands are in profiles/cms_20m_hbm_traffic.md).
-MEASURED_BYTES_PER_KMER = {'bin_hash': 36.2, 'bin_count': 37.1, 'bin_route': 91.1, 'bin_apply': 64.0, 'bin_new': 1.2,
+MEASURED_BYTES_PER_KMER = {'bin_hash': 36.3, 'bin_count': 37.1, 'bin_route': 93.6, 'bin_apply': 64.0, 'bin_new': 1.2,
                            'query': 515.5}
 
 
@@ -121,21 +121,37 @@ def random_access(N, table_bytes, n_ops=1 << 30):
     return out
 
 
-def cpu_baseline(cfg, batch, sample_reads):
+def cpu_baseline(cfg, batch, sample_reads, mt_reads=2_000_000):
     """The oracle (sequential C restatement, 1 thread) on the first
-    `sample_reads` reads: insert then query, k-mers/s."""
+    `sample_reads` reads: insert then query, k-mers/s; beside it the same on
+    the host's cores (at most 16, the share of one GPU) with atomic table
+    updates, as goetia's storages do, on the first `mt_reads` reads."""
     from oracle import pyoracle as po
+    sizes = po.primes_near_x(cfg['n_tables'], cfg['max_table'])
     n = min(sample_reads, batch.n_reads)
     off = batch.offsets[:n + 1].copy()
     seq = batch.seq[:off[-1]].copy()
-    st = po.Storage(po.BYTE, po.primes_near_x(cfg['n_tables'], cfg['max_table']))
+    st = po.Storage(po.BYTE, sizes)
     t0 = time.perf_counter()
     hashed, _ = st.insert_reads(seq, off, cfg['K'], True, po.MODE_SKIP_N)
     st.query_reads(seq, off, cfg['K'], True, po.MODE_SKIP_N)
     dt = time.perf_counter() - t0
+    del st
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n2 = min(mt_reads, batch.n_reads)
+    off2 = batch.offsets[:n2 + 1].copy()
+    seq2 = batch.seq[:off2[-1]].copy()
+    st2 = po.Storage(po.BYTE, sizes)
+    t0 = time.perf_counter()
+    hashed2 = st2.insert_reads_mt(seq2, off2, cfg['K'], True, po.MODE_SKIP_N, threads)
+    st2.query_reads_mt(seq2, off2, cfg['K'], True, po.MODE_SKIP_N, threads)
+    dt2 = time.perf_counter() - t0
     return {'value': hashed / dt, 'unit': 'kmers/s', 'cores': 1, 'kind': 'port',
             'sample': f'first {n} reads of the workload ({hashed} k-mers), insert + query_sequence, '
-                      f'oracle/oracle.c single-threaded, {dt:.1f} s'}
+                      f'oracle/oracle.c single-threaded, {dt:.1f} s',
+            'all_cores': {'value': hashed2 / dt2, 'cores': threads,
+                          'sample': f'first {n2} reads ({hashed2} k-mers), insert with atomic table updates '
+                                    f'+ query_sequence on {threads} threads, {dt2:.1f} s'}}
 
 
 def main():

@@ -7,6 +7,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -657,6 +658,118 @@ uint64_t or_diginorm_reads(or_storage* st, int K, int canonical, int mode, unsig
         n += p == 1;
     }
     return n;
+}
+
+/* ------------------------------------------------- all host cores */
+
+/* One window into Bloom / count-min tables with atomic table updates, as the
+ * reference's storages do with __sync builtins (bitstorage.hh:195-219,
+ * bytestorage.cc:60-113): OR of a bit, saturating increment of a byte by
+ * compare-and-swap.  Both commute, so after any interleaving the tables equal
+ * a sequential insert of the same windows; the counters (which depend on the
+ * order) are not kept here. */
+static void atomic_insert(or_storage* st, uint64_t h) {
+    for (uint32_t i = 0; i < st->n_tables; ++i) {
+        uint64_t bin = h % st->sizes[i];
+        if (st->kind == OR_BIT) {
+            __atomic_fetch_or(&st->tables[i][bin / 8], (uint8_t)(1u << (bin % 8)), __ATOMIC_RELAXED);
+        } else {
+            uint8_t* c = &st->tables[i][bin];
+            uint8_t cur = __atomic_load_n(c, __ATOMIC_RELAXED);
+            while (cur < 255 &&
+                   !__atomic_compare_exchange_n(c, &cur, (uint8_t)(cur + 1), 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+            }
+        }
+    }
+}
+
+typedef struct {
+    or_storage* st;
+    int K, canonical, mode;
+    const char* seq;
+    const int64_t* offsets;
+    const int64_t* kmer_off;     /* query only */
+    int16_t* counts;             /* query only */
+    uint64_t r0, r1;
+    uint64_t hashed;
+} or_mt_part;
+
+static void* mt_insert(void* arg) {
+    or_mt_part* a = (or_mt_part*)arg;
+    for (uint64_t r = a->r0; r < a->r1; ++r) {
+        or_windows w;
+        if (!windows_of(a->seq + a->offsets[r], (size_t)(a->offsets[r + 1] - a->offsets[r]), a->K, a->mode, &w))
+            continue;
+        for (size_t i = 0; i < w.n; ++i)
+            if (w.valid[i]) {
+                atomic_insert(a->st, wval(&w, i, a->canonical));
+                a->hashed++;
+            }
+        windows_free(&w);
+    }
+    return NULL;
+}
+
+static void* mt_query(void* arg) {
+    or_mt_part* a = (or_mt_part*)arg;
+    for (uint64_t r = a->r0; r < a->r1; ++r)
+        a->hashed += or_query_sequence(a->st, a->K, a->canonical, a->mode, a->seq + a->offsets[r],
+                                       (size_t)(a->offsets[r + 1] - a->offsets[r]), a->counts + a->kmer_off[r], NULL);
+    return NULL;
+}
+
+/* Reads split into n_threads ranges of about equal bases, each on its own
+ * thread. */
+static uint64_t run_parts(void* (*fn)(void*), or_mt_part* proto, uint64_t n_reads, int n_threads) {
+    if (n_threads < 1) n_threads = 1;
+    or_mt_part* parts = (or_mt_part*)calloc((size_t)n_threads, sizeof(or_mt_part));
+    pthread_t* th = (pthread_t*)calloc((size_t)n_threads, sizeof(pthread_t));
+    const int64_t total = n_reads ? proto->offsets[n_reads] : 0;
+    uint64_t r = 0;
+    for (int t = 0; t < n_threads; ++t) {
+        parts[t] = *proto;
+        parts[t].r0 = r;
+        const int64_t target = total * (t + 1) / n_threads;
+        while (r < n_reads && (t == n_threads - 1 || proto->offsets[r] < target)) ++r;
+        parts[t].r1 = r;
+        parts[t].hashed = 0;
+    }
+    for (int t = 1; t < n_threads; ++t) pthread_create(&th[t], NULL, fn, &parts[t]);
+    fn(&parts[0]);
+    uint64_t hashed = parts[0].hashed;
+    for (int t = 1; t < n_threads; ++t) {
+        pthread_join(th[t], NULL);
+        hashed += parts[t].hashed;
+    }
+    free(parts);
+    free(th);
+    return hashed;
+}
+
+/* insert_sequence over reads on n_threads threads (Bloom / count-min only;
+ * the tables end as a sequential insert leaves them, the counters are not
+ * updated).  Returns the k-mers hashed. */
+uint64_t or_insert_reads_mt(or_storage* st, int K, int canonical, int mode, const char* seq,
+                            const int64_t* offsets, uint64_t n_reads, int n_threads) {
+    if (st->kind != OR_BIT && st->kind != OR_BYTE) return 0;
+    or_mt_part proto = {st, K, canonical, mode, seq, offsets, NULL, NULL, 0, 0, 0};
+    return run_parts(mt_insert, &proto, n_reads, n_threads);
+}
+
+/* query_sequence over reads on n_threads threads, counts packed per read as
+ * or_query_reads packs them. */
+uint64_t or_query_reads_mt(const or_storage* st, int K, int canonical, int mode, const char* seq,
+                           const int64_t* offsets, uint64_t n_reads, int16_t* counts, int n_threads) {
+    int64_t* ko = (int64_t*)malloc((n_reads + 1) * sizeof(int64_t));
+    ko[0] = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        const int64_t len = offsets[r + 1] - offsets[r];
+        ko[r + 1] = ko[r] + (len >= K ? len - K + 1 : 0);
+    }
+    or_mt_part proto = {(or_storage*)st, K, canonical, mode, seq, offsets, ko, counts, 0, 0, 0};
+    const uint64_t hashed = run_parts(mt_query, &proto, n_reads, n_threads);
+    free(ko);
+    return hashed;
 }
 
 /* Counts are packed per read: read r owns max(0, len_r - K + 1) entries. */

@@ -107,6 +107,13 @@ uint64_t or_query_reads(const or_storage* st, int K, int canonical, int mode,
 
 uint64_t or_exact_keys(const or_storage* st, uint64_t* out, uint64_t cap);
 
+/* The same drivers on n_threads host threads with atomic table updates
+ * (Bloom / count-min; counters not updated by the insert). */
+uint64_t or_insert_reads_mt(or_storage* st, int K, int canonical, int mode, const char* seq,
+                            const int64_t* offsets, uint64_t n_reads, int n_threads);
+uint64_t or_query_reads_mt(const or_storage* st, int K, int canonical, int mode, const char* seq,
+                           const int64_t* offsets, uint64_t n_reads, int16_t* counts, int n_threads);
+
 /* DiginormFilter (count-min graphs): one read, then reads in order. */
 int      or_diginorm_filter(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
                             const char* s, size_t len);

@@ -70,6 +70,10 @@ def lib():
             'or_query_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
                                             C.c_uint64, i16p]),
             'or_exact_keys': (C.c_uint64, [sp, u64p, C.c_uint64]),
+            'or_insert_reads_mt': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p, C.c_uint64,
+                                                C.c_int]),
+            'or_query_reads_mt': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p, C.c_uint64,
+                                               i16p, C.c_int]),
             'or_diginorm_filter': (C.c_int, [sp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_char_p,
                                              C.c_size_t]),
             'or_diginorm_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_char_p,
@@ -249,6 +253,23 @@ class Storage:
         lib().or_diginorm_reads(self.ref, K, int(canonical), mode, cutoff, seq.ctypes.data_as(C.c_char_p),
                                 _p(offsets, C.c_int64), n_reads, _p(passed, C.c_uint8))
         return passed[:n_reads].astype(bool)
+
+    def insert_reads_mt(self, seq, offsets, K, canonical=True, mode=MODE_RAW, threads=1):
+        """insert_sequence over reads on `threads` host threads (Bloom /
+        count-min; the counters are not updated)."""
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        return lib().or_insert_reads_mt(self.ref, K, int(canonical), mode, seq.ctypes.data_as(C.c_char_p),
+                                        _p(offsets, C.c_int64), len(offsets) - 1, int(threads))
+
+    def query_reads_mt(self, seq, offsets, K, canonical=True, mode=MODE_RAW, threads=1):
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        n = int(np.maximum(np.diff(offsets) - K + 1, 0).sum())
+        out = np.zeros(max(n, 1), np.int16)
+        lib().or_query_reads_mt(self.ref, K, int(canonical), mode, seq.ctypes.data_as(C.c_char_p),
+                                _p(offsets, C.c_int64), len(offsets) - 1, _p(out, C.c_int16), int(threads))
+        return out[:n]
 
     def query_reads(self, seq, offsets, K, canonical=True, mode=MODE_RAW):
         seq = np.ascontiguousarray(seq, np.uint8)

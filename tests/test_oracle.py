@@ -282,3 +282,33 @@ def test_planted_mistake_is_caught(oracle, golden):
     wrong_rc = np.array([oracle.hash_kmer(s[i:i + 31][::-1])[0] for i in range(len(s) - 30)], np.uint64)
     assert not np.array_equal(wrong_rc, golden['hash_bytes_K31_rc'])
     np.testing.assert_array_equal(rc, golden['hash_bytes_K31_rc'])
+
+
+@pytest.mark.parametrize('kind', [1, 2])
+@pytest.mark.parametrize('threads', [1, 3, 8])
+def test_multithreaded_tables_equal_sequential(oracle, kind, threads):
+    """The all-cores CPU baseline (atomic table updates on several threads)
+    leaves the sequential insert's tables and query counts."""
+    rng = np.random.default_rng(kind * 10 + threads)
+    genome = rng.choice(np.frombuffer(b'ACGT', np.uint8), 20000)
+    reads = []
+    for _ in range(3000):
+        L = int(rng.integers(10, 200))
+        s = int(rng.integers(0, len(genome) - L))
+        r = genome[s:s + L].copy()
+        if rng.random() < 0.05:
+            r[int(rng.integers(0, L))] = ord('N')
+        reads.append(r.tobytes())
+    off = np.zeros(len(reads) + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    seq = np.frombuffer(b''.join(reads), np.uint8).copy()
+    sizes = oracle.primes_near_x(4, 5000)
+    for mode in (oracle.MODE_RAW, oracle.MODE_SKIP_N):
+        a, b = oracle.Storage(kind, sizes), oracle.Storage(kind, sizes)
+        ha, _ = a.insert_reads(seq, off, 21, True, mode)
+        hb = b.insert_reads_mt(seq, off, 21, True, mode, threads)
+        assert ha == hb
+        for x, y in zip(a.tables(), b.tables()):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(a.query_reads(seq, off, 21, True, mode),
+                                      b.query_reads_mt(seq, off, 21, True, mode, threads))
