@@ -144,3 +144,41 @@ def test_filter_processor_vs_oracle(oracle, tmp_path, kind, name, interval):
     for a, b in zip(g.get_raw(), o.tables()):
         np.testing.assert_array_equal(a, b)
     assert len(text) > 0 or kind == 'solid'
+
+
+def test_async_driver_over_samples(oracle):
+    """AsyncSequenceProcessor (processors.py:133-351) driving an
+    InserterProcessor over two samples: one Interval per advance with the
+    running (sequence, t), then SampleFinished; the tables are the oracle's
+    for both samples' reads in order."""
+    from goetia_amd.messages import Interval, SampleFinished
+    from goetia_amd.processors import AsyncSequenceProcessor
+    K, interval = 21, 300
+    paths = [os.path.join(DATA, 'test-fastq-reads.fq'), os.path.join(DATA, 'fastx', 'mixed.fq')]
+    sizes = oracle.primes_near_x(4, 60_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    proc = lg.InserterProcessor.build(g, interval)
+    driver = AsyncSequenceProcessor(proc, iter([((p,), os.path.basename(p)) for p in paths]))
+    got = []
+    driver.add_listener('worker_q', 'test').on_messages([Interval, SampleFinished],
+                                                        lambda m: got.append((type(m).__name__, m.sequence, m.t)))
+    driver.run()
+    seqs = [ofx.parse(p)['sequences'] for p in paths]
+    timer = IntervalCounter(interval)
+    exp, n = [], 0
+    for sample in seqs:
+        for s in sample:
+            n += 1
+            if timer.poll(max(0, len(s) - K + 1)):
+                exp.append(('Interval', n, timer.total()))
+        if not exp or exp[-1][1] != n:
+            exp.append(('Interval', n, timer.total()))
+        exp.append(('SampleFinished', n, timer.total()))
+    assert got == exp
+    o = oracle.Storage(oracle.BYTE, sizes)
+    for sample in seqs:
+        for s in sample:
+            if len(s) >= K:
+                o.insert_sequence(s, K, True)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
