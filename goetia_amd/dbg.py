@@ -429,13 +429,44 @@ class dBG(metaclass=_dBGMeta):
             self.set_cursor(node)
         return self.in_degree() > 1 or self.out_degree() > 1
 
-    def find_decision_kmers(self, sequence, min_count=1):
+    def build_left_kmers(self, nodes, root):
+        """unitig_walker.hh:612-621: the left neighbours' k-mers,
+        symbol + prefix(root)."""
+        prefix = self.prefix(root)
+        return [Kmer(n.hash, n.symbol + prefix) for n in nodes]
+
+    def build_right_kmers(self, nodes, root):
+        """unitig_walker.hh:631-640: suffix(root) + symbol."""
+        suffix = self.suffix(root)
+        return [Kmer(n.hash, suffix + n.symbol) for n in nodes]
+
+    def get_decision_neighbors(self, root=None):
+        """unitig_walker.hh:1085-1117: (is decision, (left k-mers, right
+        k-mers)) of the cursor (or root); the pair is empty unless it is a
+        decision k-mer, as the reference leaves `result` untouched then."""
+        if root is not None:
+            self.set_cursor(root)
+        root = self.get_cursor()
+        left = self.build_left_kmers(self.filter_nodes(self.left_extensions()), root)
+        right = self.build_right_kmers(self.filter_nodes(self.right_extensions()), root)
+        if len(left) > 1 or len(right) > 1:
+            return True, (left, right)
+        return False, ([], [])
+
+    def find_decision_kmers(self, sequence, min_count=1, neighbors=False):
         """unitig_walker.hh:1025-1045: positions and hashes of the windows of
-        `sequence` with in or out degree > 1, plus their (in, out) degrees."""
+        `sequence` with in or out degree > 1, plus their (in, out) degrees
+        (from the GPU degree kernel); with neighbors=True also each decision
+        k-mer's (left k-mers, right k-mers), as get_decision_neighbors gives
+        them (count >= 1)."""
         self._check_len(sequence)
         ind, outd = self.degrees([sequence], min_count=min_count)
         hs = list(self.hashes(sequence))
         pos = [i for i in range(len(ind)) if ind[i] != 0xFF and (ind[i] > 1 or outd[i] > 1)]
-        return pos, [hs[i] for i in pos], [(int(ind[i]), int(outd[i])) for i in pos]
+        out = (pos, [hs[i] for i in pos], [(int(ind[i]), int(outd[i])) for i in pos])
+        if not neighbors:
+            return out
+        s = sequence if isinstance(sequence, str) else _bytes(sequence).decode()
+        return out + ([self.get_decision_neighbors(s[i:i + self.K])[1] for i in pos],)
 
 

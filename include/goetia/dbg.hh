@@ -62,6 +62,7 @@ class dBG {
     template <bool Dir>
     using shift_type = Shift<hash_type, Dir>;
     typedef std::pair<std::vector<shift_type<DIR_LEFT>>, std::vector<shift_type<DIR_RIGHT>>> shift_pair_type;
+    typedef std::pair<std::vector<kmer_type>, std::vector<kmer_type>> neighbor_pair_type;   // unitig_walker.hh:188-189
 
     const uint16_t K;
 
@@ -305,6 +306,49 @@ class dBG {
         const size_t l = in_degree(), r = out_degree();
         degree = (uint8_t)(l + r);
         return l > 1 || r > 1;
+    }
+
+    // unitig_walker.hh:612-640: neighbour k-mers of a root from its shifts.
+    std::vector<kmer_type> build_left_kmers(const std::vector<shift_type<DIR_LEFT>>& nodes,
+                                            const std::string& root) {
+        std::vector<kmer_type> kmers;
+        const std::string pre = root.substr(0, K - 1);
+        for (const auto& n : nodes) kmers.push_back(kmer_type(n.hash, n.symbol + pre));
+        return kmers;
+    }
+    std::vector<kmer_type> build_right_kmers(const std::vector<shift_type<DIR_RIGHT>>& nodes,
+                                             const std::string& root) {
+        std::vector<kmer_type> kmers;
+        const std::string suf = root.substr(root.size() - K + 1);
+        for (const auto& n : nodes) kmers.push_back(kmer_type(n.hash, suf + n.symbol));
+        return kmers;
+    }
+
+    // unitig_walker.hh:1085-1117: whether root is a decision k-mer, and then
+    // its (left, right) neighbour k-mers in result.
+    bool get_decision_neighbors(const std::string& root, neighbor_pair_type& result) {
+        set_cursor(root);
+        const std::string r = get_cursor();
+        auto left = build_left_kmers(filter_nodes(left_extensions()), r);
+        auto right = build_right_kmers(filter_nodes(right_extensions()), r);
+        if (left.size() > 1 || right.size() > 1) {
+            result = std::make_pair(left, right);
+            return true;
+        }
+        return false;
+    }
+
+    // unitig_walker.hh:1025-1045 with the decision neighbours.
+    void find_decision_kmers(const std::string& sequence, std::vector<uint32_t>& decision_positions,
+                             std::vector<hash_type>& decision_hashes,
+                             std::vector<neighbor_pair_type>& decision_neighbors) {
+        const size_t first = decision_positions.size();
+        find_decision_kmers(sequence, decision_positions, decision_hashes);
+        for (size_t i = first; i < decision_positions.size(); ++i) {
+            neighbor_pair_type nb;
+            get_decision_neighbors(sequence.substr(decision_positions[i], K), nb);
+            decision_neighbors.push_back(nb);
+        }
     }
 
     // unitig_walker.hh:1025-1045 (positions and hashes of the decision

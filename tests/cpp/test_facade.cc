@@ -88,6 +88,25 @@ int main(int argc, char** argv) {
     auto nb = g3->neighbors(path.substr(1, 21));
     CHECK(nb.first.size() == 1 && nb.second.size() == 1);
     CHECK(!g3->is_decision_kmer(path.substr(1, 21)));
+    // a right fork: two right neighbour k-mers of the fork k-mer
+    {
+        auto g9 = dBG<SparseppSetStorage, CanLemireShifter>::build(SparseppSetStorage::build(), 21);
+        const std::string core = seq.substr(0, 40);
+        const char a = seq[40], b = a == 'A' ? 'C' : 'A';
+        g9->insert_sequence(core + a + seq.substr(41, 20));
+        g9->insert_sequence(core + b + seq.substr(41, 20));
+        const std::string fork = core.substr(core.size() - 21);
+        dBG<SparseppSetStorage, CanLemireShifter>::neighbor_pair_type nbp;
+        CHECK(g9->get_decision_neighbors(fork, nbp));
+        CHECK(nbp.first.size() == 1 && nbp.second.size() == 2);
+        std::set<std::string> rk{nbp.second[0].kmer, nbp.second[1].kmer};
+        CHECK(rk.count(fork.substr(1) + a) && rk.count(fork.substr(1) + b));
+        std::vector<uint32_t> dpos;
+        std::vector<dBG<SparseppSetStorage, CanLemireShifter>::hash_type> dh;
+        std::vector<dBG<SparseppSetStorage, CanLemireShifter>::neighbor_pair_type> dn;
+        g9->find_decision_kmers(core + a + seq.substr(41, 20), dpos, dh, dn);
+        CHECK(dpos.size() == dn.size() && !dpos.empty());
+    }
 
     // errors
     bool threw = false;

@@ -176,6 +176,20 @@ def test_fork_degree(st):
     assert g.is_decision_kmer(fork)
     pos, hs, deg = g.find_decision_kmers(branch_a)
     assert len(core) - K in pos
+    # get_decision_neighbors (unitig_walker.hh:1085-1117): the two right
+    # neighbours are suffix(fork) + the two branch bases, as k-mers with
+    # their hashes; a k-mer inside the core is no decision k-mer
+    is_dec, (left, right) = g.get_decision_neighbors(fork)
+    assert is_dec
+    assert sorted(k.kmer for k in right) == sorted([fork[1:] + branch_a[40], fork[1:] + alt])
+    assert [k.kmer for k in left] == [core[-K - 1:-1]]
+    for k in left + right:
+        assert k.value == g.hash(k.kmer).value
+    assert g.get_decision_neighbors(core[5:5 + K]) == (False, ([], []))
+    pos2, hs2, deg2, nbs = g.find_decision_kmers(branch_a, neighbors=True)
+    assert pos2 == pos and len(nbs) == len(pos)
+    i = pos.index(len(core) - K)
+    assert sorted(k.kmer for k in nbs[i][1]) == sorted(k.kmer for k in right)
 
 
 def test_kmer_iterator():
