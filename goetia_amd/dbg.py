@@ -359,6 +359,21 @@ class dBG(metaclass=_dBGMeta):
     def get_hash(self):
         return self._shifter.get()
 
+    def shift_right(self, c):
+        """hashextender.hh:349-353: the cursor moves one base right (c
+        enters, the first base leaves); the new hash."""
+        cur = self.get_cursor()
+        h = self._shifter.shift_right(cur[0], c)
+        self._cursor = cur[1:] + c
+        return h
+
+    def shift_left(self, c):
+        """hashextender.hh:327-331: the cursor moves one base left."""
+        cur = self.get_cursor()
+        h = self._shifter.shift_left(c, cur[-1])
+        self._cursor = c + cur[:-1]
+        return h
+
     def _extensions(self, right):
         cur = self.get_cursor()
         sh = self._shifter

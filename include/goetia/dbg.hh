@@ -238,6 +238,21 @@ class dBG {
     }
     const std::string get_cursor() const { return _cursor; }
 
+    // hashextender.hh:327-353: move the cursor one base (the K-base ring
+    // drops its other end) and return the new k-mer's hash; get() is the
+    // cursor's hash.
+    hash_type shift_right(const char c) {
+        auto h = _cursor_shifter().shift_right(_cursor.front(), c);
+        _cursor = _cursor.substr(1) + c;
+        return h;
+    }
+    hash_type shift_left(const char c) {
+        auto h = _cursor_shifter().shift_left(c, _cursor.back());
+        _cursor = c + _cursor.substr(0, K - 1);
+        return h;
+    }
+    hash_type get() { return _cursor_shifter().get(); }
+
     std::vector<shift_type<DIR_LEFT>> left_extensions() {
         auto& sh = _cursor_shifter();
         std::vector<shift_type<DIR_LEFT>> out;

@@ -106,6 +106,13 @@ int main(int argc, char** argv) {
         std::vector<dBG<SparseppSetStorage, CanLemireShifter>::neighbor_pair_type> dn;
         g9->find_decision_kmers(core + a + seq.substr(41, 20), dpos, dh, dn);
         CHECK(dpos.size() == dn.size() && !dpos.empty());
+        // cursor walk (hashextender.hh:327-353)
+        g9->set_cursor(core.substr(0, 21));
+        for (size_t i = 1; i + 21 <= core.size(); ++i) {
+            CHECK(g9->shift_right(core[i + 20]).value() == g9->hash(core.substr(i, 21)).value());
+            CHECK(g9->get_cursor() == core.substr(i, 21));
+        }
+        CHECK(g9->shift_left(core[core.size() - 22]).value() == g9->hash(core.substr(core.size() - 22, 21)).value());
     }
 
     // errors

@@ -186,6 +186,12 @@ def test_fork_degree(st):
     for k in left + right:
         assert k.value == g.hash(k.kmer).value
     assert g.get_decision_neighbors(core[5:5 + K]) == (False, ([], []))
+    # the cursor walks the core (hashextender.hh:327-353)
+    g.set_cursor(core[:K])
+    for i in range(1, len(core) - K + 1):
+        assert g.shift_right(core[i + K - 1]).value == g.hash(core[i:i + K]).value
+        assert g.get_cursor() == core[i:i + K]
+    assert g.shift_left(core[len(core) - K - 1]).value == g.hash(core[-K - 1:-1]).value
     pos2, hs2, deg2, nbs = g.find_decision_kmers(branch_a, neighbors=True)
     assert pos2 == pos and len(nbs) == len(pos)
     i = pos.index(len(core) - K)
