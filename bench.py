@@ -64,6 +64,8 @@ def parse():
     p.add_argument('--no-iq', action='store_true', help='skip the one-pass insert_and_query timing')
     p.add_argument('--no-random-access', action='store_true',
                    help='skip the random-access roofline microbenchmark (4 GiB buffer)')
+    p.add_argument('--random-access-sweep', action='store_true',
+                   help='also time random reads over buffers of 16 MiB to 4 GiB')
     p.add_argument('--copy-calibration', action='store_true',
                    help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
     p.add_argument('--workload', default='cms_20m',
@@ -85,11 +87,17 @@ def kernel_times(N, names):
     return out
 
 
-def random_access(N, table_bytes, n_ops=1 << 30):
+SWEEP_MB = (16, 64, 256, 1024, 4096)
+
+
+def random_access(N, table_bytes, n_ops=1 << 30, sweep=False):
     """The device's random-access roofline: uniform random 32-bit reads,
     atomicOr and atomicCAS (saturating byte increment) over a 4 GiB buffer
     from every CU (gt_membench_random), and random reads over a buffer of the
-    tables' size; best of three runs each, in ops/s."""
+    tables' size; best of three runs each, in ops/s.  With `sweep`, random
+    reads over buffers of SWEEP_MB MiB as well (the working set at
+    which the rate stops depending on caches) and reads with the
+    non-temporal cache policy over 4 GiB."""
     import ctypes as C
 
     import torch
@@ -116,6 +124,9 @@ def random_access(N, table_bytes, n_ops=1 << 30):
     tw = max(1, min(n_words, table_bytes // 4))
     out['table_size_bytes'] = tw * 4
     out['read_ops_per_s_table_size'] = rate(tw, 0)
+    if sweep:
+        out['read_ops_per_s_by_mb'] = {mb: rate(min(n_words, mb * (1 << 20) // 4), 0) for mb in SWEEP_MB}
+        out['nontemporal_read_ops_per_s'] = rate(n_words, 3)
     del buf
     torch.cuda.empty_cache()
     return out
@@ -272,7 +283,8 @@ def main():
     # the query against the measured random-read roofline: one random table
     # byte per k-mer and table
     if roofline is not None and not args.no_random_access:
-        ra = random_access(N, sum(store.table_bytes(i) for i in range(store.n_tables())))
+        ra = random_access(N, sum(store.table_bytes(i) for i in range(store.n_tables())),
+                           sweep=args.random_access_sweep)
         q_ms = roofline['kernel_ms_per_step'].get('query')
         if q_ms:
             q_rate = n_kmers * store.n_tables() / (q_ms / 1e3)

@@ -31,6 +31,8 @@ __device__ __forceinline__ uint64_t mb_next(uint64_t& x) {
 // Word index in [0, n_words) from the high bits of a random word.
 __device__ __forceinline__ uint64_t mb_index(uint64_t r, uint64_t n_words) { return __umul64hi(r, n_words); }
 
+// kNonTemporal: loads with the non-temporal cache policy (global_load ... nt).
+template <bool kNonTemporal>
 __global__ __launch_bounds__(kMbBlock) void mb_read_kernel(const uint32_t* buf, uint64_t n_words, uint64_t rounds,
                                                            uint64_t seed, uint32_t* sink) {
     uint64_t x = seed ^ ((blockIdx.x * (uint64_t)kMbBlock + threadIdx.x) * 0xD1B54A32D192ED03ULL);
@@ -41,7 +43,12 @@ __global__ __launch_bounds__(kMbBlock) void mb_read_kernel(const uint32_t* buf, 
         for (int u = 0; u < kMbUnroll; ++u) idx[u] = mb_index(mb_next(x), n_words);
         uint32_t v[kMbUnroll];
 #pragma unroll
-        for (int u = 0; u < kMbUnroll; ++u) v[u] = buf[idx[u]];
+        for (int u = 0; u < kMbUnroll; ++u) {
+            if constexpr (kNonTemporal)
+                v[u] = __builtin_nontemporal_load(buf + idx[u]);
+            else
+                v[u] = buf[idx[u]];
+        }
 #pragma unroll
         for (int u = 0; u < kMbUnroll; ++u) acc += v[u];
     }
@@ -86,7 +93,8 @@ __global__ __launch_bounds__(kMbBlock) void mb_cas_kernel(uint32_t* buf, uint64_
 extern "C" int gt_membench_random(void* buf, uint64_t n_words, uint64_t n_ops, int kind, uint64_t seed,
                                   void* stream, uint64_t* ops_done) {
     if (!buf || n_words == 0 || !ops_done) return gt::set_error(GT_ERR_INVALID_ARG, "null buffer or empty range");
-    if (kind < 0 || kind > 2) return gt::set_error(GT_ERR_INVALID_ARG, "kind must be 0 (read), 1 (or), 2 (cas)");
+    if (kind < 0 || kind > 3)
+        return gt::set_error(GT_ERR_INVALID_ARG, "kind must be 0 (read), 1 (or), 2 (cas), 3 (non-temporal read)");
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                                                  hipSuccess)
@@ -97,7 +105,9 @@ extern "C" int gt_membench_random(void* buf, uint64_t n_words, uint64_t n_ops, i
     hipStream_t s = (hipStream_t)stream;
     uint32_t* b = static_cast<uint32_t*>(buf);
     if (kind == 0)
-        mb_read_kernel<<<(unsigned)blocks, kMbBlock, 0, s>>>(b, n_words, rounds, seed, b);
+        mb_read_kernel<false><<<(unsigned)blocks, kMbBlock, 0, s>>>(b, n_words, rounds, seed, b);
+    else if (kind == 3)
+        mb_read_kernel<true><<<(unsigned)blocks, kMbBlock, 0, s>>>(b, n_words, rounds, seed, b);
     else if (kind == 1)
         mb_or_kernel<<<(unsigned)blocks, kMbBlock, 0, s>>>(b, n_words, rounds, seed);
     else

@@ -330,7 +330,8 @@ int gt_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
 /* Random-access roofline (no counterpart in goetia; measurement only):
  * at least n_ops uniform random 32-bit accesses to the device buffer
  * buf[0, n_words) from every CU, asynchronously on `stream`: kind 0 reads,
- * 1 atomicOr of one bit, 2 atomicCAS saturating increment of the low byte.
+ * 1 atomicOr of one bit, 2 atomicCAS saturating increment of the low byte,
+ * 3 reads with the non-temporal cache policy.
  * *ops_done receives the number issued.  The buffer's contents change for
  * kinds 1 and 2. */
 int gt_membench_random(void* buf, uint64_t n_words, uint64_t n_ops, int kind, uint64_t seed, void* stream,

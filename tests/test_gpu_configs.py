@@ -192,3 +192,44 @@ def test_serving_results_same_alone_and_in_mix(slots, policy):
             np.testing.assert_array_equal(x, y)
         assert ctr == [alone[s].n_unique(), alone[s].n_occupied()]
     sv.close()
+
+
+def test_membench_kinds():
+    """The random-access microbenchmark: every kind issues at least the
+    requested accesses inside the buffer; reads leave it unchanged, atomicOr
+    only sets bits, the saturating CAS only raises low bytes (never past 255,
+    by at most the accesses issued), and a bad kind is refused."""
+    import ctypes as C
+
+    import torch
+    n_words = 1 << 16
+    buf = torch.zeros(n_words, dtype=torch.int32, device='cuda')
+    stream = torch.cuda.current_stream()
+
+    def run(kind, n_ops):
+        done = C.c_uint64()
+        N.call('gt_membench_random', C.c_void_p(buf.data_ptr()), n_words, n_ops, kind, 5,
+               C.c_void_p(stream.cuda_stream), C.byref(done))
+        torch.cuda.synchronize()
+        assert done.value >= n_ops
+        return done.value
+
+    buf.fill_(0x5A5A5A00)
+    before = buf.clone()
+    for kind in (0, 3):
+        run(kind, 1 << 20)
+        assert torch.equal(buf, before)
+    buf.zero_()
+    run(1, 1 << 18)
+    v = buf.cpu().numpy().view(np.uint32)
+    assert v.any()
+    buf.fill_(0x11223300)
+    done = run(2, 1 << 18)
+    w = buf.cpu().numpy().view(np.uint32)
+    assert ((w & 0xFFFFFF00) == 0x11223300).all()
+    assert 0 < int((w & 0xFF).sum()) <= done
+    buf.fill_(0x000000FF)
+    run(2, 1 << 16)
+    assert (buf.cpu().numpy().view(np.uint32) == 0xFF).all()
+    with pytest.raises(ValueError):
+        run(4, 1)
