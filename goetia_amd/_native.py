@@ -119,6 +119,8 @@ SIGNATURES = {
     'gt_dbg_insert_and_query': [st_p, u16, cint, cint, vp, vp, u64, vp],
     'gt_dbg_degrees_dev': [st_p, u16, cint, cint, cint, vp, vp, u64, u64, vp, vp, vp, vp],
     'gt_dbg_degrees': [st_p, u16, cint, cint, cint, vp, vp, u64, vp, vp, vp],
+    'gt_dbg_neighbors_dev': [st_p, u16, cint, cint, cint, vp, vp, u64, u64, vp, vp, vp, vp, vp],
+    'gt_dbg_neighbors': [st_p, u16, cint, cint, cint, vp, vp, u64, vp, vp, vp, vp],
     'gt_insert_hashes_dev': [st_p, vp, u64, vp, vp],
     'gt_insert_hashes': [st_p, vp, u64, vp],
     'gt_query_hashes_dev': [st_p, vp, u64, vp, vp],

@@ -1563,9 +1563,11 @@ int gt_diginorm(gt_storage* st, uint16_t K, int canonical, int mode, const uint8
     return check_kernel_errors(st, s);
 }
 
-int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
-                       const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, const uint8_t* only,
-                       uint8_t* in_deg, uint8_t* out_deg, void* stream) {
+namespace {
+
+int degrees_impl(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                 const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, const uint8_t* only, uint8_t* in_deg,
+                 uint8_t* out_deg, uint64_t* nb_val, uint8_t* nb_mask, void* stream) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     if (!in_deg || !out_deg) return fail(GT_ERR_INVALID_ARG, "null degree outputs");
     if (min_count < 1 || min_count > 255) return fail(GT_ERR_INVALID_ARG, "min_count must be in 1..255");
@@ -1579,9 +1581,56 @@ int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int 
     {
         ProfScope ps("degrees", s);
         launch_degrees(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), min_count, only, in_deg,
-                       out_deg);
+                       out_deg, nb_val, nb_mask);
     }
     GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+}  // namespace
+
+int gt_dbg_degrees_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                       const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, const uint8_t* only,
+                       uint8_t* in_deg, uint8_t* out_deg, void* stream) {
+    return degrees_impl(st, K, canonical, mode, min_count, seq, offsets, n_reads, n_bases, only, in_deg, out_deg,
+                        nullptr, nullptr, stream);
+}
+
+int gt_dbg_neighbors_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                         const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint8_t* in_deg,
+                         uint8_t* out_deg, uint64_t* nb_values, uint8_t* nb_present, void* stream) {
+    if (!nb_values || !nb_present) return fail(GT_ERR_INVALID_ARG, "null neighbour outputs");
+    return degrees_impl(st, K, canonical, mode, min_count, seq, offsets, n_reads, n_bases, nullptr, in_deg, out_deg,
+                        nb_values, nb_present, stream);
+}
+
+int gt_dbg_neighbors(gt_storage* st, uint16_t K, int canonical, int mode, int min_count, const uint8_t* seq,
+                     const int64_t* offsets, uint64_t n_reads, uint8_t* in_deg, uint8_t* out_deg,
+                     uint64_t* nb_values, uint8_t* nb_present) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (!in_deg || !out_deg || !nb_values || !nb_present) return fail(GT_ERR_INVALID_ARG, "null outputs");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    if (nk == 0) return GT_OK;
+    DevBuf dseq, doff, din, dout, dval, dpres;
+    hipStream_t s = nullptr;
+    GT_TRY(stage_in(dseq, seq, n_bases, s));
+    GT_TRY(stage_in(doff, offsets, n_reads + 1, s));
+    GT_TRY(din.ensure(nk));
+    GT_TRY(dout.ensure(nk));
+    GT_TRY(dval.ensure(nk * 8 * sizeof(uint64_t)));
+    GT_TRY(dpres.ensure(nk));
+    GT_TRY(gt_dbg_neighbors_dev(st, K, canonical, mode, min_count, dseq.as<uint8_t>(), doff.as<int64_t>(), n_reads,
+                                n_bases, din.as<uint8_t>(), dout.as<uint8_t>(), dval.as<uint64_t>(),
+                                dpres.as<uint8_t>(), s));
+    GT_TRY(stage_out(in_deg, din, nk, s));
+    GT_TRY(stage_out(out_deg, dout, nk, s));
+    GT_TRY(stage_out(nb_values, dval, nk * 8, s));
+    GT_TRY(stage_out(nb_present, dpres, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
     return GT_OK;
 }
 

@@ -652,6 +652,8 @@ struct DegreeOp {
     const uint8_t* only;
     uint8_t* in_deg;
     uint8_t* out_deg;
+    uint64_t* nb_val;       // optional: 8 neighbour values per window (left A C G T, right A C G T)
+    uint8_t* nb_mask;       // optional: bit c left c present, bit 4 + c right c present
     const uint8_t* bytes;   // LDS tile
     const uint64_t* tf;
     const uint64_t* tr;
@@ -662,13 +664,18 @@ struct DegreeOp {
         if (!valid || (only && !st->only[i])) {
             st->in[i] = 0xFF;
             st->out[i] = 0xFF;
+            if (nb_mask) {
+                nb_mask[k0 + i] = 0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) nb_val[(k0 + i) * 8 + c] = 0;
+            }
             return;
         }
         const int K = R.K;
         const unsigned Kr = (unsigned)K & 63u;
         const uint8_t first = bytes[s - t0];
         const uint8_t last = bytes[s - t0 + K - 1];
-        uint8_t nin = 0, nout = 0;
+        uint8_t nin = 0, nout = 0, mask = 0;
         const uint64_t rf = gt_rotl(fw, 1) ^ gt_rotl(tf[first], Kr);
         const uint64_t lr = gt_rotl(rc, 1) ^ gt_rotl(tr[last], Kr);
 #pragma unroll
@@ -677,14 +684,24 @@ struct DegreeOp {
             // right extension: shift_right(first, x)
             const uint64_t F1 = rf ^ tf[x];
             const uint64_t R1 = gt_rotr(rc ^ gt_rotl(tr[x], Kr) ^ tr[first], 1);
-            nout += storage_query(T, E, kmer_value(F1, R1, R.canonical)) >= min_count;
+            const uint64_t v1 = kmer_value(F1, R1, R.canonical);
+            const bool p1 = storage_query(T, E, v1) >= min_count;
+            nout += p1;
             // left extension: shift_left(x, last)
             const uint64_t F2 = gt_rotr(fw ^ gt_rotl(tf[x], Kr) ^ tf[last], 1);
             const uint64_t R2 = lr ^ tr[x];
-            nin += storage_query(T, E, kmer_value(F2, R2, R.canonical)) >= min_count;
+            const uint64_t v2 = kmer_value(F2, R2, R.canonical);
+            const bool p2 = storage_query(T, E, v2) >= min_count;
+            nin += p2;
+            if (nb_val) {
+                nb_val[(k0 + i) * 8 + c] = v2;
+                nb_val[(k0 + i) * 8 + 4 + c] = v1;
+            }
+            mask |= (uint8_t)((p2 ? 1u : 0u) << c) | (uint8_t)((p1 ? 1u : 0u) << (4 + c));
         }
         st->in[i] = nin;
         st->out[i] = nout;
+        if (nb_mask) nb_mask[k0 + i] = mask;
     }
     __device__ void begin_tile(int64_t tile_base) {
         t0 = tile_base;
@@ -782,10 +799,12 @@ __global__ __launch_bounds__(kBlock) void hash_kernel(Reads R, uint64_t* fw, uin
 
 template <int PER>
 __global__ __launch_bounds__(kBlock) void degree_kernel(Reads R, Tables T, ExactSet E, int min_count,
-                                                      const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
+                                                      const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg,
+                                                      uint64_t* nb_val, uint8_t* nb_mask) {
     __shared__ TileSharedT<kBlock * PER> sh;
     __shared__ DegreeStage<kBlock * PER> st;
-    DegreeOp<kBlock * PER> op{T, E, R, kBlock * PER, min_count, only, in_deg, out_deg, sh.bytes, sh.tf, sh.tr, &st};
+    DegreeOp<kBlock * PER> op{T,       E,       R,        kBlock * PER, min_count, only, in_deg,
+                              out_deg, nb_val,  nb_mask,  sh.bytes,     sh.tf,     sh.tr, &st};
     walk_tiles<PER>(R, sh, op);
 }
 
@@ -2148,8 +2167,8 @@ void launch_hash(const Launch& L, const Reads& R, uint64_t* fw, uint64_t* rc, ui
 }
 
 void launch_degrees(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int min_count,
-                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg) {
-    GT_TILE_LAUNCH(L, degree_kernel, R, T, E, min_count, only, in_deg, out_deg);
+                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg, uint64_t* nb_val, uint8_t* nb_mask) {
+    GT_TILE_LAUNCH(L, degree_kernel, R, T, E, min_count, only, in_deg, out_deg, nb_val, nb_mask);
 }
 
 void launch_hash_claim(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,

@@ -32,7 +32,7 @@ void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactS
                   uint64_t* hashes);
 void launch_hash(const Launch& L, const Reads& R, uint64_t* fw, uint64_t* rc, uint8_t* valid);
 void launch_degrees(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int min_count,
-                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg);
+                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg, uint64_t* nb_val, uint8_t* nb_mask);
 
 void launch_hash_claim(const Launch& L, const uint64_t* h, uint64_t n, const Tables& T, const ClaimMap& cm,
                        uint8_t* fresh, uint64_t* counters);

@@ -322,6 +322,23 @@ class dBG(metaclass=_dBGMeta):
                N.ptr(only_arr), N.ptr(ind), N.ptr(outd))
         return ind[:nk], outd[:nk]
 
+    def window_neighbors(self, reads, min_count=1):
+        """(in degree, out degree, neighbour values, presence mask) of every
+        window: values[i, c] is the left neighbour with symbol "ACGT"[c],
+        values[i, 4 + c] the right one (left_extensions / right_extensions,
+        hashextender.hh:352-364, 386-396); bit c / 4 + c of mask[i] is set
+        when its query is >= min_count."""
+        batch = _as_batch(reads)
+        nk = int(batch.kmer_counts(self.K).sum())
+        ind = np.zeros(max(nk, 1), np.uint8)
+        outd = np.zeros(max(nk, 1), np.uint8)
+        vals = np.zeros((max(nk, 1), 8), np.uint64)
+        mask = np.zeros(max(nk, 1), np.uint8)
+        N.call('gt_dbg_neighbors', self.S.handle, self.K, int(self.canonical), self.mode,
+               int(min_count), N.ptr(batch.seq), N.ptr(batch.offsets), batch.n_reads,
+               N.ptr(ind), N.ptr(outd), N.ptr(vals), N.ptr(mask))
+        return ind[:nk], outd[:nk], vals[:nk], mask[:nk]
+
     # device-resident batches (torch tensors; asynchronous on `stream`)
     def insert_device(self, reads, n_new=None, new_flags=None, stream=None):
         N.call('gt_dbg_insert_dev', self.S.handle, self.K, int(self.canonical), self.mode,

@@ -1,7 +1,8 @@
 """The goetia names of the dBG hot path, as ``goetia.libgoetia`` exposes them
 through cppyy (interface.hh / goetia.map): storages, shifters, hash models,
-KmerIterator and dBG[Storage, Shifter], and around it FastxParser and the
-InserterProcessor chunk driver."""
+KmerIterator and dBG[Storage, Shifter], and around it FastxParser, the
+InserterProcessor chunk driver, the filters and the compactor's segment
+finder."""
 from .dbg import Kmer, Shift, dBG
 from .hashing import (CanLemireShifter, Canonical, DNA_SIMPLE, FwdLemireShifter, Hash,
                       KmerIteratorT as KmerIterator)
@@ -12,6 +13,7 @@ from ._native import (GoetiaException, GoetiaFileException, InvalidCharacterExce
 from .parsing import FastxParser, NoMoreReadsAvailable, Record
 from .processors import FilterProcessor, InserterProcessor, IntervalCounter
 from .filters import DiginormFilter, StreamingSolidFilter
+from .cdbg import StreamingCompactor
 
 __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Canonical', 'Hash',
            'DNA_SIMPLE', 'KmerIterator', 'BitStorage', 'ByteStorage', 'SparseppSetStorage',
@@ -20,4 +22,4 @@ __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Cano
            'UninitializedShifterException', 'GoetiaFileException', 'InvalidCharacterException',
            'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter',
            'FilterProcessor',
-           'StreamingSolidFilter', 'DiginormFilter']
+           'StreamingSolidFilter', 'DiginormFilter', 'StreamingCompactor']

@@ -207,6 +207,24 @@ int gt_dbg_degrees(gt_storage* st, uint16_t K, int canonical, int mode, int min_
                    const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
                    const uint8_t* only, uint8_t* in_deg, uint8_t* out_deg);
 
+/* gt_dbg_degrees plus every window's eight neighbours, as the walker's
+ * left_extensions / right_extensions give them (hashextender.hh:352-364,
+ * 386-396): nb_values[8 i + c] is the value of the left neighbour with
+ * symbol "ACGT"[c] of window i, nb_values[8 i + 4 + c] the right one;
+ * bit c / 4 + c of nb_present[i] is set when that neighbour's query is >=
+ * min_count (filter_nodes, unitig_walker.hh:407-418).  Callers that add
+ * nodes of their own to the graph (filter_nodes with extras, :457-468, as
+ * StreamingCompactor::find_new_segments does, compactor.hh:346-534) test the
+ * values against them.  Skipped windows get zero values and mask. */
+int gt_dbg_neighbors_dev(gt_storage* st, uint16_t K, int canonical, int mode, int min_count,
+                         const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                         uint64_t n_bases, uint8_t* in_deg, uint8_t* out_deg,
+                         uint64_t* nb_values, uint8_t* nb_present, void* stream);
+int gt_dbg_neighbors(gt_storage* st, uint16_t K, int canonical, int mode, int min_count,
+                     const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                     uint8_t* in_deg, uint8_t* out_deg, uint64_t* nb_values,
+                     uint8_t* nb_present);
+
 /* ------------------------------------------------------------ per-hash */
 
 /* dBG::insert(hash) / Storage::insert (dbg.hh:140-146, storage.hh:120)

@@ -13,6 +13,7 @@
 #include "goetia/hashing/kmeriterator.hh"
 #include "goetia/parsing/readers.hh"
 #include "goetia/diginorm.hh"
+#include "goetia/cdbg/compactor.hh"
 
 #include <cstdint>
 #include <cstring>
@@ -102,6 +103,73 @@ uint64_t diginorm_run(void* graph, unsigned cutoff, const char* seq, const int64
         }
     }
     return n;
+}
+
+// StreamingCompactor<G>::Compactor::find_new_segments (compactor.hh:321-534)
+// over reads in order; see ref_find_new_segments.
+template <class G>
+uint64_t segments_run(std::shared_ptr<G> g, const char* seq, const int64_t* off, uint64_t n_reads, int persist,
+                      int insert_after, uint64_t* out, uint64_t cap, char* kmers, uint64_t kcap, uint64_t* kused) {
+    using C = typename StreamingCompactor<G>::Compactor;
+    using hash_type = typename G::hash_type;
+    C cp(g);
+    uint64_t w = 0, kb = 0;
+    auto put = [&](uint64_t v) {
+        if (w < cap) out[w] = v;
+        ++w;
+    };
+    auto put_kmer = [&](const std::string& k) {
+        for (char c : k) {
+            if (kb < kcap) kmers[kb] = c;
+            ++kb;
+        }
+    };
+    std::set<hash_type> shared_new;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        std::string s(seq + off[r], (size_t)(off[r + 1] - off[r]));
+        std::vector<hash_type> hashes;
+        std::set<hash_type> fresh_new, dec;
+        std::set<hash_type>& nk = persist ? shared_new : fresh_new;
+        std::vector<typename StreamingCompactor<G>::compact_segment> segs;
+        std::deque<typename StreamingCompactor<G>::neighbor_pair_type> pairs;
+        try {
+            cp.find_new_segments(s, hashes, nk, segs, dec, pairs);
+        } catch (SequenceLengthException&) {
+            put(~0ULL);
+            continue;
+        }
+        put(segs.size());
+        for (auto& sg : segs) {
+            put(sg.left_anchor.value());
+            put(sg.right_anchor.value());
+            put(sg.left_flank.value());
+            put(sg.right_flank.value());
+            put(sg.is_decision_kmer ? 1 : 0);
+            put(sg.start_pos);
+            put(sg.length);
+        }
+        put(nk.size());
+        if (!persist || r + 1 == n_reads)
+            for (auto& h : nk) put(h.value());
+        put(dec.size());
+        for (auto& h : dec) put(h.value());
+        put(pairs.size());
+        for (auto& pr : pairs) {
+            put(pr.first.size());
+            put(pr.second.size());
+            for (auto& k : pr.first) {
+                put(k.value());
+                put_kmer(k.kmer);
+            }
+            for (auto& k : pr.second) {
+                put(k.value());
+                put_kmer(k.kmer);
+            }
+        }
+        if (insert_after && s.size() >= (size_t)g->K) g->insert_sequence(s);
+    }
+    *kused = kb;
+    return w;
 }
 
 }  // namespace
@@ -316,6 +384,33 @@ uint64_t ref_diginorm_reads(void* hv, unsigned cutoff, const char* seq, const in
     if (h->canonical) return diginorm_run<Graph<ByteStorage, CanLemireShifter>>(h->graph, cutoff, seq, off, n_reads,
                                                                              passed);
     return diginorm_run<Graph<ByteStorage, FwdLemireShifter>>(h->graph, cutoff, seq, off, n_reads, passed);
+}
+
+// find_new_segments over reads in order.  persist != 0: one new-k-mer set
+// shared by all the reads (the six-argument overload with a caller-owned
+// set); otherwise a fresh set per read (the one-argument overload).
+// insert_after != 0: insert_sequence(read) after its segments, as goetia's
+// tests do.  Per read, into `out` (u64 words): the segment count (~0 when the
+// read is shorter than K), 7 words per segment (left_anchor, right_anchor,
+// left_flank, right_flank, is_decision_kmer, start_pos, length; hashes as
+// value()), the new-k-mer set (count, values in set order; with persist, the
+// values after the last read only), the new decision
+// k-mers (count, values), the decision neighbour pairs (count; per pair the
+// left and right counts, then each k-mer's value); the neighbour k-mers'
+// strings go to `kmers`, K bytes each, in the same order.  Returns the words
+// needed; *kused the bytes.  Forward (non-canonical) graphs only: goetia's
+// find_new_segments does not compile for Canonical hashes.
+uint64_t ref_find_new_segments(void* hv, const char* seq, const int64_t* off, uint64_t n_reads, int persist,
+                               int insert_after, uint64_t* out, uint64_t cap, char* kmers, uint64_t kcap,
+                               uint64_t* kused) {
+    Handle* h = (Handle*)hv;
+    auto run = [&](auto* holder) {
+        return segments_run(holder->g, seq, off, n_reads, persist, insert_after, out, cap, kmers, kcap, kused);
+    };
+    if (h->canonical) return ~0ULL;   // compactor.hh:405,428 assign a value_type to the hash: Hash<> only
+    if (h->kind == 1) return run(static_cast<Holder<Graph<BitStorage, FwdLemireShifter>>*>(h->graph));
+    if (h->kind == 2) return run(static_cast<Holder<Graph<ByteStorage, FwdLemireShifter>>*>(h->graph));
+    return run(static_cast<Holder<Graph<SparseppSetStorage, FwdLemireShifter>>*>(h->graph));
 }
 
 void ref_free(void* hv) {

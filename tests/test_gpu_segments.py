@@ -1,0 +1,110 @@
+"""StreamingCompactor::find_new_segments over the GPU dBG against what
+goetia's compiled compactor gave on the same reads
+(tests/golden/segments_ref.npz): per read, in order, with the graph growing
+between reads as goetia's tests grow it, and over a whole batch against a
+fixed graph.  The neighbour kernel's values and presence bits against the
+oracle's left/right extensions of every window."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from segments_golden import CASES, SHORT, case, record  # noqa: E402
+
+from goetia_amd import _native as N  # noqa: E402
+from goetia_amd import libgoetia as lg  # noqa: E402
+from goetia_amd.cdbg import SegmentList, StreamingCompactor  # noqa: E402
+
+STORAGE = {1: lg.BitStorage, 2: lg.ByteStorage, 3: lg.SparseppSetStorage}
+
+
+def graph_for(c):
+    S = STORAGE[c['kind']]
+    store = S.build() if c['kind'] == 3 else S.build(c['max_table'], c['n_tables'])
+    G = lg.dBG[S, lg.FwdLemireShifter]
+    return G, G(store, c['K'])
+
+
+@pytest.mark.parametrize('name', CASES)
+def test_find_new_segments_vs_reference(name):
+    c = case(name)
+    K = c['K']
+    G, g = graph_for(c)
+    if c['bg']:
+        g.insert_sequences(c['bg'])
+    comp = StreamingCompactor[G].Compactor.build(g)
+    shared = set()
+    n = len(c['reads'])
+    for i, (s, exp) in enumerate(zip(c['reads'], c['expected'])):
+        if exp == SHORT:
+            with pytest.raises(N.SequenceLengthException):
+                comp.find_new_segments(s)
+            continue
+        hashes, segs, dks, pairs = [], SegmentList(), set(), []
+        nk = shared if c['persist'] else set()
+        comp.find_new_segments(s, hashes, nk, segs, dks, pairs)
+        assert record(segs, nk, dks, pairs, with_new=not c['persist'] or i + 1 == n) == exp, (name, i, s)
+        if c['insert_after']:
+            g.insert_sequence(s)
+
+
+@pytest.mark.parametrize('name', [n for n in CASES if n.startswith('fixed_')])
+def test_find_new_segments_batch_vs_reference(name):
+    """Two GPU launches for all the reads of a batch (the graph does not
+    change between them)."""
+    c = case(name)
+    G, g = graph_for(c)
+    g.insert_sequences(c['bg'])
+    comp = StreamingCompactor[G].Compactor.build(g)
+    shared = set() if c['persist'] else None
+    got = comp.find_new_segments_batch(c['reads'], new_kmers=shared)
+    n = len(c['reads'])
+    for i, (res, exp) in enumerate(zip(got, c['expected'])):
+        if exp == SHORT:
+            assert res is None
+            continue
+        segs, hashes, nk, dks, pairs = res
+        got = record(segs, nk, dks, pairs, with_new=not c['persist'] or i + 1 == n)
+        if c['persist'] and i + 1 < n:
+            # the shared set is read after the whole batch; the reference's
+            # size after each read is checked by the per-read test
+            got, exp = got[:1] + got[2:], exp[:1] + exp[2:]
+        assert got == exp, (name, i)
+
+
+@pytest.mark.parametrize('canonical', [False, True])
+@pytest.mark.parametrize('min_count', [1, 2])
+def test_window_neighbors_vs_oracle(oracle, canonical, min_count):
+    rng = np.random.default_rng(5 + min_count)
+    genome = ''.join(rng.choice(list('ACGT'), 3000))
+    reads = [genome[s:s + int(rng.integers(20, 160))] for s in rng.integers(0, 2800, 400)]
+    reads += ['ACGT' * 10, 'A' * 40, genome[:30] + 'N' + genome[31:60]]
+    K = 21
+    sizes = oracle.primes_near_x(4, 20_000)
+    sh = lg.CanLemireShifter if canonical else lg.FwdLemireShifter
+    g = lg.dBG[lg.ByteStorage, sh](lg.ByteStorage(sizes), K)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    g.insert_sequences(reads[:300])
+    for r in reads[:300]:
+        if len(r) >= K:
+            o.insert_sequence(r, K, canonical)
+    ind, outd, vals, mask = g.window_neighbors(reads, min_count=min_count)
+    k = 0
+    for r in reads:
+        for i in range(len(r) - K + 1):
+            oi, oo, left, right = o.kmer_degrees(r[i:i + K], K, canonical, min_count)
+            exp = list(left) + list(right)
+            assert [int(v) for v in vals[k]] == exp, (r, i)
+            bits = sum(1 << c for c, v in enumerate(exp) if o.query(v) >= min_count)
+            assert int(mask[k]) == bits
+            assert (int(ind[k]), int(outd[k])) == (oi, oo)
+            k += 1
+    assert k == len(vals)
+
+
+def test_canonical_graphs_refused():
+    G = lg.dBG[lg.SparseppSetStorage, lg.CanLemireShifter]
+    with pytest.raises(TypeError):
+        StreamingCompactor[G]
+    with pytest.raises(TypeError):
+        StreamingCompactor.Compactor.build(G(lg.SparseppSetStorage.build(), 21))

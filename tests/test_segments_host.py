@@ -1,0 +1,82 @@
+"""find_new_segments' host side (goetia_amd.cdbg.segments_from_view) on the
+CPU, given graph views computed by the oracle (query counts, hashes, the
+eight neighbours of every window and their presence): every record equals
+what goetia's compiled StreamingCompactor::find_new_segments gave on the
+same reads (tests/golden/segments_ref.npz) -- forks, merges, decision k-mers
+at read ends, bubbles, cycles, seeded reads with repeats, SNPs, errors, N and
+reads shorter than K; Bloom, count-min and exact-set graphs; one new-k-mer
+set per read or one shared set."""
+import numpy as np
+import pytest
+
+from segments_golden import CASES, SHORT, case, record
+
+from goetia_amd.cdbg import SegmentList, segments_from_view
+
+
+def oracle_view(o, s, K):
+    counts, hv = o.query_sequence(s, K, False)
+    n = len(counts)
+    vals = np.zeros((n, 8), np.uint64)
+    mask = np.zeros(n, np.uint8)
+    for i in range(n):
+        _, _, left, right = o.kmer_degrees(s[i:i + K], K, False)
+        for c, v in enumerate(list(left) + list(right)):
+            vals[i, c] = v
+            if o.query(v) != 0:
+                mask[i] |= 1 << c
+    return counts, hv, vals, mask
+
+
+def oracle_storage(oracle, c):
+    if c['kind'] == 3:
+        return oracle.Storage(oracle.EXACT)
+    return oracle.Storage.build(c['kind'], c['max_table'], c['n_tables'])
+
+
+@pytest.mark.parametrize('name', CASES)
+def test_host_segments_vs_reference(oracle, name):
+    c = case(name)
+    K = c['K']
+    o = oracle_storage(oracle, c)
+    for r in c['bg']:
+        if len(r) >= K:
+            o.insert_sequence(r, K, False)
+    shared = set()
+    n = len(c['reads'])
+    for i, (s, exp) in enumerate(zip(c['reads'], c['expected'])):
+        if len(s) < K:
+            assert exp == SHORT
+            continue
+        counts, hv, vals, mask = oracle_view(o, s, K)
+        hashes, segs, dks, pairs = [], SegmentList(), set(), []
+        nk = shared if c['persist'] else set()
+        segments_from_view(K, s, counts, hv, vals, mask, hashes, nk, segs, dks, pairs)
+        got = record(segs, nk, dks, pairs, with_new=not c['persist'] or i + 1 == n)
+        assert got == exp, (name, i, s)
+        assert [h.value for h in hashes] == [int(v) for v in hv]
+        if c['insert_after']:
+            o.insert_sequence(s, K, False)
+
+
+def test_one_argument_form_and_conveniences(oracle):
+    """The one-argument overload returns the segment vector alone; the
+    pythonized segment gives NULL, start and sequence() (pythonize_cdbg.py);
+    the cut around the decision k-mer is the one goetia's
+    test_right_decision_split expects (test_cdbg.py:105-131; the fork's
+    pivot is 50)."""
+    c = case('fork_right_decision_split_set_k15')
+    K = c['K']
+    o = oracle_storage(oracle, c)
+    branch, core = c['reads']
+    o.insert_sequence(branch, K, False)
+    segs = segments_from_view(K, core, *oracle_view(o, core, K))
+    assert segs.front().NULL and segs.back().NULL
+    assert [s.is_decision_kmer for s in segs[1:-1]] == [False, True, False]
+    d = segs[2]
+    assert len(d.sequence(core)) == K and d.left_anchor == d.right_anchor
+    pivot = 50
+    assert len(segs) == 5
+    assert segs[1].sequence(core) == core[:pivot + K - 1]
+    assert segs[3].sequence(core) == core[pivot + 1:]
+    assert d.sequence(core) == core[pivot:pivot + K]
