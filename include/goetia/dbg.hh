@@ -230,6 +230,33 @@ class dBG {
                       "degrees");
     }
 
+    // query_sequence over every read with the window hashes (value()).
+    std::vector<count_t> query_sequences(const ReadBatch& reads, std::vector<uint64_t>& hashes) {
+        std::vector<count_t> counts(reads.n_kmers(K));
+        hashes.assign(counts.size(), 0);
+        detail::check(gt_dbg_query(S->handle(), K, canonical(), _mode, reads.data(), reads.offsets.data(),
+                                   reads.size(), counts.data(), hashes.data()),
+                      "query_sequences");
+        return counts;
+    }
+
+    // Degrees plus every window's eight neighbours: values[8 i + c] the left
+    // neighbour with symbol "ACGT"[c], values[8 i + 4 + c] the right one;
+    // bit c / 4 + c of present[i] set when that neighbour's query is >=
+    // min_count (left_extensions / right_extensions, hashextender.hh:352-396).
+    void window_neighbors(const ReadBatch& reads, std::vector<uint8_t>& in_deg, std::vector<uint8_t>& out_deg,
+                          std::vector<uint64_t>& values, std::vector<uint8_t>& present, int min_count = 1) {
+        const uint64_t nk = reads.n_kmers(K);
+        in_deg.assign(nk, 0);
+        out_deg.assign(nk, 0);
+        values.assign(nk * 8, 0);
+        present.assign(nk, 0);
+        detail::check(gt_dbg_neighbors(S->handle(), K, canonical(), _mode, min_count, reads.data(),
+                                       reads.offsets.data(), reads.size(), in_deg.data(), out_deg.data(),
+                                       values.data(), present.data()),
+                      "window_neighbors");
+    }
+
     // ------------------------------------------------------------ neighbours
     hash_type set_cursor(const std::string& kmer) {
         if (kmer.length() < K) throw InvalidSequenceException("Sequence must at least length K");
@@ -294,6 +321,28 @@ class dBG {
         return out;
     }
 
+    // unitig_walker.hh:386-396, 457-468: present iff query != 0 or the
+    // node is among `extras` (nodes about to join the graph).
+    template <bool Dir>
+    std::vector<shift_type<Dir>> filter_nodes(const std::vector<shift_type<Dir>>& exts,
+                                              const std::set<hash_type>& extras) {
+        std::vector<uint64_t> hs;
+        for (const auto& e : exts) hs.push_back(e.value());
+        std::vector<count_t> c(hs.size());
+        detail::check(gt_query_hashes(S->handle(), hs.data(), hs.size(), c.data()), "filter_nodes");
+        std::vector<shift_type<Dir>> out;
+        for (size_t i = 0; i < exts.size(); ++i) {
+            if (c[i] != 0 || extras.count(exts[i].hash)) out.push_back(exts[i]);
+        }
+        return out;
+    }
+    template <bool Dir>
+    size_t count_nodes(const std::vector<shift_type<Dir>>& exts, const std::set<hash_type>& extras) {
+        return filter_nodes(exts, extras).size();
+    }
+    size_t in_degree(const std::set<hash_type>& extras) { return count_nodes(left_extensions(), extras); }
+    size_t out_degree(const std::set<hash_type>& extras) { return count_nodes(right_extensions(), extras); }
+
     size_t in_degree() { return count_nodes(left_extensions()); }
     size_t out_degree() { return count_nodes(right_extensions()); }
     size_t degree() { return in_degree() + out_degree(); }
@@ -346,6 +395,21 @@ class dBG {
         const std::string r = get_cursor();
         auto left = build_left_kmers(filter_nodes(left_extensions()), r);
         auto right = build_right_kmers(filter_nodes(right_extensions()), r);
+        if (left.size() > 1 || right.size() > 1) {
+            result = std::make_pair(left, right);
+            return true;
+        }
+        return false;
+    }
+
+    // unitig_walker.hh:1047-1082: the same with union_nodes counted as
+    // present.
+    bool get_decision_neighbors(const std::string& root, neighbor_pair_type& result,
+                                const std::set<hash_type>& union_nodes) {
+        set_cursor(root);
+        const std::string r = get_cursor();
+        auto left = build_left_kmers(filter_nodes(left_extensions(), union_nodes), r);
+        auto right = build_right_kmers(filter_nodes(right_extensions(), union_nodes), r);
         if (left.size() > 1 || right.size() > 1) {
             result = std::make_pair(left, right);
             return true;
