@@ -412,24 +412,28 @@ class dBG(metaclass=_dBGMeta):
     def right_extensions(self):
         return self._extensions(True)
 
-    def _present(self, exts, min_count=1):
+    def _present(self, exts, min_count=1, extras=None):
         counts = self.S.query_hashes([e.value for e in exts])
-        return [e for e, c in zip(exts, counts) if c >= min_count]
+        known = () if extras is None else {x.value if hasattr(x, 'value') else int(x) for x in extras}
+        return [e for e, c in zip(exts, counts) if c >= min_count or e.value in known]
 
-    def count_nodes(self, exts):
-        return len(self._present(exts))
+    def count_nodes(self, exts, extras=None):
+        """unitig_walker.hh:361-397; with extras, a node is also present when
+        it is among them."""
+        return len(self._present(exts, extras=extras))
 
-    def filter_nodes(self, exts):
-        return self._present(exts)
+    def filter_nodes(self, exts, extras=None):
+        """unitig_walker.hh:407-418, 457-468."""
+        return self._present(exts, extras=extras)
 
-    def in_degree(self):
-        return self.count_nodes(self.left_extensions())
+    def in_degree(self, extras=None):
+        return self.count_nodes(self.left_extensions(), extras)
 
-    def out_degree(self):
-        return self.count_nodes(self.right_extensions())
+    def out_degree(self, extras=None):
+        return self.count_nodes(self.right_extensions(), extras)
 
-    def degree(self):
-        return self.in_degree() + self.out_degree()
+    def degree(self, extras=None):
+        return self.in_degree(extras) + self.out_degree(extras)
 
     def left_degree(self, kmer):
         """pythonize_dbg.py:30-32."""
@@ -472,15 +476,16 @@ class dBG(metaclass=_dBGMeta):
         suffix = self.suffix(root)
         return [Kmer(n.hash, suffix + n.symbol) for n in nodes]
 
-    def get_decision_neighbors(self, root=None):
-        """unitig_walker.hh:1085-1117: (is decision, (left k-mers, right
-        k-mers)) of the cursor (or root); the pair is empty unless it is a
-        decision k-mer, as the reference leaves `result` untouched then."""
+    def get_decision_neighbors(self, root=None, union_nodes=None):
+        """unitig_walker.hh:1047-1117: (is decision, (left k-mers, right
+        k-mers)) of the cursor (or root), with union_nodes counted as present
+        when given; the pair is empty unless it is a decision k-mer, as the
+        reference leaves `result` untouched then."""
         if root is not None:
             self.set_cursor(root)
         root = self.get_cursor()
-        left = self.build_left_kmers(self.filter_nodes(self.left_extensions()), root)
-        right = self.build_right_kmers(self.filter_nodes(self.right_extensions()), root)
+        left = self.build_left_kmers(self.filter_nodes(self.left_extensions(), union_nodes), root)
+        right = self.build_right_kmers(self.filter_nodes(self.right_extensions(), union_nodes), root)
         if len(left) > 1 or len(right) > 1:
             return True, (left, right)
         return False, ([], [])

@@ -196,6 +196,20 @@ def test_fork_degree(st):
     assert pos2 == pos and len(nbs) == len(pos)
     i = pos.index(len(core) - K)
     assert sorted(k.kmer for k in nbs[i][1]) == sorted(k.kmer for k in right)
+    # union nodes (unitig_walker.hh:336-348, 457-468, 1047-1082): a third
+    # branch not in the graph counts once it is among the extras
+    alt2 = [c for c in 'ACGT' if c not in (branch_a[40], alt)][0]
+    third = g.hash(fork[1:] + alt2)
+    g.set_cursor(fork)
+    assert g.out_degree() == 2 and g.out_degree({third}) == 3 and g.in_degree({third}) == 1
+    assert g.degree({third.value}) == 4
+    is_dec, (left3, right3) = g.get_decision_neighbors(fork, union_nodes={third})
+    assert is_dec and sorted(k.kmer for k in right3) == sorted([fork[1:] + branch_a[40], fork[1:] + alt,
+                                                                fork[1:] + alt2])
+    assert len(g.filter_nodes(g.right_extensions(), [third])) == 3
+    inner = core[5:5 + K]
+    nxt = g.hash(inner[1:] + ('A' if core[5 + K] != 'A' else 'C'))
+    assert g.get_decision_neighbors(inner, union_nodes={nxt})[0]
 
 
 def test_kmer_iterator():
