@@ -20,11 +20,14 @@ The rest of the compactor (the cDBG and its updates) is not part of this
 package.
 """
 
+import numpy as np
+
 from . import _native as N
 from .dbg import Kmer, Shift, _as_batch
 from .hashing import Hash, _bytes
 
 SYMBOLS = 'ACGT'
+_POPCOUNT4 = np.array([bin(i).count('1') for i in range(16)], np.uint8)
 
 
 class compact_segment:
@@ -158,13 +161,21 @@ class _SegmentFinder:
             preprocess[1].left_flank = ln[0].hash if len(ln) == 1 else preprocess[1].left_anchor
 
         # second pass: cut each segment at its decision k-mers
-        # (compactor.hh:439-532)
+        # (compactor.hh:439-532).  The new-k-mer set is final here, so the
+        # presence of every window's neighbours is computed at once.
+        vals = np.asarray(vals, np.uint64)
+        in_set = np.isin(vals, np.fromiter(known, np.uint64, len(known))) if known else np.zeros(vals.shape, bool)
+        union = np.asarray(mask, np.uint8) | (in_set.astype(np.uint8) << np.arange(8, dtype=np.uint8)).sum(
+            axis=1, dtype=np.uint8)
+        nl, nr = _POPCOUNT4[union & 15], _POPCOUNT4[union >> 4]
+        is_decision = (nl > 1) | (nr > 1)
         for seg in preprocess:
             if seg.is_null():
                 segments.append(seg)
                 continue
             decisions = []
-            for pos in range(seg.start_pos, seg.start_pos + seg.length - K + 1):
+            a = seg.start_pos
+            for pos in (a + np.flatnonzero(is_decision[a:a + seg.length - K + 1])).tolist():
                 left, right = present(pos, 0), present(pos, 4)
                 if len(left) > 1 or len(right) > 1:
                     root = s[pos:pos + K]
