@@ -121,6 +121,9 @@ SIGNATURES = {
     'gt_dbg_degrees': [st_p, u16, cint, cint, cint, vp, vp, u64, vp, vp, vp],
     'gt_dbg_neighbors_dev': [st_p, u16, cint, cint, cint, vp, vp, u64, u64, vp, vp, vp, vp, vp],
     'gt_dbg_neighbors': [st_p, u16, cint, cint, cint, vp, vp, u64, vp, vp, vp, vp],
+    'gt_find_new_segments': [st_p, u16, cint, vp, vp, u64, vp, u64, cint, vp, u64, vp, vp, vp, vp, u64, vp, vp, u64,
+                             vp, vp],
+    'gt_segments_from_view': [u16, vp, u64, vp, vp, vp, vp, vp, u64, cint, vp, u64, vp, vp, u64, vp, vp, u64, vp, vp],
     'gt_insert_hashes_dev': [st_p, vp, u64, vp, vp],
     'gt_insert_hashes': [st_p, vp, u64, vp],
     'gt_query_hashes_dev': [st_p, vp, u64, vp, vp],

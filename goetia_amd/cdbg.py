@@ -5,11 +5,11 @@ follows compactor.hh:321-534: it splits a read into the runs of k-mers that
 are new to the graph (segments with anchors and flanks), cut at the k-mers
 that would become decision nodes once the read is in the graph, with
 compact_segment as compactor.hh:77-145 and its Python conveniences as
-pythonize_cdbg.py:53-97.  The graph work runs on the GPU, two launches per
-call (or per batch of reads): one query of every window, and one pass of the
-neighbour kernel giving every window's eight neighbours and their presence
-(gt_dbg_neighbors).  The segment bookkeeping is per read and sequential, as in
-goetia, and stays on the host.
+pythonize_cdbg.py:53-97.  The work is one library call, gt_find_new_segments:
+two GPU launches per call (or per batch of reads) give every window's query
+count and its eight neighbours with their presence (gt_dbg_neighbors), and
+the library's host code runs the two passes over each read, sequential per
+read as in goetia.
 
 goetia builds this code for forward hashes only: find_new_segments assigns
 a neighbour's value_type to a hash (compactor.hh:405, 428), which Canonical
@@ -19,15 +19,16 @@ graphs (compactor.cc, end).  A canonical graph is refused here as well.
 The rest of the compactor (the cDBG and its updates) is not part of this
 package.
 """
+import ctypes as C
 
 import numpy as np
 
 from . import _native as N
-from .dbg import Kmer, Shift, _as_batch
+from .dbg import Kmer, _as_batch
 from .hashing import Hash, _bytes
+from .reads import ReadBatch
 
 SYMBOLS = 'ACGT'
-_POPCOUNT4 = np.array([bin(i).count('1') for i in range(16)], np.uint8)
 
 
 class compact_segment:
@@ -85,197 +86,93 @@ class SegmentList(list):
         return self[-1]
 
 
-def _hash_at(hashes, i):
-    # compactor.hh:471-525 index the hash vector at start_pos +- 1 even where
-    # that runs off the read; the segment built from such an index is always
-    # shorter than K and never kept, so its value is never seen.
-    return hashes[i] if 0 <= i < len(hashes) else Hash()
+SEG_WORDS, DEC_WORDS = 7, 11
 
 
-class _SegmentFinder:
-    """The host side of find_new_segments: the two passes over one read
-    given its graph view."""
+def _values(items):
+    return np.array([h.value if isinstance(h, Hash) else int(h) for h in items], np.uint64)
 
-    def __init__(self, K):
-        self.K = K
 
-    def run(self, s, counts, hv, vals, mask, hashes, new_kmers, segments, new_decision_kmers, decision_neighbors):
-        K = self.K
-        one_arg = hashes is None
-        if one_arg:
-            hashes, new_kmers, segments = [], set(), SegmentList()
-            new_decision_kmers, decision_neighbors = set(), []
-        base = len(hashes)
-        known = {h.value if isinstance(h, Hash) else int(h) for h in new_kmers}
+class _Outputs:
+    """Output buffers of gt_find_new_segments / gt_segments_from_view, grown
+    to the totals the library reports when a first guess is too small."""
 
-        def present(i, lo):
-            """Shifts of window i (lo 0: left, 4: right) present in the graph
-            or among the new k-mers (filter_nodes with extras,
-            unitig_walker.hh:457-468), in ACGT order."""
-            m = int(mask[i])
-            out = []
-            for c in range(4):
-                v = int(vals[i, lo + c])
-                if (m >> (lo + c)) & 1 or v in known:
-                    out.append(Shift(Hash(v), SYMBOLS[c], lo == 4))
-            return out
+    def __init__(self, n_reads, n_windows):
+        self.seg_off = np.zeros(n_reads + 1, np.int64)
+        self.dec_off = np.zeros(n_reads + 1, np.int64)
+        self.seg_cap = 4 * n_reads + n_windows // 16 + 16
+        self.dec_cap = n_windows // 16 + 16
+        self.segs = np.zeros(self.seg_cap * SEG_WORDS, np.uint64)
+        self.decs = np.zeros(self.dec_cap * DEC_WORDS, np.uint64)
+        self.n_segs, self.n_decs = C.c_uint64(), C.c_uint64()
 
-        # first pass: runs of new, not yet seen k-mers (compactor.hh:352-400)
-        preprocess = []
-        current = compact_segment()
-        prev_hash = Hash()
-        prev_new = prev_seen = cur_new = cur_seen = False
-        cur_hash = None
-        n = len(hv)
-        for pos in range(n):
-            cur_hash = Hash(int(hv[pos]))
-            cur_new = int(counts[pos]) == 0
-            cur_seen = cur_hash.value in known
-            hashes.append(cur_hash)
-            if cur_new and not cur_seen:
-                if not prev_new or prev_seen:
-                    preprocess.append(current)
-                    current = self._init_segment(cur_hash, prev_hash, pos)
-                new_kmers.add(cur_hash)
-                known.add(cur_hash.value)
-            elif not current.is_null() and (not cur_new or cur_seen):
-                self._finish_segment(current, pos - 1, prev_hash, cur_hash, preprocess)
-                current = compact_segment()
-            prev_hash, prev_new, prev_seen = cur_hash, cur_new, cur_seen
+    def args(self):
+        return (N.ptr(self.seg_off), N.ptr(self.segs), self.seg_cap, N.ptr(self.dec_off), N.ptr(self.decs),
+                self.dec_cap, C.byref(self.n_segs), C.byref(self.n_decs))
 
-        if cur_new and not cur_seen:
-            # the read ends inside a segment: its right flank is the unique
-            # right neighbour, if there is one (compactor.hh:401-417)
-            right_flank = cur_hash
-            rn = present(n - 1, 4)
-            if len(rn) == 1:
-                right_flank = rn[0].hash
-            self._finish_segment(current, n - 1, cur_hash, right_flank, preprocess)
-        preprocess.append(compact_segment())
-        if len(preprocess) < 3:
-            return segments if one_arg else None
+    def call(self, name, *head):
+        try:
+            N.call(name, *head, *self.args())
+        except ValueError:
+            if self.n_segs.value <= self.seg_cap and self.n_decs.value <= self.dec_cap:
+                raise
+            self.seg_cap, self.dec_cap = max(self.n_segs.value, 1), max(self.n_decs.value, 1)
+            self.segs = np.zeros(self.seg_cap * SEG_WORDS, np.uint64)
+            self.decs = np.zeros(self.dec_cap * DEC_WORDS, np.uint64)
+            N.call(name, *head, *self.args())
 
-        if preprocess[1].start_pos == 0:
-            # compactor.hh:425-437
-            ln = present(0, 0)
-            preprocess[1].left_flank = ln[0].hash if len(ln) == 1 else preprocess[1].left_anchor
-
-        # second pass: cut each segment at its decision k-mers
-        # (compactor.hh:439-532).  The new-k-mer set is final here, so the
-        # presence of every window's neighbours is computed at once.
-        vals = np.asarray(vals, np.uint64)
-        in_set = np.isin(vals, np.fromiter(known, np.uint64, len(known))) if known else np.zeros(vals.shape, bool)
-        union = np.asarray(mask, np.uint8) | (in_set.astype(np.uint8) << np.arange(8, dtype=np.uint8)).sum(
-            axis=1, dtype=np.uint8)
-        nl, nr = _POPCOUNT4[union & 15], _POPCOUNT4[union >> 4]
-        is_decision = (nl > 1) | (nr > 1)
-        for seg in preprocess:
-            if seg.is_null():
-                segments.append(seg)
-                continue
-            decisions = []
-            a = seg.start_pos
-            for pos in (a + np.flatnonzero(is_decision[a:a + seg.length - K + 1])).tolist():
-                left, right = present(pos, 0), present(pos, 4)
-                if len(left) > 1 or len(right) > 1:
-                    root = s[pos:pos + K]
-                    lk = [Kmer(x.hash, x.symbol + root[:K - 1]) for x in left]
-                    rk = [Kmer(x.hash, root[1:] + x.symbol) for x in right]
-                    decision_neighbors.append((lk, rk))
-                    h = hashes[base + pos]
-                    new_decision_kmers.add(h)
-                    d = self._init_segment(h, h, pos)
-                    d.length = K
-                    d.right_anchor = d.left_anchor
-                    d.right_flank = d.left_flank
-                    d.is_decision_kmer = True
-                    decisions.append(d)
-            self._split(seg, decisions, hashes, segments)
-        return segments if one_arg else None
-
-    @staticmethod
-    def _init_segment(left_anchor, left_flank, start_pos):
-        seg = compact_segment()
-        seg.start_pos = start_pos
-        seg.left_anchor = left_anchor
-        seg.left_flank = left_flank
-        seg.is_decision_kmer = False
-        return seg
-
-    def _finish_segment(self, seg, end, right_anchor, right_flank, out):
-        seg.length = end - seg.start_pos + self.K
-        seg.right_anchor = right_anchor
-        seg.right_flank = right_flank
-        out.append(seg)
-
-    def _split(self, seg, decisions, hashes, segments):
-        """The cuts of compactor.hh:470-531 (hash indexes as the reference
-        takes them, from the front of the caller's hash vector)."""
-        K = self.K
-        if not decisions:
+    def read(self, r, s, K, counts, hv, hashes, new_kmers, segments, new_decision_kmers, decision_neighbors):
+        """Appends read r's results (s: its sequence; counts / hv: its
+        windows) to the caller's containers, as goetia's six-argument
+        overload does."""
+        for v in hv.tolist():
+            hashes.append(Hash(v))
+        for v, c in zip(hv.tolist(), counts.tolist()):
+            if c == 0:
+                new_kmers.add(Hash(v))
+        for j in range(int(self.seg_off[r]), int(self.seg_off[r + 1])):
+            w = self.segs[SEG_WORDS * j:SEG_WORDS * j + SEG_WORDS].tolist()
+            seg = compact_segment(Hash(w[0]), Hash(w[1]), bool(w[4]), w[5], w[6])
+            seg.left_flank, seg.right_flank = Hash(w[2]), Hash(w[3])
             segments.append(seg)
-            return
-        if len(decisions) == 1:
-            d = decisions[0]
-            right = compact_segment()
-            right.left_flank = d.right_anchor
-            right.left_anchor = _hash_at(hashes, d.start_pos + 1)
-            right.right_anchor = seg.right_anchor
-            right.right_flank = seg.right_flank
-            right.start_pos = d.start_pos + 1
-            right.length = seg.length - (right.start_pos - seg.start_pos)
-            seg.right_anchor = _hash_at(hashes, d.start_pos - 1)
-            seg.right_flank = d.left_anchor
-            seg.length = d.start_pos + K - 1 - seg.start_pos
-            if seg.length >= K:
-                segments.append(seg)
-            segments.append(d)
-            if right.length >= K:
-                segments.append(right)
-            return
-        first_d, last_d = decisions[0], decisions[-1]
-        first = compact_segment()
-        first.left_flank = seg.left_flank
-        first.left_anchor = seg.left_anchor
-        first.right_anchor = _hash_at(hashes, first_d.start_pos - 1)
-        first.right_flank = _hash_at(hashes, first_d.start_pos)
-        first.start_pos = seg.start_pos
-        first.length = first_d.start_pos + K - 1 - seg.start_pos
-        if first.length >= K:
-            segments.append(first)
-        segments.append(first_d)
-        for u, v in zip(decisions, decisions[1:]):
-            mid = compact_segment()
-            mid.left_flank = u.left_anchor
-            mid.left_anchor = _hash_at(hashes, u.start_pos + 1)
-            mid.right_anchor = _hash_at(hashes, v.start_pos - 1)
-            mid.right_flank = v.left_anchor
-            mid.start_pos = u.start_pos + 1
-            mid.length = v.start_pos + K - 1 - mid.start_pos
-            if mid.length >= K:
-                segments.append(mid)
-            segments.append(v)
-        last = compact_segment()
-        last.right_anchor = seg.right_anchor
-        last.right_flank = seg.right_flank
-        last.left_flank = last_d.right_anchor
-        last.left_anchor = _hash_at(hashes, last_d.start_pos + 1)
-        last.start_pos = last_d.start_pos + 1
-        last.length = seg.length - (last.start_pos - seg.start_pos)
-        if last.length >= K:
-            segments.append(last)
+        for j in range(int(self.dec_off[r]), int(self.dec_off[r + 1])):
+            w = self.decs[DEC_WORDS * j:DEC_WORDS * j + DEC_WORDS].tolist()
+            pos, h, m, nb = w[0], w[1], w[2], w[3:]
+            root = s[pos:pos + K]
+            lk = [Kmer(Hash(nb[c]), SYMBOLS[c] + root[:K - 1]) for c in range(4) if (m >> c) & 1]
+            rk = [Kmer(Hash(nb[4 + c]), root[1:] + SYMBOLS[c]) for c in range(4) if (m >> (4 + c)) & 1]
+            decision_neighbors.append((lk, rk))
+            new_decision_kmers.add(Hash(h))
 
+
+def _containers(hashes, new_kmers, segments, new_decision_kmers, decision_neighbors):
+    if hashes is None:
+        return True, ([], set(), SegmentList(), set(), [])
+    return False, (hashes, new_kmers, segments, new_decision_kmers, decision_neighbors)
 
 
 def segments_from_view(K, sequence, counts, hash_values, nb_values, nb_present, hashes=None, new_kmers=None,
                        segments=None, new_decision_kmers=None, decision_neighbors=None):
-    """find_new_segments of one read from its graph view: per window the
-    query count, the hash value, the eight neighbour values (left A C G T,
-    right A C G T) and the presence mask (bit c left, 4 + c right), as
-    dBG.window_neighbors gives them.  Arguments and result as
+    """find_new_segments of one read from its graph view (per window the
+    query count, the hash value, the eight neighbour values -- left A C G T,
+    right A C G T -- and the presence mask, bit c left, 4 + c right, as
+    dBG.window_neighbors gives them), through the library's host code
+    (gt_segments_from_view; no device).  Arguments and result as
     Compactor.find_new_segments."""
-    return _SegmentFinder(K).run(sequence, counts, hash_values, nb_values, nb_present, hashes, new_kmers, segments,
-                                 new_decision_kmers, decision_neighbors)
+    s = sequence if isinstance(sequence, str) else _bytes(sequence).decode()
+    one_arg, (hs, nk, segs, dks, pairs) = _containers(hashes, new_kmers, segments, new_decision_kmers,
+                                                       decision_neighbors)
+    counts = np.ascontiguousarray(counts, np.int16)
+    hv = np.ascontiguousarray(hash_values, np.uint64)
+    vals = np.ascontiguousarray(nb_values, np.uint64).reshape(-1, 8)
+    mask = np.ascontiguousarray(nb_present, np.uint8)
+    off = np.array([0, len(s)], np.int64)
+    known, prefix = _values(nk), _values(hs)
+    out = _Outputs(1, len(hv))
+    out.call('gt_segments_from_view', K, N.ptr(off), 1, N.ptr(counts), N.ptr(hv), N.ptr(vals), N.ptr(mask),
+             N.ptr(known), len(known), 0, N.ptr(prefix), len(prefix))
+    out.read(0, s, K, counts, hv, hs, nk, segs, dks, pairs)
+    return segs if one_arg else None
 
 
 class Compactor:
@@ -296,13 +193,15 @@ class Compactor:
     def __smartptr__(self):
         return self
 
-    # ------------------------------------------------------------ graph side
-    def _graph_view(self, reads):
-        """Per window of `reads`: the query count, the hash value, the eight
-        neighbour values and their presence mask (two GPU launches)."""
-        counts, hv = self.dbg.query_sequences(reads, with_hashes=True)
-        _, _, vals, mask = self.dbg.window_neighbors(reads)
-        return counts, hv, vals, mask
+    def _run(self, batch, known, shared, prefix):
+        nk = int(batch.kmer_counts(self.K).sum())
+        counts = np.zeros(max(nk, 1), np.int16)
+        hv = np.zeros(max(nk, 1), np.uint64)
+        out = _Outputs(batch.n_reads, nk)
+        out.call('gt_find_new_segments', self.dbg.S.handle, self.K, self.dbg.mode, N.ptr(batch.seq),
+                 N.ptr(batch.offsets), batch.n_reads, N.ptr(known), len(known), int(shared), N.ptr(prefix),
+                 len(prefix), N.ptr(counts), N.ptr(hv))
+        return out, counts, hv
 
     def find_new_segments(self, sequence, hashes=None, new_kmers=None, segments=None, new_decision_kmers=None,
                           decision_neighbors=None):
@@ -314,36 +213,41 @@ class Compactor:
         s = sequence if isinstance(sequence, str) else _bytes(sequence).decode()
         if len(s) < self.K:
             raise N.SequenceLengthException('KmerIterator: sequence shorter than K')
-        counts, hv, vals, mask = self._graph_view([s])
-        return segments_from_view(self.K, s, counts, hv, vals, mask, hashes, new_kmers, segments,
-                                  new_decision_kmers, decision_neighbors)
+        one_arg, (hs, nk, segs, dks, pairs) = _containers(hashes, new_kmers, segments, new_decision_kmers,
+                                                           decision_neighbors)
+        out, counts, hv = self._run(ReadBatch.from_strings([s]), _values(nk), 0, _values(hs))
+        n = len(s) - self.K + 1
+        out.read(0, s, self.K, counts[:n], hv[:n], hs, nk, segs, dks, pairs)
+        return segs if one_arg else None
 
     def find_new_segments_batch(self, reads, new_kmers=None):
         """find_new_segments over every read of a batch against the graph as
-        it is (nothing inserted in between), with two GPU launches for the
-        whole batch.  new_kmers: None for a fresh set per read (the
-        one-argument overload), or one set shared by the reads in order.
-        Returns per read (segments, hashes, new k-mers, new decision k-mers,
-        decision neighbour pairs); reads shorter than K give None."""
+        it is (nothing inserted in between): two GPU launches and one pass of
+        the library's host code for the whole batch.  new_kmers: None for a
+        fresh set per read (the one-argument overload), or one set shared by
+        the reads in order.  Returns per read (segments, hashes, new k-mers,
+        new decision k-mers, decision neighbour pairs); reads shorter than K
+        give None."""
         batch = _as_batch(reads)
-        counts, hv, vals, mask = self._graph_view(batch)
+        shared = new_kmers is not None
+        out, counts, hv = self._run(batch, _values(new_kmers) if shared else _values(()), shared, _values(()))
         K = self.K
-        out = []
+        res = []
         k = 0
         for r in range(batch.n_reads):
             a, b = int(batch.offsets[r]), int(batch.offsets[r + 1])
             if b - a < K:
-                out.append(None)
+                res.append(None)
                 continue
             n = b - a - K + 1
             s = bytes(batch.seq[a:b]).decode()
             hs, segs, dks, pairs = [], SegmentList(), set(), []
-            nk = set() if new_kmers is None else new_kmers
-            segments_from_view(K, s, counts[k:k + n], hv[k:k + n], vals[k:k + n], mask[k:k + n], hs, nk, segs, dks,
-                               pairs)
-            out.append((segs, hs, nk, dks, pairs))
+            nk = new_kmers if shared else set()
+            out.read(r, s, K, counts[k:k + n], hv[k:k + n], hs, nk, segs, dks, pairs)
+            res.append((segs, hs, nk, dks, pairs))
             k += n
-        return out
+        return res
+
 
 class _CompactorFamily(type):
     """StreamingCompactor[graph_type] -> a namespace with .Compactor and

@@ -225,6 +225,41 @@ int gt_dbg_neighbors(gt_storage* st, uint16_t K, int canonical, int mode, int mi
                      uint8_t* in_deg, uint8_t* out_deg, uint64_t* nb_values,
                      uint8_t* nb_present);
 
+/* StreamingCompactor<dBG<S, FwdLemireShifter>>::Compactor::find_new_segments
+ * (compactor.hh:321-534) over a batch of reads against the graph as it is
+ * (nothing is inserted).  Forward hashes (goetia builds the compactor for
+ * Hash<> only).  Each read starts from the new-k-mer set `known` (the
+ * one-argument overload: n_known 0), or with shared != 0 one set runs through
+ * the reads in order (a caller-owned set across calls).  prefix / n_prefix:
+ * hashes already in the caller's hash vector, which goetia indexes from its
+ * front (one read only).  Per window: counts (query) and hashes (value) out.
+ * Segments of read r are segs[7 j .. 7 j + 6] for j in [seg_off[r],
+ * seg_off[r + 1]): left_anchor, right_anchor, left_flank, right_flank,
+ * is_decision_kmer, start_pos, length (null delimiters included, none for a
+ * read shorter than K or without new k-mers).  Decision k-mers of read r are
+ * decs[11 j ..] for j in [dec_off[r], dec_off[r + 1]): window, hash, a mask
+ * of the neighbours present in the graph or the set (bit c left "ACGT"[c],
+ * 4 + c right; get_decision_neighbors with union nodes,
+ * unitig_walker.hh:1047-1082) and the eight neighbour values.  *n_segs /
+ * *n_decs receive the totals; GT_ERR_INVALID_ARG when seg_cap / dec_cap are
+ * smaller (call again with them). */
+int gt_find_new_segments(gt_storage* st, uint16_t K, int mode, const uint8_t* seq,
+                         const int64_t* offsets, uint64_t n_reads, const uint64_t* known,
+                         uint64_t n_known, int shared, const uint64_t* prefix, uint64_t n_prefix,
+                         int16_t* counts, uint64_t* hashes, int64_t* seg_off, uint64_t* segs,
+                         uint64_t seg_cap, int64_t* dec_off, uint64_t* decs, uint64_t dec_cap,
+                         uint64_t* n_segs, uint64_t* n_decs);
+/* The host part of gt_find_new_segments, given each window's graph view
+ * (count, hash, neighbour values and presence as gt_dbg_neighbors gives
+ * them); needs no device. */
+int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_t n_reads,
+                          const int16_t* counts, const uint64_t* hashes,
+                          const uint64_t* nb_values, const uint8_t* nb_present,
+                          const uint64_t* known, uint64_t n_known, int shared,
+                          const uint64_t* prefix, uint64_t n_prefix, int64_t* seg_off,
+                          uint64_t* segs, uint64_t seg_cap, int64_t* dec_off, uint64_t* decs,
+                          uint64_t dec_cap, uint64_t* n_segs, uint64_t* n_decs);
+
 /* ------------------------------------------------------------ per-hash */
 
 /* dBG::insert(hash) / Storage::insert (dbg.hh:140-146, storage.hh:120)

@@ -6,6 +6,8 @@
 // insert_after", the count and lines of the reads inserted first, then the
 // count and lines of the reads.  WORDS gets per case its word count and
 // words, KMERS per case its byte count on a line and the neighbour k-mers.
+// For cases whose graph does not change between reads, the batch form must
+// give what the per-read calls gave (checked here; exit status 1 if not).
 #include <cstdio>
 #include <cstdint>
 #include <deque>
@@ -42,6 +44,11 @@ int run(std::shared_ptr<S> store, int K, bool persist, bool insert_after, const 
     auto cp = SC::Compactor::build(g);
     std::set<typename G::hash_type> shared;
     auto put = [&](uint64_t v) { words.push_back(v); };
+    // per-read results, compared below with the batch form when the graph
+    // does not change between reads
+    std::vector<std::vector<typename SC::compact_segment>> all_segs(reads.size());
+    std::vector<std::set<typename G::hash_type>> all_dec(reads.size());
+    std::vector<std::deque<typename G::neighbor_pair_type>> all_pairs(reads.size());
     for (size_t r = 0; r < reads.size(); ++r) {
         std::vector<typename G::hash_type> hashes;
         std::set<typename G::hash_type> fresh, dec;
@@ -82,7 +89,42 @@ int run(std::shared_ptr<S> store, int K, bool persist, bool insert_after, const 
                 kmers += k.kmer;
             }
         }
+        all_segs[r] = segs;
+        all_dec[r] = dec;
+        all_pairs[r] = pairs;
         if (insert_after && reads[r].size() >= (size_t)K) g->insert_sequence(reads[r]);
+    }
+    if (!insert_after) {
+        std::vector<std::vector<typename SC::compact_segment>> bs;
+        std::vector<std::set<typename G::hash_type>> bd;
+        std::vector<std::deque<typename G::neighbor_pair_type>> bp;
+        std::set<typename G::hash_type> bshared;
+        cp->find_new_segments(ReadBatch(reads), bs, bd, bp, persist ? &bshared : nullptr);
+        for (size_t r = 0; r < reads.size(); ++r) {
+            bool same = bs[r].size() == all_segs[r].size() && bd[r] == all_dec[r] && bp[r].size() == all_pairs[r].size();
+            for (size_t j = 0; same && j < bs[r].size(); ++j) {
+                const auto &a = bs[r][j], &b = all_segs[r][j];
+                same = a.left_anchor == b.left_anchor && a.right_anchor == b.right_anchor &&
+                       a.left_flank == b.left_flank && a.right_flank == b.right_flank &&
+                       a.is_decision_kmer == b.is_decision_kmer && a.start_pos == b.start_pos && a.length == b.length;
+            }
+            for (size_t j = 0; same && j < bp[r].size(); ++j) {
+                const auto &a = bp[r][j], &b = all_pairs[r][j];
+                same = a.first.size() == b.first.size() && a.second.size() == b.second.size();
+                for (size_t k = 0; same && k < a.first.size(); ++k)
+                    same = a.first[k].value() == b.first[k].value() && a.first[k].kmer == b.first[k].kmer;
+                for (size_t k = 0; same && k < a.second.size(); ++k)
+                    same = a.second[k].value() == b.second[k].value() && a.second[k].kmer == b.second[k].kmer;
+            }
+            if (!same) {
+                std::fprintf(stderr, "batch form differs at read %zu\n", r);
+                return 1;
+            }
+        }
+        if (persist && bshared != shared) {
+            std::fprintf(stderr, "batch form's shared set differs\n");
+            return 1;
+        }
     }
     return 0;
 }
@@ -103,13 +145,16 @@ int main(int argc, char** argv) {
         auto reads = read_lines(in);
         std::vector<uint64_t> words;
         std::string kmers;
+        int rc;
         if (kind == 1)
-            run(BitStorage::build(max_table, (uint16_t)n_tables), K, persist, insert_after, bg, reads, words, kmers);
+            rc = run(BitStorage::build(max_table, (uint16_t)n_tables), K, persist, insert_after, bg, reads, words,
+                     kmers);
         else if (kind == 2)
-            run(ByteStorage::build(max_table, (uint16_t)n_tables), K, persist, insert_after, bg, reads, words,
-                kmers);
+            rc = run(ByteStorage::build(max_table, (uint16_t)n_tables), K, persist, insert_after, bg, reads, words,
+                     kmers);
         else
-            run(SparseppSetStorage::build(), K, persist, insert_after, bg, reads, words, kmers);
+            rc = run(SparseppSetStorage::build(), K, persist, insert_after, bg, reads, words, kmers);
+        if (rc) return rc;
         std::fprintf(wf, "%zu\n", words.size());
         for (uint64_t w : words) std::fprintf(wf, "%llu\n", (unsigned long long)w);
         std::fprintf(kf, "%zu\n%s\n", kmers.size(), kmers.c_str());
