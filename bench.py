@@ -70,7 +70,7 @@ def parse():
                    help='run one 1 GiB device copy first (reference kernel for HBM byte counters)')
     p.add_argument('--workload', default='cms_20m',
                    choices=['cms_20m', 'bloom_100k', 'exact_mixed', 'neighbors_5m', 'serving_64', 'fastx',
-                            'diginorm'],
+                            'diginorm', 'segments_5m'],
                    help='cms_20m is the headline; the others print their own JSON line')
     p.add_argument('--plain-chunk-kmers', type=int, default=500_000,
                    help='chunk size of the plain path in k-mers (goetia IntervalCounter default)')

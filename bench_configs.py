@@ -356,6 +356,97 @@ def diginorm(args):
     print(json.dumps(out), flush=True)
 
 
+def segments(args):
+    """StreamingCompactor::find_new_segments over a batch against a fixed
+    graph (gt_find_new_segments: GPU view + the library's host passes), and
+    the compiled goetia compactor (oracle/_ref, when built) on a sample of the
+    same reads, with the segment and decision counts compared."""
+    import ctypes as C
+    import os
+    from goetia_amd import _native as N
+    from goetia_amd import libgoetia as lg
+    from goetia_amd import workload
+    t0 = time.perf_counter()
+    cfg, bg, q = workload.segments_5m(args.scale)
+    K = cfg['K']
+    log(f'generated {bg.n_reads} + {q.n_reads} reads in {time.perf_counter() - t0:.1f}s')
+    G = lg.dBG[lg.SparseppSetStorage, lg.FwdLemireShifter]
+    g = G(lg.SparseppSetStorage.build(), K)
+    g.insert_sequences(bg)
+    nk = int(q.kmer_counts(K).sum())
+    counts = np.zeros(nk, np.int16)
+    hv = np.zeros(nk, np.uint64)
+    seg_off = np.zeros(q.n_reads + 1, np.int64)
+    dec_off = np.zeros(q.n_reads + 1, np.int64)
+    seg_cap, dec_cap = 8 * q.n_reads + nk // 4, nk // 4 + 16
+    segs = np.zeros(seg_cap * 7, np.uint64)
+    decs = np.zeros(dec_cap * 11, np.uint64)
+    n_segs, n_decs = C.c_uint64(), C.c_uint64()
+
+    def call(n_reads):
+        N.call('gt_find_new_segments', g.S.handle, K, 0, N.ptr(q.seq), N.ptr(q.offsets), n_reads, None, 0, 0, None,
+               0, N.ptr(counts), N.ptr(hv), N.ptr(seg_off), N.ptr(segs), seg_cap, N.ptr(dec_off), N.ptr(decs),
+               dec_cap, C.byref(n_segs), C.byref(n_decs))
+
+    call(q.n_reads)                                        # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call(q.n_reads)
+    dt = (time.perf_counter() - t0) / reps
+    out = {'metric': 'reads/s find_new_segments against a fixed graph (1 GPU + host passes)',
+           'workload': 'segments_5m', 'value': round(q.n_reads / dt, 1), 'unit': 'reads/s',
+           'reads': q.n_reads, 'kmers': nk, 'graph_kmers': g.n_unique(), 'ms_per_batch': round(dt * 1e3, 2),
+           'segments': int(n_segs.value), 'decision_kmers': int(n_decs.value),
+           'new_kmers': int((counts == 0).sum())}
+    ref_so = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'oracle', '_ref', 'libgoetia_ref.so')
+    if os.path.exists(ref_so) and not args.no_cpu_baseline:
+        L = C.CDLL(ref_so)
+        L.ref_new.restype = C.c_void_p
+        L.ref_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint16]
+        L.ref_insert_reads.restype = C.c_uint64
+        L.ref_insert_reads.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.ref_find_new_segments.restype = C.c_uint64
+        L.ref_find_new_segments.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
+                                            C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64, C.c_void_p]
+        h = L.ref_new(3, 0, K, 0, 0)
+        L.ref_insert_reads(h, K, 0, bg.seq.ctypes.data_as(C.c_char_p), bg.offsets.ctypes.data, bg.n_reads, None)
+        n = min(q.n_reads, 20_000)
+        cap, kcap = 1 << 24, 1 << 26
+        words = np.zeros(cap, np.uint64)
+        km = np.zeros(kcap, np.uint8)
+        used = C.c_uint64()
+        t0 = time.perf_counter()
+        w = L.ref_find_new_segments(h, q.seq.ctypes.data_as(C.c_char_p), q.offsets.ctypes.data, n, 0, 0,
+                                    words.ctypes.data, cap, km.ctypes.data_as(C.c_char_p), kcap, C.byref(used))
+        t_ref = time.perf_counter() - t0
+        # segment and decision counts per read from the reference's stream
+        ref_segs = ref_decs = 0
+        i = 0
+        for _ in range(n):
+            ns = int(words[i])
+            i += 1
+            if ns == (1 << 64) - 1:
+                continue
+            ref_segs += ns
+            i += 7 * ns
+            i += 1 + int(words[i])
+            i += 1 + int(words[i])
+            npairs = int(words[i])            # one pair per decision window
+            ref_decs += npairs
+            i += 1
+            for _ in range(npairs):
+                i += 2 + int(words[i]) + int(words[i + 1])
+        assert i == w
+        call(n)
+        out['cpu_baseline'] = {'value': round(n / t_ref, 1), 'unit': 'reads/s', 'cores': 1, 'kind': 'reference',
+                               'sample': f'first {n} reads, compiled goetia StreamingCompactor (oracle/_ref), '
+                                         f'{t_ref:.1f} s',
+                               'same_counts': bool(ref_segs == n_segs.value and ref_decs == n_decs.value)}
+    print(json.dumps(out), flush=True)
+
+
 def run(args):
     return {'bloom_100k': bloom, 'exact_mixed': exact, 'neighbors_5m': neighbors,
-            'serving_64': serving, 'fastx': fastx, 'diginorm': diginorm}[args.workload](args)
+            'serving_64': serving, 'fastx': fastx, 'diginorm': diginorm,
+            'segments_5m': segments}[args.workload](args)

@@ -14,7 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
-#include <unordered_set>
+#include <thread>
 #include <vector>
 
 namespace gt {
@@ -22,6 +22,72 @@ int set_error(int code, const std::string& msg);
 }
 
 namespace {
+
+// A set of hash values: open addressing with linear probing, cleared in O(1)
+// by a generation stamp so one instance serves many reads.
+class FlatSet {
+  public:
+    void clear() {
+        if (++gen_ == 0) {
+            std::fill(stamp_.begin(), stamp_.end(), 0u);
+            gen_ = 1;
+        }
+        size_ = 0;
+    }
+    void reserve(size_t n) {
+        if (2 * n > keys_.size()) rehash(2 * n);
+    }
+    bool contains(uint64_t k) const {
+        if (keys_.empty()) return false;
+        const size_t mask = keys_.size() - 1;
+        for (size_t i = slot(k, mask);; i = (i + 1) & mask) {
+            if (stamp_[i] != gen_) return false;
+            if (keys_[i] == k) return true;
+        }
+    }
+    void insert(uint64_t k) {
+        if (2 * (size_ + 1) > keys_.size()) rehash(2 * (size_ + 1));
+        const size_t mask = keys_.size() - 1;
+        for (size_t i = slot(k, mask);; i = (i + 1) & mask) {
+            if (stamp_[i] != gen_) {
+                stamp_[i] = gen_;
+                keys_[i] = k;
+                ++size_;
+                return;
+            }
+            if (keys_[i] == k) return;
+        }
+    }
+    template <class F>
+    void for_each(F&& f) const {
+        for (size_t i = 0; i < keys_.size(); ++i)
+            if (stamp_[i] == gen_) f(keys_[i]);
+    }
+
+  private:
+    std::vector<uint64_t> keys_;
+    std::vector<uint32_t> stamp_;
+    uint32_t gen_ = 1;
+    size_t size_ = 0;
+
+    static size_t slot(uint64_t k, size_t mask) {
+        k ^= k >> 33;
+        k *= 0xff51afd7ed558ccdULL;
+        k ^= k >> 33;
+        return (size_t)k & mask;
+    }
+    void rehash(size_t want) {
+        size_t cap = 64;
+        while (cap < want) cap <<= 1;
+        std::vector<uint64_t> old;
+        for_each([&](uint64_t k) { old.push_back(k); });
+        keys_.assign(cap, 0);
+        stamp_.assign(cap, 0);
+        gen_ = 1;
+        size_ = 0;
+        for (uint64_t k : old) insert(k);
+    }
+};
 
 constexpr uint64_t kHashDefault = ~0ULL;   // Hash<>() (canonical.hh:42-45)
 constexpr int kSegWords = 7;
@@ -42,7 +108,7 @@ struct Dec {
 
 class Finder {
   public:
-    Finder(int K, std::unordered_set<uint64_t>& known, const uint64_t* prefix, uint64_t n_prefix)
+    Finder(int K, FlatSet& known, const uint64_t* prefix, uint64_t n_prefix)
         : K_((uint64_t)K), known_(known), prefix_(prefix), n_prefix_(n_prefix) {}
 
     // One read of n >= 1 windows.
@@ -58,7 +124,7 @@ class Finder {
         for (uint64_t pos = 0; pos < n; ++pos) {
             cur_hash = hv[pos];
             cur_new = counts[pos] == 0;
-            cur_seen = known_.count(cur_hash) != 0;
+            cur_seen = known_.contains(cur_hash);
             if (cur_new && !cur_seen) {
                 if (!prev_new || prev_seen) {
                     pre.push_back(cur);
@@ -73,24 +139,23 @@ class Finder {
             prev_new = cur_new;
             prev_seen = cur_seen;
         }
-        // presence of every neighbour in the graph or the (now final) set
-        present_.resize(n);
-        for (uint64_t i = 0; i < n; ++i) {
+        // presence of a window's neighbours in the graph or the (now final) set
+        auto present = [&](uint64_t i) {
             uint8_t m = mask[i];
             for (int c = 0; c < 8; ++c)
-                if (known_.count(nb[8 * i + c])) m |= (uint8_t)(1u << c);
-            present_[i] = m;
-        }
+                if (!((m >> c) & 1) && known_.contains(nb[8 * i + c])) m |= (uint8_t)(1u << c);
+            return m;
+        };
         if (cur_new && !cur_seen) {   // compactor.hh:401-417
             uint64_t right_flank = cur_hash;
-            const uint8_t r = present_[n - 1] >> 4;
+            const uint8_t r = present(n - 1) >> 4;
             if (__builtin_popcount(r) == 1) right_flank = nb[8 * (n - 1) + 4 + __builtin_ctz(r)];
             finish_segment(cur, n - 1, cur_hash, right_flank, pre);
         }
         pre.push_back(Seg());
         if (pre.size() < 3) return;
         if (pre[1].start == 0) {      // compactor.hh:425-437
-            const uint8_t l = present_[0] & 15u;
+            const uint8_t l = present(0) & 15u;
             pre[1].left_flank = __builtin_popcount(l) == 1 ? nb[__builtin_ctz(l)] : pre[1].left_anchor;
         }
         // cuts at decision k-mers (compactor.hh:439-532)
@@ -101,7 +166,7 @@ class Finder {
             }
             std::vector<Seg> ds;
             for (uint64_t pos = seg.start; pos + K_ <= seg.start + seg.length; ++pos) {
-                const uint8_t p = present_[pos];
+                const uint8_t p = present(pos);
                 if (__builtin_popcount(p & 15u) > 1 || __builtin_popcount(p >> 4) > 1) {
                     Dec d;
                     d.pos = pos;
@@ -123,12 +188,11 @@ class Finder {
 
   private:
     uint64_t K_;
-    std::unordered_set<uint64_t>& known_;
+    FlatSet& known_;
     const uint64_t* prefix_;
     uint64_t n_prefix_;
     const uint64_t* hv_ = nullptr;
     uint64_t n_ = 0;
-    std::vector<uint8_t> present_;
 
     // The caller's hash vector as goetia indexes it: its earlier entries,
     // then this read's; goetia reads past the end where a decision k-mer
@@ -212,6 +276,38 @@ class Finder {
     }
 };
 
+
+// Results of a run of consecutive reads.
+struct Part {
+    std::vector<Seg> segs;
+    std::vector<Dec> decs;
+    std::vector<uint64_t> seg_end, dec_end;   // per read, within the part
+};
+
+// Reads [r0, r1) whose windows start at w0; `set` is their new-k-mer set
+// (fresh per read unless shared).
+void run_reads(int K, const int64_t* offsets, uint64_t r0, uint64_t r1, uint64_t w0, const int16_t* counts,
+               const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present, const uint64_t* known,
+               uint64_t n_known, bool shared, const uint64_t* prefix, uint64_t n_prefix, FlatSet& set, Part& part) {
+    uint64_t w = w0;
+    for (uint64_t r = r0; r < r1; ++r) {
+        const int64_t len = offsets[r + 1] - offsets[r];
+        if (len >= K) {
+            const uint64_t n = (uint64_t)(len - K + 1);
+            if (!shared) {
+                set.clear();
+                set.reserve(n_known + n);
+                for (uint64_t i = 0; i < n_known; ++i) set.insert(known[i]);
+            }
+            Finder f(K, set, prefix, n_prefix);
+            f.run(counts + w, hashes + w, nb_values + 8 * w, nb_present + w, n, part.segs, part.decs);
+            w += n;
+        }
+        part.seg_end.push_back(part.segs.size());
+        part.dec_end.push_back(part.decs.size());
+    }
+}
+
 }  // namespace
 
 extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_t n_reads, const int16_t* counts,
@@ -227,56 +323,81 @@ extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_
     if (n_known && !known) return gt::set_error(GT_ERR_INVALID_ARG, "null known set");
     if (n_prefix && (!prefix || n_reads != 1))
         return gt::set_error(GT_ERR_INVALID_ARG, "a hash prefix needs exactly one read");
-    uint64_t nk = 0;
+    std::vector<uint64_t> wstart(n_reads + 1, 0);   // packed window index of each read
     for (uint64_t r = 0; r < n_reads; ++r) {
         const int64_t len = offsets[r + 1] - offsets[r];
         if (len < 0) return gt::set_error(GT_ERR_INVALID_ARG, "offsets must not decrease");
-        if (len >= K) nk += (uint64_t)(len - K + 1);
+        wstart[r + 1] = wstart[r] + (len >= K ? (uint64_t)(len - K + 1) : 0);
     }
+    const uint64_t nk = wstart[n_reads];
     if (nk && (!counts || !hashes || !nb_values || !nb_present))
         return gt::set_error(GT_ERR_INVALID_ARG, "null graph view");
-    std::unordered_set<uint64_t> base_known(known, known + n_known);
-    std::vector<Seg> out;
-    std::vector<Dec> ds;
-    uint64_t w = 0;
+
+    // Reads are independent unless they share one set: split them over
+    // threads by windows (at most 16, the host share of one GPU).
+    unsigned n_parts = 1;
+    if (!shared && nk >= (1u << 16)) {
+        n_parts = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+        n_parts = (unsigned)std::min<uint64_t>(n_parts, n_reads);
+    }
+    std::vector<uint64_t> cut(n_parts + 1, n_reads);
+    cut[0] = 0;
+    for (unsigned p = 1; p < n_parts; ++p)
+        cut[p] = (uint64_t)(std::lower_bound(wstart.begin(), wstart.end(), nk * p / n_parts) - wstart.begin());
+    for (unsigned p = 1; p <= n_parts; ++p) cut[p] = std::max(cut[p], cut[p - 1]);
+    std::vector<Part> parts(n_parts);
+    auto work = [&](unsigned p) {
+        FlatSet set;
+        if (shared)
+            for (uint64_t i = 0; i < n_known; ++i) set.insert(known[i]);
+        run_reads(K, offsets, cut[p], cut[p + 1], wstart[cut[p]], counts, hashes, nb_values, nb_present, known,
+                  n_known, shared != 0, prefix, n_prefix, set, parts[p]);
+    };
+    if (n_parts == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned p = 0; p < n_parts; ++p) th.emplace_back(work, p);
+        for (auto& t : th) t.join();
+    }
+    uint64_t total_s = 0, total_d = 0;
+    for (const Part& part : parts) {
+        total_s += part.segs.size();
+        total_d += part.decs.size();
+    }
+    *n_segs = total_s;
+    *n_decs = total_d;
+    if (total_s > seg_cap || total_d > dec_cap || (total_s && !segs) || (total_d && !decs))
+        return gt::set_error(GT_ERR_INVALID_ARG, "segment outputs too small: need " + std::to_string(total_s) +
+                                                      " segments and " + std::to_string(total_d) + " decisions");
+    uint64_t s0 = 0, d0 = 0, r = 0;
     seg_off[0] = 0;
     dec_off[0] = 0;
-    for (uint64_t r = 0; r < n_reads; ++r) {
-        const int64_t len = offsets[r + 1] - offsets[r];
-        if (len >= K) {
-            const uint64_t n = (uint64_t)(len - K + 1);
-            std::unordered_set<uint64_t> fresh;
-            if (!shared) fresh = base_known;
-            std::unordered_set<uint64_t>& set = shared ? base_known : fresh;
-            Finder f(K, set, prefix, n_prefix);
-            f.run(counts + w, hashes + w, nb_values + 8 * w, nb_present + w, n, out, ds);
-            w += n;
+    for (const Part& part : parts) {
+        for (size_t i = 0; i < part.segs.size(); ++i) {
+            const Seg& sg = part.segs[i];
+            uint64_t* o = segs + kSegWords * (s0 + i);
+            o[0] = sg.left_anchor;
+            o[1] = sg.right_anchor;
+            o[2] = sg.left_flank;
+            o[3] = sg.right_flank;
+            o[4] = sg.decision ? 1 : 0;
+            o[5] = sg.start;
+            o[6] = sg.length;
         }
-        seg_off[r + 1] = (int64_t)out.size();
-        dec_off[r + 1] = (int64_t)ds.size();
-    }
-    *n_segs = out.size();
-    *n_decs = ds.size();
-    if (out.size() > seg_cap || ds.size() > dec_cap || (out.size() && !segs) || (ds.size() && !decs))
-        return gt::set_error(GT_ERR_INVALID_ARG, "segment outputs too small: need " + std::to_string(out.size()) +
-                                                      " segments and " + std::to_string(ds.size()) + " decisions");
-    for (size_t i = 0; i < out.size(); ++i) {
-        const Seg& s = out[i];
-        uint64_t* o = segs + kSegWords * i;
-        o[0] = s.left_anchor;
-        o[1] = s.right_anchor;
-        o[2] = s.left_flank;
-        o[3] = s.right_flank;
-        o[4] = s.decision ? 1 : 0;
-        o[5] = s.start;
-        o[6] = s.length;
-    }
-    for (size_t i = 0; i < ds.size(); ++i) {
-        uint64_t* o = decs + kDecWords * i;
-        o[0] = ds[i].pos;
-        o[1] = ds[i].hash;
-        o[2] = ds[i].present;
-        std::memcpy(o + 3, ds[i].nb, sizeof(ds[i].nb));
+        for (size_t i = 0; i < part.decs.size(); ++i) {
+            uint64_t* o = decs + kDecWords * (d0 + i);
+            o[0] = part.decs[i].pos;
+            o[1] = part.decs[i].hash;
+            o[2] = part.decs[i].present;
+            std::memcpy(o + 3, part.decs[i].nb, sizeof(part.decs[i].nb));
+        }
+        for (size_t k = 0; k < part.seg_end.size(); ++k, ++r) {
+            seg_off[r + 1] = (int64_t)(s0 + part.seg_end[k]);
+            dec_off[r + 1] = (int64_t)(d0 + part.dec_end[k]);
+        }
+        s0 += part.segs.size();
+        d0 += part.decs.size();
     }
     return GT_OK;
 }

@@ -118,3 +118,17 @@ def neighbors_5m(scale=1.0):
     max_table = 2 ** 28 if scale >= 1.0 else max(100_000, int(2 ** 28 * scale))
     return dict(K=31, storage='ByteStorage', max_table=max_table, n_tables=4, mode=0,
                 min_count=2), ReadBatch(seq, off)
+
+
+def segments_5m(scale=1.0):
+    """find_new_segments against a fixed graph: a 5 Mbp genome with 100
+    copies of a 2 kb repeat and a 0.1% SNP haplotype; 400k x 150 bp reads of
+    the first haplotype at 0.5% error make the graph (K=31, forward hashes,
+    the exact set, as goetia instantiates its compactor), then 400k x 150 bp
+    reads of the second haplotype at 1% error are cut into segments."""
+    L = _scaled(5_000_000, scale, 100_000)
+    h1, h2 = repeat_genome(6, L, 2000, _scaled(100, scale, 5), 0.001)
+    n = _scaled(400_000, scale)
+    bg = reads(1006, h1, n, 150, 150, 0.005, 0.0)
+    q = reads(2006, h2, n, 150, 150, 0.01, 0.0)
+    return dict(K=31, storage='SparseppSetStorage', mode=0), bg, q

@@ -80,3 +80,49 @@ def test_one_argument_form_and_conveniences(oracle):
     assert segs[1].sequence(core) == core[:pivot + K - 1]
     assert segs[3].sequence(core) == core[pivot + 1:]
     assert d.sequence(core) == core[pivot:pivot + K]
+
+
+def test_batch_split_over_threads_equals_one_read_at_a_time():
+    """A batch large enough to be split over host threads gives, read for
+    read, what one call per read gives (random graph views with runs of new
+    k-mers, neighbours present in the graph or among the read's new k-mers)."""
+    import ctypes as C
+
+    from goetia_amd import _native as N
+    from goetia_amd.cdbg import _Outputs
+    rng = np.random.default_rng(3)
+    K, n_reads = 21, 1500
+    lens = rng.integers(10, 260, n_reads)
+    off = np.zeros(n_reads + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    wins = np.maximum(lens - K + 1, 0)
+    nk = int(wins.sum())
+    assert nk >= 1 << 16
+    counts = (rng.random(nk) < 0.3).astype(np.int16)          # 0 = new
+    runs = np.repeat(rng.random(nk // 8 + 1) < 0.5, 8)[:nk]
+    counts[runs] = 0
+    hv = rng.integers(0, 1 << 20, nk, dtype=np.uint64)          # small range: repeats within reads
+    vals = rng.integers(0, 1 << 20, (nk, 8), dtype=np.uint64)
+    mask = (rng.random((nk, 8)) < 0.2).astype(np.uint8) @ (1 << np.arange(8)).astype(np.uint8)
+    mask = mask.astype(np.uint8)
+    whole = _Outputs(n_reads, nk)
+    whole.call('gt_segments_from_view', K, N.ptr(off), n_reads, N.ptr(counts), N.ptr(hv), N.ptr(vals), N.ptr(mask),
+               None, 0, 0, None, 0)
+    w = 0
+    n_seg = n_dec = 0
+    for r in range(n_reads):
+        n = int(wins[r])
+        one = _Outputs(1, max(n, 1))
+        o1 = np.array([0, lens[r]], np.int64)
+        one.call('gt_segments_from_view', K, N.ptr(o1), 1, N.ptr(counts[w:w + n]), N.ptr(hv[w:w + n]),
+                 N.ptr(np.ascontiguousarray(vals[w:w + n])), N.ptr(mask[w:w + n]), None, 0, 0, None, 0)
+        a, b = int(whole.seg_off[r]), int(whole.seg_off[r + 1])
+        assert b - a == int(one.seg_off[1])
+        np.testing.assert_array_equal(whole.segs[7 * a:7 * b], one.segs[:7 * (b - a)])
+        a, b = int(whole.dec_off[r]), int(whole.dec_off[r + 1])
+        assert b - a == int(one.dec_off[1])
+        np.testing.assert_array_equal(whole.decs[11 * a:11 * b], one.decs[:11 * (b - a)])
+        n_seg += int(one.seg_off[1])
+        n_dec += int(one.dec_off[1])
+        w += n
+    assert n_seg == whole.n_segs.value and n_dec == whole.n_decs.value and n_dec > 0
