@@ -611,12 +611,12 @@ bool use_binned(const gt_storage* st, const Scratch& sc, const Reads& R) {
 }
 
 // Binned sub-batch in start positions, a multiple of kTileMax: 2^27 (or
-// GOETIA_AMD_BIN_SUB, at most 2^28), capped by GOETIA_AMD_SUB_BATCH.
+// GOETIA_AMD_BIN_SUB, at most 2^29), capped by GOETIA_AMD_SUB_BATCH.
 uint64_t bin_sub_batch(uint64_t n_bases) {
     uint64_t sb = 1ULL << 27;
     if (const char* v = std::getenv("GOETIA_AMD_BIN_SUB")) {
         const unsigned long long x = std::strtoull(v, nullptr, 10);
-        if (x) sb = std::min<uint64_t>(x, 1ULL << 28);
+        if (x) sb = std::min<uint64_t>(x, 1ULL << 29);
     }
     if (const uint64_t cap = sub_batch_cap()) sb = std::min(sb, cap);
     sb = std::max<uint64_t>((sb + kTileMax - 1) / kTileMax * kTileMax, kTileMax);
