@@ -610,10 +610,12 @@ bool use_binned(const gt_storage* st, const Scratch& sc, const Reads& R) {
     return (uint64_t)R.n_bases >= (1ULL << 20) && (uint64_t)R.n_bases * 8 >= bins;
 }
 
-// Binned sub-batch in start positions, a multiple of kTileMax: 2^27 (or
-// GOETIA_AMD_BIN_SUB, at most 2^29), capped by GOETIA_AMD_SUB_BATCH.
-uint64_t bin_sub_batch(uint64_t n_bases) {
-    uint64_t sb = 1ULL << 27;
+// Binned sub-batch in start positions, a multiple of kTileMax: 2^27 for an
+// insert, 2^26 for insert_and_query (its four result bytes per window are
+// written scattered, and 2^26 windows keep them in a 256 MiB range), or
+// GOETIA_AMD_BIN_SUB (at most 2^29); capped by GOETIA_AMD_SUB_BATCH.
+uint64_t bin_sub_batch(uint64_t n_bases, bool iq) {
+    uint64_t sb = iq ? 1ULL << 26 : 1ULL << 27;
     if (const char* v = std::getenv("GOETIA_AMD_BIN_SUB")) {
         const unsigned long long x = std::strtoull(v, nullptr, 10);
         if (x) sb = std::min<uint64_t>(x, 1ULL << 29);
@@ -697,10 +699,10 @@ void bin_route(const Reads& R, const Tables& T, BinPlan& P, Scratch& sc, int64_t
 int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_new, uint8_t* new_flags,
                        int16_t* iq_counts, hipStream_t s) {
     const uint32_t nt = st->n_tables;
-    const uint64_t sub = bin_sub_batch((uint64_t)R.n_bases);
+    const bool iq = iq_counts != nullptr;
+    const uint64_t sub = bin_sub_batch((uint64_t)R.n_bases, iq);
     BinPlan P;
     GT_TRY(bin_prepare(st, sc, sub / kBinTile, P));
-    const bool iq = iq_counts != nullptr;
     uint8_t* flags = new_flags;
     if (iq) {
         GT_TRY(sc.bin_vq.ensure((size_t)R.n_bases * kBinMaxTables));
