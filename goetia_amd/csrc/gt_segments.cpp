@@ -408,6 +408,8 @@ extern "C" int gt_find_new_segments(gt_storage* st, uint16_t K, int mode, const 
                                     int64_t* seg_off, uint64_t* segs, uint64_t seg_cap, int64_t* dec_off,
                                     uint64_t* decs, uint64_t dec_cap, uint64_t* n_segs, uint64_t* n_decs) {
     if (K < 1 || K > 255) return gt::set_error(GT_ERR_INVALID_ARG, "K must be in 1..255");
+    if (mode != GT_MODE_RAW)
+        return gt::set_error(GT_ERR_UNSUPPORTED, "find_new_segments hashes raw bytes (GT_MODE_RAW), as goetia does");
     if (n_reads && !offsets) return gt::set_error(GT_ERR_INVALID_ARG, "null offsets");
     uint64_t nk = 0;
     for (uint64_t r = 0; r < n_reads; ++r) {

@@ -227,9 +227,11 @@ int gt_dbg_neighbors(gt_storage* st, uint16_t K, int canonical, int mode, int mi
 
 /* StreamingCompactor<dBG<S, FwdLemireShifter>>::Compactor::find_new_segments
  * (compactor.hh:321-534) over a batch of reads against the graph as it is
- * (nothing is inserted).  Forward hashes (goetia builds the compactor for
- * Hash<> only).  Each read starts from the new-k-mer set `known` (the
- * one-argument overload: n_known 0), or with shared != 0 one set runs through
+ * (nothing is inserted).  Forward hashes of raw bytes (goetia builds the
+ * compactor for Hash<> only and its k-mer iterator hashes every byte), so
+ * mode must be GT_MODE_RAW (GT_ERR_UNSUPPORTED otherwise).  Each read
+ * starts from the new-k-mer set `known` (the one-argument overload:
+ * n_known 0), or with shared != 0 one set runs through
  * the reads in order (a caller-owned set across calls).  prefix / n_prefix:
  * hashes already in the caller's hash vector, which goetia indexes from its
  * front (one read only).  Per window: counts (query) and hashes (value) out.

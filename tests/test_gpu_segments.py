@@ -108,3 +108,10 @@ def test_canonical_graphs_refused():
         StreamingCompactor[G]
     with pytest.raises(TypeError):
         StreamingCompactor.Compactor.build(G(lg.SparseppSetStorage.build(), 21))
+
+
+def test_skip_n_mode_refused():
+    G = lg.dBG[lg.SparseppSetStorage, lg.FwdLemireShifter]
+    g = G(lg.SparseppSetStorage.build(), 21, mode=N.MODE_SKIP_N)
+    with pytest.raises(N.UnsupportedError):
+        StreamingCompactor[G].Compactor.build(g).find_new_segments('ACGT' * 10)
