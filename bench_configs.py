@@ -80,7 +80,7 @@ def exact(args):
     import torch
     from goetia_amd import libgoetia as lg
     from goetia_amd import workload
-    from goetia_amd.pipeline import Pipeline, autotune, tune_get
+    from goetia_amd.pipeline import Pipeline, autotune
     from goetia_amd.reads import ReadBatch
     t0 = time.perf_counter()
     cfg, batch = workload.exact_mixed(args.scale)
@@ -232,7 +232,6 @@ def fastx(args):
     from goetia_amd import _native as N
     from goetia_amd import libgoetia as lg
     from goetia_amd import workload
-    from goetia_amd.parsing import FastxParser
     n_reads = max(1000, int(5_000_000 * args.scale))
     g = workload.genome(2, 50_000_000)
     batch = workload.reads(2, g, n_reads, 150, 150, 0.005, 0.0)

@@ -13,10 +13,9 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .hashing import (CanLemireShifter, Canonical, DNA_SIMPLE, FwdLemireShifter, Hash,
-                      KmerIterator, _LemireShifter, _bytes)
+from .hashing import CanLemireShifter, DNA_SIMPLE, Hash, KmerIterator, _LemireShifter, _bytes
 from .reads import DeviceReads, ReadBatch
-from .storage import SparseppSetStorage, _Storage
+from .storage import _Storage
 
 
 class Shift:

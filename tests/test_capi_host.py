@@ -1,8 +1,6 @@
 """CPU checks of the C ABI library: it loads, exports every entry point of
 include/goetia_amd.h, its host-only functions agree with the oracle, and its
 device entry points fail loudly (no fallback) when no GPU is visible."""
-import ctypes as C
-
 import numpy as np
 import pytest
 

@@ -2,7 +2,6 @@
 golden vectors of the compiled reference.  Integer work, so every
 comparison is bit-exact."""
 import hashlib
-import os
 
 import numpy as np
 import pytest

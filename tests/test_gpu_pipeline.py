@@ -6,7 +6,6 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from goetia_amd import libgoetia as lg  # noqa: E402
-from goetia_amd import _native as N  # noqa: E402
 from goetia_amd import workload  # noqa: E402
 from goetia_amd.pipeline import Pipeline, autotune, tune_get, tune_set  # noqa: E402
 from goetia_amd.reads import ReadBatch  # noqa: E402

@@ -86,8 +86,6 @@ def test_batch_split_over_threads_equals_one_read_at_a_time():
     """A batch large enough to be split over host threads gives, read for
     read, what one call per read gives (random graph views with runs of new
     k-mers, neighbours present in the graph or among the read's new k-mers)."""
-    import ctypes as C
-
     from goetia_amd import _native as N
     from goetia_amd.cdbg import _Outputs
     rng = np.random.default_rng(3)
