@@ -371,8 +371,8 @@ extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_
         return gt::set_error(GT_ERR_INVALID_ARG, "segment outputs too small: need " + std::to_string(total_s) +
                                                       " segments and " + std::to_string(total_d) + " decisions");
     uint64_t s0 = 0, d0 = 0, r = 0;
-    seg_off[0] = 0;
-    dec_off[0] = 0;
+    if (seg_off) seg_off[0] = 0;
+    if (dec_off) dec_off[0] = 0;
     for (const Part& part : parts) {
         for (size_t i = 0; i < part.segs.size(); ++i) {
             const Seg& sg = part.segs[i];

@@ -124,3 +124,21 @@ def test_batch_split_over_threads_equals_one_read_at_a_time():
         n_dec += int(one.dec_off[1])
         w += n
     assert n_seg == whole.n_segs.value and n_dec == whole.n_decs.value and n_dec > 0
+
+
+def test_empty_batch_and_bad_arguments():
+    import ctypes as C
+
+    from goetia_amd import _native as N
+    ns, nd = C.c_uint64(), C.c_uint64()
+    N.call('gt_segments_from_view', 21, None, 0, None, None, None, None, None, 0, 0, None, 0, None, None, 0, None,
+           None, 0, C.byref(ns), C.byref(nd))
+    assert (ns.value, nd.value) == (0, 0)
+    off = np.array([0, 50], np.int64)
+    with pytest.raises(ValueError):      # K out of range
+        N.call('gt_segments_from_view', 0, N.ptr(off), 1, None, None, None, None, None, 0, 0, None, 0, None, None, 0,
+               None, None, 0, C.byref(ns), C.byref(nd))
+    with pytest.raises(ValueError):      # windows but no graph view
+        seg_off = np.zeros(2, np.int64)
+        N.call('gt_segments_from_view', 21, N.ptr(off), 1, None, None, None, None, None, 0, 0, None, 0,
+               N.ptr(seg_off), None, 0, N.ptr(seg_off), None, 0, C.byref(ns), C.byref(nd))
