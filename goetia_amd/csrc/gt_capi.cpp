@@ -225,6 +225,7 @@ struct gt_storage {
     uint64_t ecount_ub = 0;            // host upper bound on the set size
     Scratch own;                       // default scratch of calls on this storage
     DevBuf h_seq, h_off, h_out1, h_out2, h_out3, h_out4;   // staging of host-pointer calls
+    DevBuf seg_nb, seg_pres, seg_idx, seg_nbc, seg_pc;     // find_new_segments' window view
     DevBuf solid_counts;               // insert_and_query counts of a solid-filter batch
     // batched diginorm: read of every routed event and window, windows with
     // events, windows failing the cutoff, per-read state, iteration control
@@ -1660,6 +1661,65 @@ int gt_dbg_degrees(gt_storage* st, uint16_t K, int canonical, int mode, int min_
     GT_HIP(hipStreamSynchronize(s));
     return GT_OK;
 }
+
+}  // extern "C"
+
+// find_new_segments' graph view (gt_segments.cpp): every window's count and
+// hash (forward, raw bytes), and the eight neighbour values and presence mask
+// of the windows whose count is 0 -- the only ones its passes look at --
+// gathered on the device so that only they cross to the host.
+namespace gt {
+int segment_view(gt_storage* st, uint16_t K, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                 int16_t* counts, uint64_t* hashes, std::vector<uint64_t>& nbc, std::vector<uint8_t>& pc) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_host_offsets(offsets, n_reads));
+    const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
+    GT_TRY(check_read_args(K, kRaw, seq, offsets, n_reads, n_bases));
+    GT_TRY(ensure_device());
+    nbc.clear();
+    pc.clear();
+    const uint64_t nk = packed_kmers(offsets, n_reads, K);
+    if (nk == 0) return GT_OK;
+    hipStream_t s = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
+    GT_TRY(stage_in(st->h_off, offsets, n_reads + 1, s));
+    GT_TRY(st->h_out1.ensure(nk * sizeof(int16_t)));
+    GT_TRY(st->h_out3.ensure(nk * sizeof(uint64_t)));
+    GT_TRY(st->h_out2.ensure(2 * nk));
+    GT_TRY(st->seg_nb.ensure(nk * 8 * sizeof(uint64_t)));
+    GT_TRY(st->seg_pres.ensure(nk));
+    const uint8_t* dseq = st->h_seq.as<uint8_t>();
+    const int64_t* doff = st->h_off.as<int64_t>();
+    GT_TRY(gt_dbg_query_dev(st, K, 0, kRaw, dseq, doff, n_reads, n_bases, st->h_out1.as<int16_t>(),
+                            st->h_out3.as<uint64_t>(), s));
+    GT_TRY(gt_dbg_neighbors_dev(st, K, 0, kRaw, 1, dseq, doff, n_reads, n_bases, st->h_out2.as<uint8_t>(),
+                                st->h_out2.as<uint8_t>() + nk, st->seg_nb.as<uint64_t>(), st->seg_pres.as<uint8_t>(),
+                                s));
+    GT_TRY(stage_out(counts, st->h_out1, nk, s));
+    GT_TRY(stage_out(hashes, st->h_out3, nk, s));
+    GT_HIP(hipStreamSynchronize(s));
+    std::vector<uint64_t> idx;
+    for (uint64_t w = 0; w < nk; ++w)
+        if (counts[w] == 0) idx.push_back(w);
+    const uint64_t n_new = idx.size();
+    if (n_new == 0) return GT_OK;
+    nbc.resize(n_new * 8);
+    pc.resize(n_new);
+    GT_TRY(stage_in(st->seg_idx, idx.data(), n_new, s));
+    GT_TRY(st->seg_nbc.ensure(n_new * 8 * sizeof(uint64_t)));
+    GT_TRY(st->seg_pc.ensure(n_new));
+    launch_seg_gather(st->seg_nb.as<uint64_t>(), st->seg_pres.as<uint8_t>(), st->seg_idx.as<uint64_t>(), n_new,
+                      st->seg_nbc.as<uint64_t>(), st->seg_pc.as<uint8_t>(), s);
+    GT_HIP(hipGetLastError());
+    GT_TRY(stage_out(nbc.data(), st->seg_nbc, n_new * 8, s));
+    GT_TRY(stage_out(pc.data(), st->seg_pc, n_new, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+}  // namespace gt
+
+extern "C" {
 
 // ------------------------------------------------------------ per-hash
 

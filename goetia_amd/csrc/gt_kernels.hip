@@ -1724,6 +1724,18 @@ __global__ __launch_bounds__(kBlock) void bin_read_new_kernel(const int64_t* kme
     add_counter(counters + kNUnique, tot);
 }
 
+// out[j] = the eight neighbour values and the presence mask of window idx[j]
+// (every idx[j] is below the view's window count).
+__global__ void seg_gather_kernel(const uint64_t* nb, const uint8_t* mask, const uint64_t* idx, uint64_t n,
+                                  uint64_t* out_nb, uint8_t* out_mask) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t w = idx[j];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) out_nb[8 * j + c] = nb[8 * w + c];
+        out_mask[j] = mask[w];
+    }
+}
+
 // insert_and_query results from the per-table counts after each event:
 // count = min over tables (the reference returns 1 for a new k-mer, and a
 // k-mer is new exactly when that minimum is 1); windows without events (N
@@ -2306,6 +2318,13 @@ void launch_dn_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int 
 void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t s) {
     const uint64_t g = (max_n + 255) / 256;
     zero_bytes_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(p, n);
+}
+
+void launch_seg_gather(const uint64_t* nb, const uint8_t* mask, const uint64_t* idx, uint64_t n, uint64_t* out_nb,
+                       uint8_t* out_mask, hipStream_t s) {
+    const uint64_t g = (n + 255) / 256;
+    seg_gather_kernel<<<(unsigned)(g < 16384 ? (g ? g : 1) : 16384), 256, 0, s>>>(nb, mask, idx, n, out_nb,
+                                                                               out_mask);
 }
 
 void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,

@@ -67,6 +67,8 @@ void launch_zero_bytes(uint8_t* p, const int64_t* n, uint64_t max_n, hipStream_t
 void launch_zero_words(uint32_t* p, const int64_t* n, uint64_t max_n, hipStream_t s);
 void launch_solid_reads(const int64_t* off, const int64_t* kmer_off, uint64_t n_reads, int K, const int16_t* counts,
                         uint32_t threshold, float min_prop_solid, uint8_t* passed, hipStream_t s);
+void launch_seg_gather(const uint64_t* nb, const uint8_t* mask, const uint64_t* idx, uint64_t n, uint64_t* out_nb,
+                       uint8_t* out_mask, hipStream_t s);
 void launch_bin_iq_combine(const uint32_t* vq, const int64_t* nk, uint64_t max_n, uint32_t n_tables, int16_t* counts,
                            uint64_t* counters, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);

@@ -1,13 +1,15 @@
 // StreamingCompactor<dBG<S, FwdLemireShifter>>::Compactor::find_new_segments
 // (goetia compactor.hh:321-534) over a batch of reads.
 //
-// The graph view of a read -- every window's query count, hash and eight
-// neighbour values with their presence bits -- comes from two GPU launches
-// (gt_dbg_query, gt_dbg_neighbors) for the whole batch; the two passes over
-// each read (runs of new k-mers, then the cuts at decision k-mers with
-// goetia's anchor and flank bookkeeping) are sequential per read and run
-// here on the host.  gt_segments_from_view takes the view from the caller
-// (no device needed); gt_find_new_segments builds it on the GPU first.
+// The graph view of a batch -- every window's query count and hash, and the
+// eight neighbour values with their presence bits of every new window (count
+// 0, the only ones the passes look at) -- comes from three GPU launches for
+// the whole batch (query, neighbours, gather of the new windows; see
+// gt::segment_view).  The two passes over each read (runs of new k-mers, then
+// the cuts at decision k-mers with goetia's anchor and flank bookkeeping) are
+// sequential per read and run here on the host, independent reads on several
+// threads.  gt_segments_from_view takes a full per-window view from the
+// caller (no device needed); gt_find_new_segments builds it on the GPU.
 
 #include "goetia_amd.h"
 
@@ -19,6 +21,10 @@
 
 namespace gt {
 int set_error(int code, const std::string& msg);
+// gt_capi.cpp: counts and hashes of every window, neighbour values and
+// presence of the new ones (count 0), in window order.
+int segment_view(gt_storage* st, uint16_t K, const uint8_t* seq, const int64_t* offsets, uint64_t n_reads,
+                 int16_t* counts, uint64_t* hashes, std::vector<uint64_t>& nbc, std::vector<uint8_t>& pc);
 }
 
 namespace {
@@ -111,11 +117,22 @@ class Finder {
     Finder(int K, FlatSet& known, const uint64_t* prefix, uint64_t n_prefix)
         : K_((uint64_t)K), known_(known), prefix_(prefix), n_prefix_(n_prefix) {}
 
-    // One read of n >= 1 windows.
-    void run(const int16_t* counts, const uint64_t* hv, const uint64_t* nb, const uint8_t* mask, uint64_t n,
-             std::vector<Seg>& out, std::vector<Dec>& decs) {
+    // One read of n >= 1 windows.  nb / mask: the neighbour values and
+    // presence of every window, or with `compact` of its new windows (count
+    // 0) only, in window order -- the only ones the passes look at.  Returns
+    // the number of new windows.
+    uint64_t run(const int16_t* counts, const uint64_t* hv, const uint64_t* nb, const uint8_t* mask, uint64_t n,
+                 bool compact, std::vector<Seg>& out, std::vector<Dec>& decs) {
         hv_ = hv;
         n_ = n;
+        uint64_t n_new = 0;
+        if (compact) {
+            cidx_.resize(n);
+            for (uint64_t i = 0; i < n; ++i) cidx_[i] = counts[i] == 0 ? n_new++ : 0;
+        } else {
+            for (uint64_t i = 0; i < n; ++i) n_new += counts[i] == 0;
+        }
+        auto at = [&](uint64_t i) { return compact ? cidx_[i] : i; };
         // runs of new, not yet seen k-mers (compactor.hh:352-400)
         std::vector<Seg> pre;
         Seg cur;
@@ -141,22 +158,23 @@ class Finder {
         }
         // presence of a window's neighbours in the graph or the (now final) set
         auto present = [&](uint64_t i) {
-            uint8_t m = mask[i];
+            const uint64_t j = at(i);
+            uint8_t m = mask[j];
             for (int c = 0; c < 8; ++c)
-                if (!((m >> c) & 1) && known_.contains(nb[8 * i + c])) m |= (uint8_t)(1u << c);
+                if (!((m >> c) & 1) && known_.contains(nb[8 * j + c])) m |= (uint8_t)(1u << c);
             return m;
         };
         if (cur_new && !cur_seen) {   // compactor.hh:401-417
             uint64_t right_flank = cur_hash;
             const uint8_t r = present(n - 1) >> 4;
-            if (__builtin_popcount(r) == 1) right_flank = nb[8 * (n - 1) + 4 + __builtin_ctz(r)];
+            if (__builtin_popcount(r) == 1) right_flank = nb[8 * at(n - 1) + 4 + __builtin_ctz(r)];
             finish_segment(cur, n - 1, cur_hash, right_flank, pre);
         }
         pre.push_back(Seg());
-        if (pre.size() < 3) return;
+        if (pre.size() < 3) return n_new;
         if (pre[1].start == 0) {      // compactor.hh:425-437
             const uint8_t l = present(0) & 15u;
-            pre[1].left_flank = __builtin_popcount(l) == 1 ? nb[__builtin_ctz(l)] : pre[1].left_anchor;
+            pre[1].left_flank = __builtin_popcount(l) == 1 ? nb[8 * at(0) + __builtin_ctz(l)] : pre[1].left_anchor;
         }
         // cuts at decision k-mers (compactor.hh:439-532)
         for (Seg seg : pre) {
@@ -172,7 +190,7 @@ class Finder {
                     d.pos = pos;
                     d.hash = hv[pos];
                     d.present = p;
-                    std::memcpy(d.nb, nb + 8 * pos, sizeof(d.nb));
+                    std::memcpy(d.nb, nb + 8 * at(pos), sizeof(d.nb));
                     decs.push_back(d);
                     Seg s = init_segment(hv[pos], hv[pos], pos);
                     s.length = K_;
@@ -184,6 +202,7 @@ class Finder {
             }
             split(seg, ds, out);
         }
+        return n_new;
     }
 
   private:
@@ -193,6 +212,7 @@ class Finder {
     uint64_t n_prefix_;
     const uint64_t* hv_ = nullptr;
     uint64_t n_ = 0;
+    std::vector<uint64_t> cidx_;
 
     // The caller's hash vector as goetia indexes it: its earlier entries,
     // then this read's; goetia reads past the end where a decision k-mer
@@ -284,12 +304,14 @@ struct Part {
     std::vector<uint64_t> seg_end, dec_end;   // per read, within the part
 };
 
-// Reads [r0, r1) whose windows start at w0; `set` is their new-k-mer set
-// (fresh per read unless shared).
-void run_reads(int K, const int64_t* offsets, uint64_t r0, uint64_t r1, uint64_t w0, const int16_t* counts,
-               const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present, const uint64_t* known,
-               uint64_t n_known, bool shared, const uint64_t* prefix, uint64_t n_prefix, FlatSet& set, Part& part) {
-    uint64_t w = w0;
+// Reads [r0, r1) whose windows start at w0 (and, with `compact`, whose new
+// windows start at c0 in nb_values / nb_present); `set` is their new-k-mer
+// set (fresh per read unless shared).
+void run_reads(int K, const int64_t* offsets, uint64_t r0, uint64_t r1, uint64_t w0, uint64_t c0,
+               const int16_t* counts, const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present,
+               bool compact, const uint64_t* known, uint64_t n_known, bool shared, const uint64_t* prefix,
+               uint64_t n_prefix, FlatSet& set, Part& part) {
+    uint64_t w = w0, c = c0;
     for (uint64_t r = r0; r < r1; ++r) {
         const int64_t len = offsets[r + 1] - offsets[r];
         if (len >= K) {
@@ -300,7 +322,8 @@ void run_reads(int K, const int64_t* offsets, uint64_t r0, uint64_t r1, uint64_t
                 for (uint64_t i = 0; i < n_known; ++i) set.insert(known[i]);
             }
             Finder f(K, set, prefix, n_prefix);
-            f.run(counts + w, hashes + w, nb_values + 8 * w, nb_present + w, n, part.segs, part.decs);
+            const uint64_t v = compact ? c : w;
+            c += f.run(counts + w, hashes + w, nb_values + 8 * v, nb_present + v, n, compact, part.segs, part.decs);
             w += n;
         }
         part.seg_end.push_back(part.segs.size());
@@ -308,14 +331,13 @@ void run_reads(int K, const int64_t* offsets, uint64_t r0, uint64_t r1, uint64_t
     }
 }
 
-}  // namespace
-
-extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_t n_reads, const int16_t* counts,
-                                     const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present,
-                                     const uint64_t* known, uint64_t n_known, int shared, const uint64_t* prefix,
-                                     uint64_t n_prefix, int64_t* seg_off, uint64_t* segs, uint64_t seg_cap,
-                                     int64_t* dec_off, uint64_t* decs, uint64_t dec_cap, uint64_t* n_segs,
-                                     uint64_t* n_decs) {
+// gt_segments_from_view, with the neighbour view either per window or, with
+// `compact`, for the new windows (count 0) only, in window order.
+int segments_core(uint16_t K, const int64_t* offsets, uint64_t n_reads, const int16_t* counts,
+                  const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present, bool compact,
+                  const uint64_t* known, uint64_t n_known, int shared, const uint64_t* prefix, uint64_t n_prefix,
+                  int64_t* seg_off, uint64_t* segs, uint64_t seg_cap, int64_t* dec_off, uint64_t* decs,
+                  uint64_t dec_cap, uint64_t* n_segs, uint64_t* n_decs) {
     if (K < 1 || K > 255) return gt::set_error(GT_ERR_INVALID_ARG, "K must be in 1..255");
     if (n_reads && (!offsets || !seg_off || !dec_off))
         return gt::set_error(GT_ERR_INVALID_ARG, "null offsets or outputs");
@@ -330,7 +352,8 @@ extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_
         wstart[r + 1] = wstart[r] + (len >= K ? (uint64_t)(len - K + 1) : 0);
     }
     const uint64_t nk = wstart[n_reads];
-    if (nk && (!counts || !hashes || !nb_values || !nb_present))
+    // (a compact view is empty when no window is new)
+    if (nk && (!counts || !hashes || (!compact && (!nb_values || !nb_present))))
         return gt::set_error(GT_ERR_INVALID_ARG, "null graph view");
 
     // Reads are independent unless they share one set: split them over
@@ -345,13 +368,22 @@ extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_
     for (unsigned p = 1; p < n_parts; ++p)
         cut[p] = (uint64_t)(std::lower_bound(wstart.begin(), wstart.end(), nk * p / n_parts) - wstart.begin());
     for (unsigned p = 1; p <= n_parts; ++p) cut[p] = std::max(cut[p], cut[p - 1]);
+    // with a compact view, each part's first new window
+    std::vector<uint64_t> cbase(n_parts, 0);
+    if (compact) {
+        uint64_t c = 0;
+        for (unsigned p = 0; p < n_parts; ++p) {
+            cbase[p] = c;
+            for (uint64_t w = wstart[cut[p]]; w < wstart[cut[p + 1]]; ++w) c += counts[w] == 0;
+        }
+    }
     std::vector<Part> parts(n_parts);
     auto work = [&](unsigned p) {
         FlatSet set;
         if (shared)
             for (uint64_t i = 0; i < n_known; ++i) set.insert(known[i]);
-        run_reads(K, offsets, cut[p], cut[p + 1], wstart[cut[p]], counts, hashes, nb_values, nb_present, known,
-                  n_known, shared != 0, prefix, n_prefix, set, parts[p]);
+        run_reads(K, offsets, cut[p], cut[p + 1], wstart[cut[p]], cbase[p], counts, hashes, nb_values, nb_present,
+                  compact, known, n_known, shared != 0, prefix, n_prefix, set, parts[p]);
     };
     if (n_parts == 1) {
         work(0);
@@ -402,6 +434,18 @@ extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_
     return GT_OK;
 }
 
+}  // namespace
+
+extern "C" int gt_segments_from_view(uint16_t K, const int64_t* offsets, uint64_t n_reads, const int16_t* counts,
+                                     const uint64_t* hashes, const uint64_t* nb_values, const uint8_t* nb_present,
+                                     const uint64_t* known, uint64_t n_known, int shared, const uint64_t* prefix,
+                                     uint64_t n_prefix, int64_t* seg_off, uint64_t* segs, uint64_t seg_cap,
+                                     int64_t* dec_off, uint64_t* decs, uint64_t dec_cap, uint64_t* n_segs,
+                                     uint64_t* n_decs) {
+    return segments_core(K, offsets, n_reads, counts, hashes, nb_values, nb_present, false, known, n_known, shared,
+                         prefix, n_prefix, seg_off, segs, seg_cap, dec_off, decs, dec_cap, n_segs, n_decs);
+}
+
 extern "C" int gt_find_new_segments(gt_storage* st, uint16_t K, int mode, const uint8_t* seq, const int64_t* offsets,
                                     uint64_t n_reads, const uint64_t* known, uint64_t n_known, int shared,
                                     const uint64_t* prefix, uint64_t n_prefix, int16_t* counts, uint64_t* hashes,
@@ -417,18 +461,13 @@ extern "C" int gt_find_new_segments(gt_storage* st, uint16_t K, int mode, const 
         if (len < 0) return gt::set_error(GT_ERR_INVALID_ARG, "offsets must not decrease");
         if (len >= K) nk += (uint64_t)(len - K + 1);
     }
-    if (nk && (!counts || !hashes)) return gt::set_error(GT_ERR_INVALID_ARG, "null per-window outputs");
-    std::vector<uint8_t> in(nk), out(nk), present(nk);
-    std::vector<uint64_t> nb(nk * 8);
+    if (nk && (!counts || !hashes || !seq)) return gt::set_error(GT_ERR_INVALID_ARG, "null reads or per-window outputs");
+    std::vector<uint64_t> nbc;     // neighbour values of the new windows only
+    std::vector<uint8_t> pc;       // and their presence masks
     if (nk) {
-        // forward hashes (goetia builds the compactor for Hash<> only)
-        int rc = gt_dbg_query(st, K, 0, mode, seq, offsets, n_reads, counts, hashes);
-        if (rc != GT_OK) return rc;
-        rc = gt_dbg_neighbors(st, K, 0, mode, 1, seq, offsets, n_reads, in.data(), out.data(), nb.data(),
-                              present.data());
+        const int rc = gt::segment_view(st, K, seq, offsets, n_reads, counts, hashes, nbc, pc);
         if (rc != GT_OK) return rc;
     }
-    return gt_segments_from_view(K, offsets, n_reads, counts, hashes, nb.data(), present.data(), known, n_known,
-                                 shared, prefix, n_prefix, seg_off, segs, seg_cap, dec_off, decs, dec_cap, n_segs,
-                                 n_decs);
+    return segments_core(K, offsets, n_reads, counts, hashes, nbc.data(), pc.data(), true, known, n_known, shared,
+                         prefix, n_prefix, seg_off, segs, seg_cap, dec_off, decs, dec_cap, n_segs, n_decs);
 }

@@ -14,6 +14,7 @@ from segments_golden import CASES, SHORT, case, record  # noqa: E402
 from goetia_amd import _native as N  # noqa: E402
 from goetia_amd import libgoetia as lg  # noqa: E402
 from goetia_amd.cdbg import SegmentList, StreamingCompactor  # noqa: E402
+from goetia_amd.reads import ReadBatch  # noqa: E402
 
 STORAGE = {1: lg.BitStorage, 2: lg.ByteStorage, 3: lg.SparseppSetStorage}
 
@@ -115,3 +116,44 @@ def test_skip_n_mode_refused():
     g = G(lg.SparseppSetStorage.build(), 21, mode=N.MODE_SKIP_N)
     with pytest.raises(N.UnsupportedError):
         StreamingCompactor[G].Compactor.build(g).find_new_segments('ACGT' * 10)
+
+
+@pytest.mark.parametrize('shared', [False, True])
+def test_large_batch_equals_one_read_at_a_time(shared):
+    """A batch big enough to be split over host threads (and whose new
+    windows' neighbours are gathered on the device) gives what one call per
+    read gives, with fresh or shared new-k-mer sets."""
+    from goetia_amd import workload
+    h1, h2 = workload.repeat_genome(21, 200_000, 500, 20, 0.002)
+    bg = workload.reads(22, h1, 3000, 150, 150, 0.005, 0.0)
+    q = workload.reads(23, h2, 3000, 60, 240, 0.01, 0.0)
+    G = lg.dBG[lg.SparseppSetStorage, lg.FwdLemireShifter]
+    g = G(lg.SparseppSetStorage.build(), 25)
+    g.insert_sequences(bg)
+    comp = StreamingCompactor[G].Compactor.build(g)
+    assert int(q.kmer_counts(25).sum()) >= 1 << 16
+    if shared:
+        # one set through the reads in order: every read is checked, on a
+        # shorter batch (a shared set keeps the batch on one thread anyway)
+        q = ReadBatch(q.seq[:q.offsets[600]], q.offsets[:601])
+    big_set = set() if shared else None
+    got = comp.find_new_segments_batch(q, new_kmers=big_set)
+    one_set = set()
+    n_dec = 0
+    for r in range(0, q.n_reads, 1 if shared else 5):
+        s = q.read(r).decode() if isinstance(q.read(r), bytes) else q.read(r)
+        if len(s) < 25:
+            assert got[r] is None
+            continue
+        hashes, segs, dks, pairs = [], SegmentList(), set(), []
+        nk = one_set if shared else set()
+        comp.find_new_segments(s, hashes, nk, segs, dks, pairs)
+        bsegs, bhashes, bnk, bdks, bpairs = got[r]
+        assert [x.as_tuple() for x in bsegs] == [x.as_tuple() for x in segs], r
+        assert bdks == dks and [h.value for h in bhashes] == [h.value for h in hashes]
+        assert [([k.kmer for k in a], [k.kmer for k in b]) for a, b in bpairs] == \
+            [([k.kmer for k in a], [k.kmer for k in b]) for a, b in pairs]
+        n_dec += len(pairs)
+    if shared:
+        assert big_set == one_set
+    assert n_dec > 0
