@@ -6,8 +6,9 @@ are new to the graph (segments with anchors and flanks), cut at the k-mers
 that would become decision nodes once the read is in the graph, with
 compact_segment as compactor.hh:77-145 and its Python conveniences as
 pythonize_cdbg.py:53-97.  The work is one library call, gt_find_new_segments:
-two GPU launches per call (or per batch of reads) give every window's query
-count and its eight neighbours with their presence (gt_dbg_neighbors), and
+three GPU launches per call (or per batch of reads) give every window's
+query count and, for the windows new to the graph, the eight neighbours with
+their presence (gt_dbg_neighbors and a gather), and
 the library's host code runs the two passes over each read, sequential per
 read as in goetia.
 
@@ -222,7 +223,7 @@ class Compactor:
 
     def find_new_segments_batch(self, reads, new_kmers=None):
         """find_new_segments over every read of a batch against the graph as
-        it is (nothing inserted in between): two GPU launches and one pass of
+        it is (nothing inserted in between): three GPU launches and one pass of
         the library's host code for the whole batch.  new_kmers: None for a
         fresh set per read (the one-argument overload), or one set shared by
         the reads in order.  Returns per read (segments, hashes, new k-mers,

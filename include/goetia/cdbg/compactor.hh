@@ -4,8 +4,8 @@
  * k-mers new to the graph (compact_segment, :77-145), with anchors and
  * flanks, split at the k-mers that become decision nodes once the read is in
  * the graph.  One library call per read or per batch of reads
- * (gt_find_new_segments): two GPU launches give every window's count and its
- * neighbours' values and presence bits, and the library's host code runs the
+ * (gt_find_new_segments): three GPU launches give every window's count and
+ * the new windows' neighbour values and presence bits, and the library's host code runs the
  * two passes over each read, as in goetia.
  *
  * goetia compiles find_new_segments for forward hashes only (it assigns a
@@ -102,7 +102,7 @@ struct StreamingCompactor<GraphType<StorageType, ShifterType>> {
         }
 
         // find_new_segments over every read of a batch against the graph as
-        // it is (nothing inserted in between): two GPU launches and one pass
+        // it is (nothing inserted in between): three GPU launches and one pass
         // of the library's host code for the whole batch.  With `shared`,
         // one new-k-mer set (new_kmers) runs through the reads in order;
         // otherwise each read starts from an empty one.  Results per read are
