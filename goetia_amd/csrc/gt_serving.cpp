@@ -119,6 +119,7 @@ struct gt_server {
     std::mutex mu;
     std::condition_variable cv;
     std::thread worker;
+    int device = 0;                    // the device the server was created on
     bool stop = false;
     uint64_t next_ticket = 1;
     uint64_t clock = 0;
@@ -253,7 +254,7 @@ void finish(gt_server* S, ExecSlot& x) {
 }
 
 void worker_loop(gt_server* S) {
-    hipSetDevice(0);
+    (void)hipSetDevice(S->device);     // checked when the server was created
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
         bool progress = false;
@@ -384,6 +385,7 @@ int gt_server_new(int kind, const uint64_t* tablesizes, uint32_t n_tables, uint1
         hip(hipEventCreateWithFlags(&x.done, hipEventDisableTiming), "hipEventCreate");
         if (rc == GT_OK && gt_workspace_new(&x.ws) != GT_OK) rc = sfail(GT_ERR_OUT_OF_MEMORY, gt_last_error());
     }
+    hip(hipGetDevice(&S->device), "hipGetDevice");
     S->stats.resident_slots = n_resident;
     S->stats.bytes_per_stream = per_stream;
     if (rc != GT_OK) {
