@@ -124,6 +124,9 @@ def random_access(N, table_bytes, n_ops=1 << 30, sweep=False):
     tw = max(1, min(n_words, table_bytes // 4))
     out['table_size_bytes'] = tw * 4
     out['read_ops_per_s_table_size'] = rate(tw, 0)
+    # the same rates as bandwidth at a 64-byte access granule
+    for k in ('read_ops_per_s', 'atomic_or_ops_per_s', 'atomic_cas_ops_per_s', 'read_ops_per_s_table_size'):
+        out[k.replace('ops_per_s', 'gbps_64B')] = round(out[k] * 64 / 1e9, 1)
     if sweep:
         out['read_ops_per_s_by_mb'] = {mb: rate(min(n_words, mb * (1 << 20) // 4), 0) for mb in SWEEP_MB}
         out['nontemporal_read_ops_per_s'] = rate(n_words, 3)
