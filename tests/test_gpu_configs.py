@@ -134,6 +134,35 @@ def test_cms_full_size_properties():
         assert np.array_equal(x, y)
 
 
+def test_cms_full_size_insert_and_query_properties():
+    """One-pass insert_and_query over the count-min workload at full size (36
+    sub-batches of 2^26 windows): the tables equal a plain insert of the same
+    reads, every counted window reads >= 1, and the windows reading 1 are
+    exactly the new k-mers (n_unique)."""
+    import torch
+    cfg, batch = workload.make('cms_20m', 1.0)
+    K = cfg['K']
+    d = batch.to_device()
+    nk = int(batch.kmer_counts(K).sum())
+    g1 = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(cfg['max_table'], 4), K)
+    g1.mode = N.MODE_SKIP_N
+    counts = torch.empty(nk, dtype=torch.int16, device='cuda')
+    g1.insert_and_query_device(d, counts)
+    torch.cuda.synchronize()
+    valid = counts >= 0
+    assert int(valid.sum().item()) > 0.99 * nk
+    assert int((valid & (counts < 1)).sum().item()) == 0
+    assert int((counts == 1).sum().item()) == g1.n_unique()
+    t1 = g1.get_raw()
+    del g1, counts
+    g2 = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage.build(cfg['max_table'], 4), K)
+    g2.mode = N.MODE_SKIP_N
+    g2.insert_device(d)
+    torch.cuda.synchronize()
+    for x, y in zip(t1, g2.get_raw()):
+        assert np.array_equal(x, y)
+
+
 def test_workspace_shared_across_storages():
     import ctypes as C
     import torch
