@@ -209,6 +209,41 @@ int main(int argc, char** argv) {
         CHECK(n_pass == d1 && lines == 4 * n_pass && d3 == 0);
         std::remove(out.c_str());
     }
+
+    // window_neighbors (GPU) == the cursor's left / right extensions and the
+    // union-node filter (host shifts, per-hash queries)
+    {
+        auto gw = dBG<ByteStorage, CanLemireShifter>::build(ByteStorage::build(100000, 4), 21);
+        gw->insert_sequence(seq);
+        const std::string other = seq.substr(3, 30) + "ACGTTGCA";
+        ReadBatch wb({seq, other});
+        std::vector<uint8_t> in_deg, out_deg, present;
+        std::vector<uint64_t> vals;
+        gw->window_neighbors(wb, in_deg, out_deg, vals, present);
+        size_t k = 0;
+        for (const std::string& r : {seq, other}) {
+            for (size_t i = 0; i + 21 <= r.size(); ++i, ++k) {
+                gw->set_cursor(r.substr(i, 21));
+                auto l = gw->left_extensions();
+                auto rt = gw->right_extensions();
+                uint8_t bits = 0;
+                for (int c = 0; c < 4; ++c) {
+                    CHECK(vals[8 * k + c] == l[c].value());
+                    CHECK(vals[8 * k + 4 + c] == rt[c].value());
+                }
+                for (auto& e : gw->filter_nodes(l))
+                    bits |= (uint8_t)(1u << (std::string("ACGT").find(e.symbol)));
+                for (auto& e : gw->filter_nodes(rt))
+                    bits |= (uint8_t)(1u << (4 + std::string("ACGT").find(e.symbol)));
+                CHECK(present[k] == bits);
+                CHECK(in_deg[k] == __builtin_popcount(bits & 15u) && out_deg[k] == __builtin_popcount(bits >> 4));
+                std::set<CanLemireShifter::hash_type> extra{l[0].hash, rt[3].hash};
+                CHECK(gw->in_degree(extra) == gw->filter_nodes(l, extra).size());
+                CHECK(gw->out_degree(extra) == (size_t)__builtin_popcount((bits >> 4) | 8u));
+            }
+        }
+        CHECK(k == vals.size() / 8);
+    }
     std::printf("FACADE OK\n");
     return 0;
 }
