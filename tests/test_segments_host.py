@@ -142,3 +142,36 @@ def test_empty_batch_and_bad_arguments():
         seg_off = np.zeros(2, np.int64)
         N.call('gt_segments_from_view', 21, N.ptr(off), 1, None, None, None, None, None, 0, 0, None, 0,
                N.ptr(seg_off), None, 0, N.ptr(seg_off), None, 0, C.byref(ns), C.byref(nd))
+
+
+def test_output_capacity_retry(oracle):
+    """Too small output arrays: the call reports the totals it needs and
+    fails; the wrapper grows the arrays and the second call succeeds with the
+    same records as a generous first call."""
+    import ctypes as C
+
+    from goetia_amd import _native as N
+    from goetia_amd.cdbg import _Outputs
+    c = case('genome_set_k21')
+    K = c['K']
+    o = oracle_storage(oracle, c)
+    s = c['reads'][5]
+    counts, hv, vals, mask = oracle_view(o, s, K)
+    off = np.array([0, len(s)], np.int64)
+    head = (K, N.ptr(off), 1, N.ptr(counts), N.ptr(hv), N.ptr(np.ascontiguousarray(vals)), N.ptr(mask), None, 0, 0,
+            None, 0)
+    big = _Outputs(1, len(hv))
+    big.call('gt_segments_from_view', *head)
+    assert big.n_segs.value >= 3
+    small = _Outputs(1, len(hv))
+    small.seg_cap, small.dec_cap = 1, 0
+    small.segs = np.zeros(7, np.uint64)
+    small.decs = np.zeros(11, np.uint64)
+    ns, nd = C.c_uint64(), C.c_uint64()
+    with pytest.raises(ValueError):
+        N.call('gt_segments_from_view', *head, N.ptr(small.seg_off), N.ptr(small.segs), 1, N.ptr(small.dec_off),
+               N.ptr(small.decs), 0, C.byref(ns), C.byref(nd))
+    assert ns.value == big.n_segs.value and nd.value == big.n_decs.value
+    small.call('gt_segments_from_view', *head)
+    n = big.n_segs.value
+    np.testing.assert_array_equal(small.segs[:7 * n], big.segs[:7 * n])
