@@ -357,11 +357,10 @@ def diginorm(args):
 
 def segments(args):
     """StreamingCompactor::find_new_segments over a batch against a fixed
-    graph (gt_find_new_segments: GPU view + the library's host passes), and
-    the compiled goetia compactor (oracle/_ref, when built) on a sample of the
-    same reads, with the segment and decision counts compared."""
+    graph (gt_find_new_segments: GPU view + the library's host passes).  The
+    compiled goetia compactor is timed on the same reads on the host where the
+    goetia checkout is (oracle/time_ref.py), never on the GPU host."""
     import ctypes as C
-    import os
     from goetia_amd import _native as N
     from goetia_amd import libgoetia as lg
     from goetia_amd import workload
@@ -398,50 +397,11 @@ def segments(args):
            'reads': q.n_reads, 'kmers': nk, 'graph_kmers': g.n_unique(), 'ms_per_batch': round(dt * 1e3, 2),
            'segments': int(n_segs.value), 'decision_kmers': int(n_decs.value),
            'new_kmers': int((counts == 0).sum())}
-    ref_so = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'oracle', '_ref', 'libgoetia_ref.so')
-    if os.path.exists(ref_so) and not args.no_cpu_baseline:
-        L = C.CDLL(ref_so)
-        L.ref_new.restype = C.c_void_p
-        L.ref_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint16]
-        L.ref_insert_reads.restype = C.c_uint64
-        L.ref_insert_reads.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_void_p, C.c_uint64, C.c_void_p]
-        L.ref_find_new_segments.restype = C.c_uint64
-        L.ref_find_new_segments.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
-                                            C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64, C.c_void_p]
-        h = L.ref_new(3, 0, K, 0, 0)
-        L.ref_insert_reads(h, K, 0, bg.seq.ctypes.data_as(C.c_char_p), bg.offsets.ctypes.data, bg.n_reads, None)
-        n = min(q.n_reads, 20_000)
-        cap, kcap = 1 << 24, 1 << 26
-        words = np.zeros(cap, np.uint64)
-        km = np.zeros(kcap, np.uint8)
-        used = C.c_uint64()
-        t0 = time.perf_counter()
-        w = L.ref_find_new_segments(h, q.seq.ctypes.data_as(C.c_char_p), q.offsets.ctypes.data, n, 0, 0,
-                                    words.ctypes.data, cap, km.ctypes.data_as(C.c_char_p), kcap, C.byref(used))
-        t_ref = time.perf_counter() - t0
-        # segment and decision counts per read from the reference's stream
-        ref_segs = ref_decs = 0
-        i = 0
-        for _ in range(n):
-            ns = int(words[i])
-            i += 1
-            if ns == (1 << 64) - 1:
-                continue
-            ref_segs += ns
-            i += 7 * ns
-            i += 1 + int(words[i])
-            i += 1 + int(words[i])
-            npairs = int(words[i])            # one pair per decision window
-            ref_decs += npairs
-            i += 1
-            for _ in range(npairs):
-                i += 2 + int(words[i]) + int(words[i + 1])
-        assert i == w
-        call(n)
-        out['cpu_baseline'] = {'value': round(n / t_ref, 1), 'unit': 'reads/s', 'cores': 1, 'kind': 'reference',
-                               'sample': f'first {n} reads, compiled goetia StreamingCompactor (oracle/_ref), '
-                                         f'{t_ref:.1f} s',
-                               'same_counts': bool(ref_segs == n_segs.value and ref_decs == n_decs.value)}
+    # counts on the first 20k reads, the sample oracle/time_ref.py times the
+    # compiled goetia compactor on
+    n = min(q.n_reads, 20_000)
+    call(n)
+    out['first_reads'] = {'reads': n, 'segments': int(n_segs.value), 'decision_kmers': int(n_decs.value)}
     print(json.dumps(out), flush=True)
 
 

@@ -105,6 +105,11 @@ SIGNATURES = {
     'gt_storage_exact_keys': [st_p, vp, u64, u64p],
     'gt_storage_save_async': [st_p, vp, vp, vp],
     'gt_storage_load_async': [st_p, vp, vp, vp],
+    'gt_storage_save': [st_p, C.c_char_p, u16],
+    'gt_storage_load': [st_p, C.c_char_p, C.POINTER(u16)],
+    'gt_oxli_write': [C.c_char_p, cint, u16, u32, u64p, u64, vp],
+    'gt_oxli_read_header': [C.c_char_p, cint, C.POINTER(u16), u32p, u64p, u64p],
+    'gt_oxli_read': [C.c_char_p, cint, vp],
     'gt_kmer_offsets_dev': [u16, vp, u64, vp, vp],
     'gt_hash_reads_dev': [u16, cint, vp, vp, u64, u64, vp, vp, vp, vp],
     'gt_hash_reads': [u16, cint, vp, vp, u64, vp, vp, vp],

@@ -573,6 +573,7 @@ int enqueue_subbatch(gt_storage* st, Scratch& sc, const Launch& L, const Reads& 
             ProfScope ps("exact_resolve", s);
             launch_exact_resolve(L, Rs, E, fresh, n_new, new_flags, ctr);
         }
+        launch_exact_tail(ctr, s);
     } else {
         const Tables T = st->tables_view();
         {
@@ -793,6 +794,7 @@ int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* i
             launch_hash_exact_lookup(L, h + i0, m, E, fresh, ctr);
             launch_hash_exact_insert(L, h + i0, m, E, fresh, ctr);
             launch_hash_exact_resolve(L, h + i0, m, E, fresh, is_new ? is_new + i0 : nullptr, ctr);
+            launch_exact_tail(ctr, s);
         } else {
             const Tables T = st->tables_view();
             const ClaimMap cm = claim_view(st->own);
@@ -845,6 +847,40 @@ namespace gt {
 // Sets this thread's gt_last_error() message from the library's other
 // translation units; returns code.
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+// Replaces a Bloom / count-min storage's tables with zeroed tables of the
+// given sizes (the file loaders: goetia's load() takes the file's table
+// sizes, bitstorage.cc:239-263, bytestorage.cc:196-240).  Counters are kept;
+// captured pipelines see a new epoch and capture again.
+int storage_reshape(gt_storage* st, const uint64_t* sizes, uint32_t n_tables) {
+    if (!valid_storage(st) || st->kind == kExact || !sizes || n_tables == 0 || n_tables > (uint32_t)kMaxTables)
+        return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    for (uint32_t i = 0; i < n_tables; ++i)
+        if (sizes[i] == 0 || sizes[i] >= kMaxBins) return fail(GT_ERR_INVALID_ARG, "table sizes must be in 1..2^32-1");
+    GT_TRY(ensure_device());
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    if (st->n_tables == n_tables && std::equal(sizes, sizes + n_tables, st->sizes.begin())) return GT_OK;
+    GT_HIP(hipDeviceSynchronize());
+    std::vector<uint64_t> sz(sizes, sizes + n_tables), nb, off;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_tables; ++i) {
+        nb.push_back(st->kind == kBit ? sz[i] / 8 + 1 : sz[i]);
+        off.push_back(total);
+        total += (nb[i] + 255) / 256 * 256;
+    }
+    DevBuf t;
+    GT_TRY(t.ensure(total));
+    std::swap(st->tables.p, t.p);
+    std::swap(st->tables.bytes, t.bytes);
+    st->n_tables = n_tables;
+    st->sizes = sz;
+    st->nbytes = nb;
+    st->table_off = off;
+    GT_TRY(zero_tables(st, nullptr));
+    GT_HIP(hipStreamSynchronize(nullptr));
+    st->epoch++;
+    return GT_OK;
+}
 }  // namespace gt
 
 extern "C" {
@@ -965,7 +1001,7 @@ int gt_storage_reset(gt_storage* st) {
     std::lock_guard<std::recursive_mutex> lk(st->mu);
     GT_TRY(zero_tables(st, nullptr));
     if (st->kind == kExact) {
-        launch_fill_u64(st->counters.as<uint64_t>() + kNUnique, 1, 0, nullptr);
+        launch_fill_u64(st->counters.as<uint64_t>() + kNUnique, 2, 0, nullptr);   // n_unique, last-was-present
         st->ecount_ub = 0;
     }
     GT_HIP(hipStreamSynchronize(nullptr));
@@ -997,6 +1033,20 @@ int gt_storage_table_bytes(const gt_storage* st, uint32_t i, uint64_t* nbytes) {
     return GT_OK;
 }
 
+// The exact set's n_occupied is its container's bucket count
+// (sparseppstorage.hh:80-86).  sparsepp starts at 32 buckets, grows before an
+// insert whose size + 1 would pass half the buckets, by doubling
+// (spp.h:2691-2747, HT_OCCUPANCY_PCT = 50), and clear() of a non-empty set
+// returns to 32 (spp.h:2980-2989).  Every insert call checks, also for a value
+// already present, so after n distinct values the count covers n + 1 when the
+// last value inserted was already there.
+static uint64_t set_bucket_count(uint64_t n_unique, bool last_was_present) {
+    const uint64_t need = n_unique + (n_unique && last_was_present ? 1 : 0);
+    uint64_t b = 32;
+    while (need > b / 2) b *= 2;
+    return b;
+}
+
 int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(ensure_device());
@@ -1006,7 +1056,8 @@ int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied
     GT_TRY(read_counters(st, c, nullptr));
     if (c[kErrors]) return fail(GT_ERR_INTERNAL, "device table probe limit was reached; counters are not exact");
     if (n_unique) *n_unique = c[kNUnique];
-    if (n_occupied) *n_occupied = st->kind == kExact ? st->ecap : c[kNOccupied];
+    if (n_occupied) *n_occupied = st->kind == kExact ? set_bucket_count(c[kNUnique], c[kNOccupied] != 0)
+                                                     : c[kNOccupied];
     return GT_OK;
 }
 

@@ -29,7 +29,12 @@ enum Mode : int { kRaw = 0, kSkipN = 1 };
 // Device counters of one storage (u64 each).  kGen is the storage's
 // first-touch generation (exact-set stamps): bumped on the device once per
 // sub-batch, so replays of a captured batch never reuse a generation.
-enum Counter : int { kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kGen = 4, kNCounters = 5 };
+// For the exact set, kNOccupied holds 1 when the last value inserted was
+// already present (its bucket count depends on it), and kLastIns / kLastNew
+// the position + 1 of a sub-batch's last inserted and last new window.
+enum Counter : int {
+    kNUnique = 0, kNOccupied = 1, kFresh = 2, kErrors = 3, kGen = 4, kLastIns = 5, kLastNew = 6, kNCounters = 7
+};
 
 // Error bits OR-ed into counters[kErrors] by kernels.
 enum ErrBit : uint64_t { kErrProbe = 1, kErrClaim = 2 };

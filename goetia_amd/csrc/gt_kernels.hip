@@ -54,6 +54,20 @@ __device__ __forceinline__ void add_counter(uint64_t* ctr, uint64_t v) {
     if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)ctr, (unsigned long long)v);
 }
 
+__device__ __forceinline__ uint64_t wave_max(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Raises a device counter to the wave's largest value, one atomic per wave.
+__device__ __forceinline__ void max_counter(uint64_t* ctr, uint64_t v) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicMax((unsigned long long*)ctr, (unsigned long long)v);
+}
+
 // ------------------------------------------------------------ claim map
 
 __device__ __forceinline__ uint64_t claim_key(uint32_t gen, uint32_t table, uint64_t bin) {
@@ -541,12 +555,15 @@ struct ExactResolveOp {
     uint8_t* new_flags;
     ReadAcc acc;
     uint64_t n_new = 0;
+    uint64_t last_ins = 0, last_new = 0;   // position + 1 of the last inserted / new window
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
         bool is_new = false;
         if (valid && fresh[s - b0]) {
             const uint64_t slot = exact_find(E, kmer_value(fw, rc, canonical));
             is_new = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)(s - b0));
         }
+        if (valid) last_ins = (uint64_t)(s - b0) + 1;
+        if (is_new) last_new = (uint64_t)(s - b0) + 1;
         n_new += is_new;
         acc.add(r, is_new);
         if (new_flags) new_flags[kmer_off[r] + (s - rstart)] = is_new;
@@ -779,6 +796,8 @@ __global__ __launch_bounds__(kBlock) void exact_resolve_kernel(Reads R, ExactSet
     ExactResolveOp op{E, fresh, R.b0, R.canonical, current_gen(counters), R.kmer_off, new_flags, ReadAcc{n_new}};
     walk_tiles<PER>(R, sh, op);
     add_counter(counters + kNUnique, op.n_new);
+    max_counter(counters + kLastIns, op.last_ins);
+    max_counter(counters + kLastNew, op.last_new);
 }
 
 template <int PER>
@@ -883,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void hash_exact_insert_kernel(const uint64_
 __global__ __launch_bounds__(kBlock) void hash_exact_resolve_kernel(const uint64_t* h, uint64_t n, ExactSet E,
                                                                   const uint8_t* fresh, uint8_t* is_new,
                                                                   uint64_t* counters) {
-    uint64_t nn = 0;
+    uint64_t nn = 0, last_ins = 0, last_new = 0;
     const uint32_t gen = current_gen(counters);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         bool nw = false;
@@ -892,9 +911,13 @@ __global__ __launch_bounds__(kBlock) void hash_exact_resolve_kernel(const uint64
             nw = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)i);
         }
         nn += nw;
+        last_ins = i + 1;
+        if (nw) last_new = i + 1;
         if (is_new) is_new[i] = nw;
     }
     add_counter(counters + kNUnique, nn);
+    max_counter(counters + kLastIns, last_ins);
+    max_counter(counters + kLastNew, last_new);
 }
 
 __global__ __launch_bounds__(kBlock) void hash_query_kernel(const uint64_t* h, uint64_t n, Tables T, ExactSet E,
@@ -1045,6 +1068,18 @@ __global__ void bump_gen_kernel(uint64_t* storage_gen, uint64_t* claim_gen) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (storage_gen) *storage_gen += 1;
         if (claim_gen) *claim_gen += 1;
+    }
+}
+
+// After an exact-set sub-batch: if it inserted anything, record whether its
+// last inserted value was already present (kNOccupied), and clear the
+// positions for the next sub-batch.
+__global__ void exact_tail_kernel(uint64_t* counters) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint64_t li = counters[kLastIns], ln = counters[kLastNew];
+        if (li) counters[kNOccupied] = li != ln;
+        counters[kLastIns] = 0;
+        counters[kLastNew] = 0;
     }
 }
 
@@ -2167,6 +2202,10 @@ void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, co
 
 void launch_bump_gen(uint64_t* storage_gen, uint64_t* claim_gen, hipStream_t s) {
     bump_gen_kernel<<<1, 64, 0, s>>>(storage_gen, claim_gen);
+}
+
+void launch_exact_tail(uint64_t* counters, hipStream_t s) {
+    exact_tail_kernel<<<1, 64, 0, s>>>(counters);
 }
 
 void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,

@@ -26,6 +26,7 @@ void launch_apply(const Launch& L, const Reads& R, const Tables& T, const ClaimM
 void launch_exact_lookup(const Launch& L, const Reads& R, const ExactSet& E, uint8_t* fresh, uint64_t* counters);
 void launch_exact_insert(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                          uint64_t* counters);
+void launch_exact_tail(uint64_t* counters, hipStream_t s);
 void launch_exact_resolve(const Launch& L, const Reads& R, const ExactSet& E, const uint8_t* fresh,
                           uint64_t* n_new, uint8_t* new_flags, uint64_t* counters);
 void launch_query(const Launch& L, const Reads& R, const Tables& T, const ExactSet& E, int16_t* counts,

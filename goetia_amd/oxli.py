@@ -1,18 +1,16 @@
-"""khmer "OXLI" v4 table files for BitStorage and ByteStorage.
+"""khmer "OXLI" v4 table files for BitStorage and ByteStorage, as goetia
+writes and reads them (BitStorage::save / load, bitstorage.cc:151-307;
+ByteStorage::save / load through ByteStorageFile, bytestorage.cc:153-667,
+gzip-compressed when a count-min file's name ends in ".gz").
 
-Layout (little-endian), as written by BitStorage::save (bitstorage.cc:151-189)
-and ByteStorageFileWriter (bytestorage.cc:480-536):
-
-  "OXLI" | version u8 = 4 | type u8 (2 = hashbits, 1 = counting)
-  [counting only: use_bigcount u8]
-  ksize u32 | n_tables u8 | occupied_bins u64
-  n_tables x ( tablesize u64 | table bytes: tablesize/8+1 (Bloom) or tablesize )
-  [counting only: n_bigcounts u64 (+ pairs, always 0 here)]
-
-The bytes are the device tables copied to host; a load writes them back and
-restores n_occupied (n_unique is not stored by the format).
+The files are written and read by libgoetia_amd (gt_storage_save /
+gt_storage_load, and the host-only gt_oxli_* for tables in host memory), so
+the bytes -- compressed ones included -- are goetia's.  A load replaces the
+storage's tables, sizes included, and restores n_occupied (n_unique is not
+in the format).  Files holding bigcount entries are refused: the device
+storages keep no bigcount map.
 """
-import struct
+import ctypes as C
 
 import numpy as np
 
@@ -24,62 +22,52 @@ SAVED_COUNTING_HT = 1
 SAVED_HASHBITS = 2
 
 
-def save_storage(store, filename, ksize):
+def _kind(store):
     from .storage import BitStorage, ByteStorage
     if isinstance(store, BitStorage):
-        kind = SAVED_HASHBITS
-    elif isinstance(store, ByteStorage):
-        kind = SAVED_COUNTING_HT
-    else:
-        raise N.UnsupportedError('OXLI files hold Bloom and count-min tables only')
-    sizes = store.get_tablesizes()
-    tables = store.get_raw_tables()
-    with open(filename, 'wb') as f:
-        f.write(SIGNATURE)
-        f.write(struct.pack('<BB', VERSION, kind))
-        if kind == SAVED_COUNTING_HT:
-            f.write(struct.pack('<B', 0))
-        f.write(struct.pack('<IBQ', int(ksize), len(sizes), store.n_occupied()))
-        for size, tab in zip(sizes, tables):
-            f.write(struct.pack('<Q', size))
-            f.write(tab.tobytes())
-        if kind == SAVED_COUNTING_HT:
-            f.write(struct.pack('<Q', 0))
+        return N.BIT
+    if isinstance(store, ByteStorage):
+        return N.BYTE
+    raise N.UnsupportedError('OXLI files hold Bloom and count-min tables only '
+                             '(the exact set has no file format, sparseppstorage.cc:52-58)')
+
+
+def save_storage(store, filename, ksize):
+    _kind(store)
+    N.call('gt_storage_save', store.handle, str(filename).encode(), int(ksize))
 
 
 def load_storage(store, filename):
-    """Loads an OXLI file into `store` (same table sizes); returns ksize."""
-    from .storage import BitStorage, ByteStorage
-    with open(filename, 'rb') as f:
-        data = f.read()
-    if data[:4] != SIGNATURE:
-        raise OSError(f'{filename}: not an OXLI file')
-    version, kind = struct.unpack_from('<BB', data, 4)
-    if version != VERSION:
-        raise OSError(f'{filename}: OXLI version {version} is not supported')
-    pos = 6
-    if kind == SAVED_COUNTING_HT:
-        if not isinstance(store, ByteStorage):
-            raise OSError(f'{filename}: counting table loaded into {type(store).__name__}')
-        (use_big,) = struct.unpack_from('<B', data, pos)
-        pos += 1
-    elif kind == SAVED_HASHBITS:
-        if not isinstance(store, BitStorage):
-            raise OSError(f'{filename}: Bloom table loaded into {type(store).__name__}')
-    else:
-        raise OSError(f'{filename}: unknown OXLI table type {kind}')
-    ksize, n_tables, occupied = struct.unpack_from('<IBQ', data, pos)
-    pos += 13
-    if n_tables != store.n_tables():
-        raise OSError(f'{filename}: {n_tables} tables, storage has {store.n_tables()}')
-    sizes = store.get_tablesizes()
-    for i in range(n_tables):
-        (size,) = struct.unpack_from('<Q', data, pos)
-        pos += 8
-        if size != sizes[i]:
-            raise OSError(f'{filename}: table {i} has size {size}, storage has {sizes[i]}')
-        nb = size // 8 + 1 if kind == SAVED_HASHBITS else size
-        store.set_raw_table(i, np.frombuffer(data, np.uint8, nb, pos))
-        pos += nb
-    N.call('gt_storage_set_counters', store.handle, store.n_unique_kmers(), int(occupied))
-    return ksize
+    """Loads an OXLI file into `store` (its tables take the file's sizes);
+    returns the file's ksize."""
+    _kind(store)
+    k = C.c_uint16()
+    N.call('gt_storage_load', store.handle, str(filename).encode(), C.byref(k))
+    return k.value
+
+
+def read_file(filename, kind):
+    """Host-only read of a file of `kind` (N.BIT / N.BYTE): (ksize,
+    tablesizes, occupied, tables as uint8 arrays)."""
+    p = str(filename).encode()
+    k, n, occ = C.c_uint16(), C.c_uint32(), C.c_uint64()
+    sizes = np.zeros(16, np.uint64)
+    N.call('gt_oxli_read_header', p, int(kind), C.byref(k), C.byref(n), sizes.ctypes.data_as(N.u64p),
+           C.byref(occ))
+    sizes = [int(s) for s in sizes[:n.value]]
+    tables = [np.zeros(s // 8 + 1 if kind == N.BIT else s, np.uint8) for s in sizes]
+    ptrs = (C.c_void_p * len(tables))(*[t.ctypes.data for t in tables])
+    N.call('gt_oxli_read', p, int(kind), ptrs)
+    return k.value, sizes, occ.value, tables
+
+
+def write_file(filename, kind, ksize, tablesizes, occupied, tables):
+    """Host-only write of a file of `kind` from tables in host memory."""
+    tables = [np.ascontiguousarray(t, np.uint8) for t in tables]
+    sizes = np.array(tablesizes, np.uint64)
+    for s, t in zip(tablesizes, tables):
+        if len(t) != (s // 8 + 1 if kind == N.BIT else s):
+            raise ValueError('table bytes do not match the table size')
+    ptrs = (C.c_void_p * len(tables))(*[t.ctypes.data for t in tables])
+    N.call('gt_oxli_write', str(filename).encode(), int(kind), int(ksize), len(tables),
+           sizes.ctypes.data_as(N.u64p), int(occupied), ptrs)

@@ -273,11 +273,22 @@ class SparseppSetStorage(_Storage):
 
 
 class PHMapStorage(SparseppSetStorage):
-    """phmapstorage.hh:43,90-95: same set semantics as SparseppSetStorage."""
+    """phmapstorage.hh:43,90-95: same set semantics as SparseppSetStorage.
+    n_occupied() is sparsepp's bucket count here, not that of phmap's
+    parallel set (a sum over 16 hash-selected sub-tables)."""
 
 
 class BTreeStorage(SparseppSetStorage):
     """btreestorage.hh:43,90-95: same set semantics as SparseppSetStorage."""
+
+    def n_buckets(self):
+        """btreestorage.hh:78-80: the btree's max_size(), a constant
+        (btree.h:1872, the largest signed 64-bit value as goetia's build
+        reports it)."""
+        return 2 ** 63 - 1
+
+    def n_occupied(self):
+        return self.n_buckets()
 
 
 typenames = [(SparseppSetStorage, 'SparseppSetStorage'), (PHMapStorage, 'PHMapStorage'),

@@ -64,6 +64,8 @@ class SparseppSetStorage : public Storage<uint64_t>, public detail::DeviceStorag
 };
 
 // phmapstorage.hh:43,90-95 and btreestorage.hh:43,90-95: the same set.
+// PHMapStorage reports sparsepp's bucket count as n_occupied, not that of
+// phmap's parallel set.
 class PHMapStorage : public SparseppSetStorage {
   public:
     using SparseppSetStorage::SparseppSetStorage;
@@ -74,6 +76,10 @@ class BTreeStorage : public SparseppSetStorage {
   public:
     using SparseppSetStorage::SparseppSetStorage;
     static std::shared_ptr<BTreeStorage> build() { return std::make_shared<BTreeStorage>(); }
+    // btreestorage.hh:78-84: the btree's max_size() (btree.h:1872), a
+    // constant: the largest signed 64-bit value in goetia's build.
+    const uint64_t n_buckets() const { return 0x7FFFFFFFFFFFFFFFULL; }
+    const uint64_t n_occupied() const override { return n_buckets(); }
 };
 
 template <>

@@ -104,8 +104,11 @@ int gt_storage_tablesizes(const gt_storage* st, uint64_t* out);
  * bitstorage.hh:151), p (count-min, bytestorage.hh:131). */
 int gt_storage_table_bytes(const gt_storage* st, uint32_t i, uint64_t* nbytes);
 /* n_unique_kmers() / n_occupied(): bitstorage.hh:173-181,
- * bytestorage.hh:199-209, sparseppstorage.hh:72-86 (for the exact set
- * n_occupied is the slot count of the device table, its bucket_count). */
+ * bytestorage.hh:199-209, sparseppstorage.hh:72-86.  For the exact set
+ * n_occupied is sparsepp's bucket_count() after the same inserts: 32
+ * buckets, doubled before any insert (of a new or a present value) that
+ * would take the size + 1 past half the buckets (spp.h:2691-2747), back to
+ * 32 on reset (spp.h:2980-2989). */
 int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied);
 /* Restores counters after load() (bitstorage.cc:196-307). */
 int gt_storage_set_counters(gt_storage* st, uint64_t n_unique, uint64_t n_occupied);
@@ -129,6 +132,31 @@ int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint
 int gt_storage_save_async(gt_storage* st, uint8_t* const* host_tables, uint64_t* counters, void* stream);
 int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, const uint64_t* counters,
                           void* stream);
+
+/* OXLI v4 table files, as goetia writes and reads them: BitStorage::save /
+ * load (bitstorage.cc:151-307, always uncompressed) and ByteStorage::save /
+ * load (bytestorage.cc:638-667 through ByteStorageFile::save / load,
+ * :153-167, :620-636), which write and read gzip (bytestorage.cc:304-478,
+ * 538-618) when the name's text after its last '.' is "gz" and plain files
+ * otherwise (:170-302, 480-536).  load replaces the storage's tables, sizes
+ * included, with the file's and restores n_occupied; n_unique is kept (the
+ * format does not hold it).  The device storages keep no bigcount map: save
+ * writes use_bigcount 0 and no pairs, load refuses (GT_ERR_UNSUPPORTED) a file
+ * with use_bigcount set or bigcount pairs.  I/O failures are GT_ERR_IO
+ * (GoetiaFileException).  The exact set has no file format
+ * (sparseppstorage.cc:52-58 are empty): GT_ERR_UNSUPPORTED. */
+int gt_storage_save(gt_storage* st, const char* path, uint16_t ksize);
+int gt_storage_load(gt_storage* st, const char* path, uint16_t* ksize);
+/* The host halves of the two above, for tables already in host memory (no
+ * device): write one file of `kind` GT_BIT_STORAGE or GT_BYTE_STORAGE from
+ * tables[n_tables] in the reference layout; read a file's header
+ * (tablesizes has room for 16) and then its tables into caller buffers of
+ * the sizes the header gave. */
+int gt_oxli_write(const char* path, int kind, uint16_t ksize, uint32_t n_tables, const uint64_t* tablesizes,
+                  uint64_t occupied, const uint8_t* const* tables);
+int gt_oxli_read_header(const char* path, int kind, uint16_t* ksize, uint32_t* n_tables, uint64_t* tablesizes,
+                        uint64_t* occupied);
+int gt_oxli_read(const char* path, int kind, uint8_t* const* tables);
 
 /* ------------------------------------------------------------ hashing */
 

@@ -261,6 +261,7 @@ int or_storage_init(or_storage* st, int kind, const uint64_t* sizes, uint32_t n_
     memset(st, 0, sizeof(*st));
     st->kind = kind;
     if (kind == OR_EXACT) {
+        st->n_occupied = 32;   /* spp.h:2558 HT_DEFAULT_STARTING_BUCKETS */
         st->capacity = 1024;
         st->keys = (uint64_t*)malloc(st->capacity * sizeof(uint64_t));
         if (!st->keys) return -1;
@@ -397,11 +398,31 @@ static int16_t byte_query(const or_storage* st, uint64_t h) {
     return m;
 }
 
+/* The exact set's n_occupied is sparsepp's bucket_count()
+ * (sparseppstorage.hh:80-86).  Every insert first runs _resize_delta(1)
+ * (spp.h:3223-3227): with no erasures, it keeps the buckets while
+ * size + 1 <= enlarge_threshold = buckets * 0.5 (spp.h:2706-2710, 419-421,
+ * HT_OCCUPANCY_PCT = 50 at :3542) and otherwise grows to
+ * min_buckets(size + 1, buckets) (spp.h:2717-2723, 443-458). */
+static uint64_t spp_min_buckets(uint64_t num_elts, uint64_t min_buckets_wanted) {
+    uint64_t sz = 4;   /* HT_MIN_BUCKETS */
+    while (sz < min_buckets_wanted || num_elts >= sz / 2) sz *= 2;
+    return sz;
+}
+
+static void spp_resize_delta(or_storage* st) {
+    uint64_t need = st->count + 1;
+    if (st->n_occupied >= 4 && need <= st->n_occupied / 2) return;
+    if (spp_min_buckets(need, 0) <= st->n_occupied) return;
+    st->n_occupied = spp_min_buckets(need, st->n_occupied);
+}
+
 int or_insert(or_storage* st, uint64_t h) {
     switch (st->kind) {
         case OR_BIT:  return bit_insert(st, h);
         case OR_BYTE: return byte_insert(st, h);
         default: {
+            spp_resize_delta(st);
             int n = exact_insert(st, h);
             if (n) st->n_unique++;
             return n;
@@ -606,6 +627,54 @@ uint64_t or_insert_reads(or_storage* st, int K, int canonical, int mode,
     return total;
 }
 
+/* or_insert_reads with each window's first-touch flag, packed
+ * max(0, len - K + 1) per read. */
+uint64_t or_insert_reads_flags(or_storage* st, int K, int canonical, int mode,
+                               const char* seq, const int64_t* offsets, uint64_t n_reads,
+                               uint64_t* n_new, uint8_t* new_flags) {
+    uint64_t total = 0, pos = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        uint64_t nn = 0;
+        const size_t len = (size_t)(offsets[r + 1] - offsets[r]);
+        total += or_insert_sequence(st, K, canonical, mode, seq + offsets[r], len, &nn,
+                                    new_flags ? new_flags + pos : NULL);
+        if (n_new) n_new[r] = nn;
+        if (len >= (size_t)K) pos += len - (size_t)K + 1;
+    }
+    return total;
+}
+
+/* insert_and_query_sequence over reads in order (dbg.hh:327-340), counts
+ * packed per read. */
+uint64_t or_insert_and_query_reads(or_storage* st, int K, int canonical, int mode,
+                                   const char* seq, const int64_t* offsets, uint64_t n_reads,
+                                   int16_t* counts) {
+    uint64_t total = 0, pos = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        const size_t len = (size_t)(offsets[r + 1] - offsets[r]);
+        if (len < (size_t)K) continue;
+        total += or_insert_and_query_sequence(st, K, canonical, mode, seq + offsets[r], len, counts + pos);
+        pos += len - (size_t)K + 1;
+    }
+    return total;
+}
+
+/* The value hashed for every window (min(fw, rc) or fw; 0 where skipped),
+ * packed per read. */
+uint64_t or_hash_reads(int K, int canonical, int mode, const char* seq, const int64_t* offsets,
+                       uint64_t n_reads, uint64_t* out) {
+    uint64_t pos = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        or_windows w;
+        const size_t len = (size_t)(offsets[r + 1] - offsets[r]);
+        if (!windows_of(seq + offsets[r], len, K, mode, &w)) continue;
+        for (size_t i = 0; i < w.n; ++i) out[pos + i] = w.valid[i] ? wval(&w, i, canonical) : 0;
+        pos += w.n;
+        windows_free(&w);
+    }
+    return pos;
+}
+
 /* DiginormFilter<dBG<ByteStorage, Sh>>::Filter::filter_sequence
  * (diginorm.hh:105-119) for one read: -1 when the read is shorter than K (the
  * k-mer iterator raises SequenceLengthException, kmeriterator.hh:72-74), 0
@@ -718,6 +787,41 @@ static void* mt_query(void* arg) {
     return NULL;
 }
 
+typedef struct {
+    or_mt_part base;
+    int min_count;
+    const uint8_t* only;
+    uint8_t* in_deg;
+    uint8_t* out_deg;
+} or_mt_deg;
+
+/* or_degrees_sequence for the windows with only[w] set (all when only is
+ * NULL); 0xFF elsewhere. */
+static void* mt_degrees(void* arg) {
+    or_mt_deg* d = (or_mt_deg*)arg;
+    or_mt_part* a = &d->base;
+    for (uint64_t r = a->r0; r < a->r1; ++r) {
+        const char* s = a->seq + a->offsets[r];
+        const size_t len = (size_t)(a->offsets[r + 1] - a->offsets[r]);
+        if (len < (size_t)a->K) continue;
+        const size_t n = len - (size_t)a->K + 1;
+        const int64_t k0 = a->kmer_off[r];
+        uint8_t* valid = (uint8_t*)malloc(n);
+        window_mask(s, len, a->K, a->mode, valid);
+        for (size_t i = 0; i < n; ++i) {
+            if (!valid[i] || (d->only && !d->only[k0 + i])) {
+                d->in_deg[k0 + i] = d->out_deg[k0 + i] = 0xFF;
+                continue;
+            }
+            or_kmer_degrees(a->st, a->K, a->canonical, d->min_count, s + i, &d->in_deg[k0 + i],
+                            &d->out_deg[k0 + i], NULL, NULL);
+            a->hashed++;
+        }
+        free(valid);
+    }
+    return NULL;
+}
+
 /* Reads split into n_threads ranges of about equal bases, each on its own
  * thread. */
 static uint64_t run_parts(void* (*fn)(void*), or_mt_part* proto, uint64_t n_reads, int n_threads) {
@@ -784,4 +888,45 @@ uint64_t or_query_reads(const or_storage* st, int K, int canonical, int mode,
         if (len >= (size_t)K) pos += len - (size_t)K + 1;
     }
     return total;
+}
+
+/* or_degrees_sequence over reads on n_threads threads, for the windows with
+ * only[w] set (every window when only is NULL); in/out degrees packed per
+ * read, 0xFF where skipped or not selected.  The storage is only read. */
+uint64_t or_degrees_reads_mt(const or_storage* st, int K, int canonical, int mode, int min_count,
+                             const char* seq, const int64_t* offsets, uint64_t n_reads, const uint8_t* only,
+                             uint8_t* in_deg, uint8_t* out_deg, int n_threads) {
+    if (n_threads < 1) n_threads = 1;
+    int64_t* ko = (int64_t*)malloc((n_reads + 1) * sizeof(int64_t));
+    ko[0] = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        const int64_t len = offsets[r + 1] - offsets[r];
+        ko[r + 1] = ko[r] + (len >= K ? len - K + 1 : 0);
+    }
+    or_mt_deg* parts = (or_mt_deg*)calloc((size_t)n_threads, sizeof(or_mt_deg));
+    pthread_t* th = (pthread_t*)calloc((size_t)n_threads, sizeof(pthread_t));
+    const int64_t total = n_reads ? offsets[n_reads] : 0;
+    uint64_t r = 0;
+    for (int t = 0; t < n_threads; ++t) {
+        or_mt_part p = {(or_storage*)st, K, canonical, mode, seq, offsets, ko, NULL, r, 0, 0};
+        const int64_t target = total * (t + 1) / n_threads;
+        while (r < n_reads && (t == n_threads - 1 || offsets[r] < target)) ++r;
+        p.r1 = r;
+        parts[t].base = p;
+        parts[t].min_count = min_count;
+        parts[t].only = only;
+        parts[t].in_deg = in_deg;
+        parts[t].out_deg = out_deg;
+    }
+    for (int t = 1; t < n_threads; ++t) pthread_create(&th[t], NULL, mt_degrees, &parts[t]);
+    mt_degrees(&parts[0]);
+    uint64_t done = parts[0].base.hashed;
+    for (int t = 1; t < n_threads; ++t) {
+        pthread_join(th[t], NULL);
+        done += parts[t].base.hashed;
+    }
+    free(parts);
+    free(th);
+    free(ko);
+    return done;
 }

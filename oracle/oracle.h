@@ -120,6 +120,18 @@ int      or_diginorm_filter(or_storage* st, int K, int canonical, int mode, unsi
 uint64_t or_diginorm_reads(or_storage* st, int K, int canonical, int mode, unsigned cutoff,
                            const char* seq, const int64_t* offsets, uint64_t n_reads, uint8_t* passed);
 
+uint64_t or_insert_reads_flags(or_storage* st, int K, int canonical, int mode,
+                               const char* seq, const int64_t* offsets, uint64_t n_reads,
+                               uint64_t* n_new, uint8_t* new_flags);
+uint64_t or_insert_and_query_reads(or_storage* st, int K, int canonical, int mode,
+                                   const char* seq, const int64_t* offsets, uint64_t n_reads,
+                                   int16_t* counts);
+uint64_t or_hash_reads(int K, int canonical, int mode, const char* seq, const int64_t* offsets,
+                       uint64_t n_reads, uint64_t* out);
+uint64_t or_degrees_reads_mt(const or_storage* st, int K, int canonical, int mode, int min_count,
+                             const char* seq, const int64_t* offsets, uint64_t n_reads, const uint8_t* only,
+                             uint8_t* in_deg, uint8_t* out_deg, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,13 @@ def lib():
                                              C.c_size_t]),
             'or_diginorm_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_char_p,
                                                i64p, C.c_uint64, u8p]),
+            'or_insert_reads_flags': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
+                                                   C.c_uint64, u64p, u8p]),
+            'or_insert_and_query_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
+                                                       C.c_uint64, i16p]),
+            'or_hash_reads': (C.c_uint64, [C.c_int, C.c_int, C.c_int, C.c_char_p, i64p, C.c_uint64, u64p]),
+            'or_degrees_reads_mt': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
+                                                 C.c_uint64, u8p, u8p, u8p, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -109,6 +116,21 @@ def hash_read(seq, K):
     rc = np.zeros(max(n, 1), np.uint64)
     lib().or_hash_read(b, len(b), K, _p(fw, C.c_uint64), _p(rc, C.c_uint64))
     return fw[:n], rc[:n]
+
+
+def _n_windows(offsets, K):
+    return int(np.maximum(np.diff(offsets) - K + 1, 0).sum())
+
+
+def hash_reads(seq, offsets, K, canonical=True, mode=MODE_RAW):
+    """The value of every window, packed per read (0 where skipped)."""
+    seq = np.ascontiguousarray(seq, np.uint8)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    n = _n_windows(offsets, K)
+    out = np.zeros(max(n, 1), np.uint64)
+    lib().or_hash_reads(K, int(canonical), mode, seq.ctypes.data_as(C.c_char_p), _p(offsets, C.c_int64),
+                        len(offsets) - 1, _p(out, C.c_uint64))
+    return out[:n]
 
 
 def canonical(fw, rc):
@@ -238,6 +260,44 @@ class Storage:
                                       seq.ctypes.data_as(C.c_char_p), _p(offsets, C.c_int64),
                                       n_reads, _p(nn, C.c_uint64))
         return total, nn[:n_reads]
+
+    def insert_reads_flags(self, seq, offsets, K, canonical=True, mode=MODE_RAW):
+        """insert_reads plus each window's first-touch flag (packed)."""
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        n_reads = len(offsets) - 1
+        nn = np.zeros(max(n_reads, 1), np.uint64)
+        n = _n_windows(offsets, K)
+        fl = np.zeros(max(n, 1), np.uint8)
+        total = lib().or_insert_reads_flags(self.ref, K, int(canonical), mode, seq.ctypes.data_as(C.c_char_p),
+                                            _p(offsets, C.c_int64), n_reads, _p(nn, C.c_uint64),
+                                            _p(fl, C.c_uint8))
+        return total, nn[:n_reads], fl[:n]
+
+    def insert_and_query_reads(self, seq, offsets, K, canonical=True, mode=MODE_RAW):
+        """insert_and_query_sequence over reads in order, counts packed."""
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        n = _n_windows(offsets, K)
+        out = np.zeros(max(n, 1), np.int16)
+        lib().or_insert_and_query_reads(self.ref, K, int(canonical), mode, seq.ctypes.data_as(C.c_char_p),
+                                        _p(offsets, C.c_int64), len(offsets) - 1, _p(out, C.c_int16))
+        return out[:n]
+
+    def degrees_reads_mt(self, seq, offsets, K, canonical=True, mode=MODE_RAW, min_count=1, only=None,
+                         threads=1):
+        """In / out degrees of the windows selected by `only` (every window
+        when None), packed; 0xFF elsewhere."""
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        n = _n_windows(offsets, K)
+        ind = np.zeros(max(n, 1), np.uint8)
+        outd = np.zeros(max(n, 1), np.uint8)
+        o = None if only is None else _p(np.ascontiguousarray(only, np.uint8), C.c_uint8)
+        lib().or_degrees_reads_mt(self.ref, K, int(canonical), mode, int(min_count),
+                                  seq.ctypes.data_as(C.c_char_p), _p(offsets, C.c_int64), len(offsets) - 1,
+                                  o, _p(ind, C.c_uint8), _p(outd, C.c_uint8), int(threads))
+        return ind[:n], outd[:n]
 
     def diginorm_filter(self, seq, K, cutoff, canonical=True, mode=MODE_RAW):
         """DiginormFilter::Filter::filter_sequence: 1 inserted, 0 dropped,

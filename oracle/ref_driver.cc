@@ -9,10 +9,13 @@
 #include "goetia/storage/bitstorage.hh"
 #include "goetia/storage/bytestorage.hh"
 #include "goetia/storage/sparseppstorage.hh"
+#include "goetia/storage/phmapstorage.hh"
+#include "goetia/storage/btreestorage.hh"
 #include "goetia/hashing/hashshifter.hh"
 #include "goetia/hashing/kmeriterator.hh"
 #include "goetia/parsing/readers.hh"
 #include "goetia/diginorm.hh"
+#include "goetia/solidifier.hh"
 #include "goetia/cdbg/compactor.hh"
 
 #include <cstdint>
@@ -91,6 +94,27 @@ template <class G>
 uint64_t diginorm_run(void* graph, unsigned cutoff, const char* seq, const int64_t* off, uint64_t n_reads,
                       uint8_t* passed) {
     auto f = DiginormFilter<G>::Filter::build(static_cast<Holder<G>*>(graph)->g, cutoff);
+    uint64_t n = 0;
+    for (uint64_t r = 0; r < n_reads; ++r) {
+        std::string s(seq + off[r], (size_t)(off[r + 1] - off[r]));
+        passed[r] = 0;
+        try {
+            bool p = std::get<0>(f->filter_sequence(s));
+            passed[r] = p ? 1 : 0;
+            n += p;
+        } catch (SequenceLengthException&) {
+        }
+    }
+    return n;
+}
+
+// StreamingSolidFilter<G>::Filter::filter_sequence (solidifier.hh:58-73)
+// over reads in order; a read shorter than K raises SequenceLengthException
+// and does not pass, as in FilterProcessor::process_sequence.
+template <class G>
+uint64_t solid_run(void* graph, float min_prop, uint32_t threshold, const char* seq, const int64_t* off,
+                   uint64_t n_reads, uint8_t* passed) {
+    auto f = StreamingSolidFilter<G>::Filter::build(static_cast<Holder<G>*>(graph)->g, min_prop, threshold);
     uint64_t n = 0;
     for (uint64_t r = 0; r < n_reads; ++r) {
         std::string s(seq + off[r], (size_t)(off[r + 1] - off[r]));
@@ -411,6 +435,84 @@ uint64_t ref_find_new_segments(void* hv, const char* seq, const int64_t* off, ui
     if (h->kind == 1) return run(static_cast<Holder<Graph<BitStorage, FwdLemireShifter>>*>(h->graph));
     if (h->kind == 2) return run(static_cast<Holder<Graph<ByteStorage, FwdLemireShifter>>*>(h->graph));
     return run(static_cast<Holder<Graph<SparseppSetStorage, FwdLemireShifter>>*>(h->graph));
+}
+
+// StreamingSolidFilter over reads (Bloom or count-min graphs); returns the
+// number of reads that pass.
+uint64_t ref_solid_filter_reads(void* hv, float min_prop, uint32_t threshold, const char* seq, const int64_t* off,
+                                uint64_t n_reads, uint8_t* passed) {
+    Handle* h = (Handle*)hv;
+    if (h->kind == 1) {
+        if (h->canonical)
+            return solid_run<Graph<BitStorage, CanLemireShifter>>(h->graph, min_prop, threshold, seq, off, n_reads, passed);
+        return solid_run<Graph<BitStorage, FwdLemireShifter>>(h->graph, min_prop, threshold, seq, off, n_reads, passed);
+    }
+    if (h->kind == 2) {
+        if (h->canonical)
+            return solid_run<Graph<ByteStorage, CanLemireShifter>>(h->graph, min_prop, threshold, seq, off, n_reads, passed);
+        return solid_run<Graph<ByteStorage, FwdLemireShifter>>(h->graph, min_prop, threshold, seq, off, n_reads, passed);
+    }
+    return ~0ULL;
+}
+
+// dBG::save(filename) (dbg.hh:396-399): the graph's storage writes its OXLI
+// file (BitStorage::save, ByteStorage::save through ByteStorageFile).
+void ref_save(void* hv, const char* path) {
+    Handle* h = (Handle*)hv;
+    dispatch(h, [&](auto* g) {
+        g->save(std::string(path));
+        return 0;
+    });
+}
+
+// A Bloom (kind 1) or count-min (kind 2) storage loaded from an OXLI file by
+// the reference's loader: ksize, table count, sizes (room for 16), occupied
+// bins, and the tables concatenated into out (cap bytes).  Returns the bytes
+// the tables need, or ~0 when the loader threw.
+uint64_t ref_load_file(int kind, const char* path, uint16_t* ksize, uint32_t* n_tables, uint64_t* sizes,
+                       uint64_t* occupied, uint8_t* out, uint64_t cap) {
+    auto run = [&](auto store) -> uint64_t {
+        uint16_t k = 0;
+        try {
+            store->load(std::string(path), k);
+        } catch (std::exception&) {
+            return ~0ULL;
+        }
+        *ksize = k;
+        auto ts = store->get_tablesizes();
+        *n_tables = (uint32_t)ts.size();
+        *occupied = store->n_occupied();
+        byte_t** t = store->get_raw_tables();
+        uint64_t w = 0;
+        for (size_t i = 0; i < ts.size() && i < 16; ++i) {
+            sizes[i] = ts[i];
+            const uint64_t nb = kind == 1 ? ts[i] / 8 + 1 : ts[i];
+            for (uint64_t j = 0; j < nb; ++j, ++w)
+                if (w < cap) out[w] = t[i][j];
+        }
+        return w;
+    };
+    if (kind == 1) return run(BitStorage::build(1000, 1));
+    return run(ByteStorage::build(1000, 1));
+}
+
+// Storage-level insert trace of an exact set (flavor 0 SparseppSetStorage,
+// 1 PHMapStorage, 2 BTreeStorage): before value i, reset() when
+// reset_before[i]; after it, the insert's result and the storage's
+// n_unique_kmers() / n_occupied().
+void ref_set_trace(int flavor, const uint64_t* h, uint64_t n, const uint8_t* reset_before, uint8_t* is_new,
+                   uint64_t* n_unique, uint64_t* n_occupied) {
+    auto run = [&](auto store) {
+        for (uint64_t i = 0; i < n; ++i) {
+            if (reset_before[i]) store->reset();
+            is_new[i] = store->insert(h[i]) ? 1 : 0;
+            n_unique[i] = store->n_unique_kmers();
+            n_occupied[i] = store->n_occupied();
+        }
+    };
+    if (flavor == 0) run(SparseppSetStorage::build());
+    else if (flavor == 1) run(PHMapStorage::build());
+    else run(BTreeStorage::build());
 }
 
 void ref_free(void* hv) {

@@ -210,16 +210,82 @@ def test_serving_results_same_alone_and_in_mix(slots, policy):
     assert st['completed'] == len(chunks) and st['resident_slots'] == 2
     assert st['evictions'] > 0 and st['loads'] > 0   # the budget forces evictions
     assert all(x >= 0 for x in lat)
-    # each stream alone, its chunks in order, through the plain dBG path
+    # each stream alone, its chunks in order, through the plain dBG path and
+    # through the oracle
+    from oracle import pyoracle as po
     alone = {s: lg.dBG[lg.BitStorage, lg.CanLemireShifter].build(lg.BitStorage(sizes), K) for s in range(n_streams)}
+    orc = {s: po.Storage(po.BIT, sizes) for s in range(n_streams)}
     for (s, b), (_, out) in zip(chunks, outs):
         exp = alone[s].insert_sequences(b)
         np.testing.assert_array_equal(out[:b.n_reads], exp)
+        _, o_nn = orc[s].insert_reads(b.seq, b.offsets, K, True, po.MODE_RAW)
+        np.testing.assert_array_equal(out[:b.n_reads], o_nn)
     for s in range(n_streams):
         tabs, ctr = sv.stream_tables(s)
-        for x, y in zip(tabs, alone[s].get_raw()):
+        for x, y, z in zip(tabs, alone[s].get_raw(), orc[s].tables()):
             np.testing.assert_array_equal(x, y)
-        assert ctr == [alone[s].n_unique(), alone[s].n_occupied()]
+            np.testing.assert_array_equal(x, z)
+        assert ctr == [alone[s].n_unique(), alone[s].n_occupied()] == [orc[s].n_unique, orc[s].n_occupied]
+    sv.close()
+
+
+def test_serving_64_streams_of_2e30_bit_tables():
+    """The serving configuration's shape: 64 streams, each its own
+    BitStorage(2^30, 4) (537 MB), K=31, an HBM budget of 8 resident streams
+    (so most chunks evict another stream to host memory and load their own),
+    chunks of 1k-6k reads of 100-250 bp at 1% error from per-stream genomes:
+    96 chunks submitted at once, then 96 more with Poisson arrivals.  Every
+    chunk's per-read n_new and every stream's tables and counters in the mix
+    (four execution slots, resident-first scheduling) equal the oracle fed
+    that stream's chunks alone, in order."""
+    import time
+
+    from goetia_amd.serving import Server, make_chunks, make_streams
+    from oracle import pyoracle as po
+    n_streams, K = 64, 31
+    sizes = lg.get_n_primes_near_x(4, 2 ** 30)
+    per_stream = sum(s // 8 + 1 for s in sizes)
+    assert per_stream == 536_870_881
+    genomes = make_streams(n_streams, seed=100, min_mbp=5, max_mbp=50, scale=0.1)
+    first = make_chunks(genomes, 96, seed=7, lo=1000, hi=6000, median=2000, sigma=0.8)
+    second = make_chunks(genomes, 96, seed=8, lo=1000, hi=6000, median=2000, sigma=0.8)
+    assert len({s for s, _ in first}) > 40
+    sv = Server(lg.BitStorage, sizes, K, n_streams, hbm_budget=8 * per_stream, n_slots=4, policy=1)
+    t0 = time.perf_counter()
+    tickets = [sv.submit(s, b) for s, b in first]
+    lat = [sv.wait(t) for t, _ in tickets]
+    wall = time.perf_counter() - t0
+    sv.drain()
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(wall / len(second), len(second)))
+    t0 = time.perf_counter()
+    for i, (s, b) in enumerate(second):
+        d = arrivals[i] - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        tickets.append(sv.submit(s, b))
+    lat2 = [sv.wait(t) for t, _ in tickets[len(first):]]
+    sv.drain()
+    st = sv.stats()
+    print(f'serving 64 x BitStorage(2^30): saturated p99 {np.percentile(lat, 99) * 1e3:.1f} ms, '
+          f'Poisson p99 {np.percentile(lat2, 99) * 1e3:.1f} ms, loads {st["loads"]}, '
+          f'evictions {st["evictions"]}', flush=True)
+    assert st['completed'] == len(first) + len(second) and st['resident_slots'] == 8
+    assert st['evictions'] > 0 and st['loads'] > 0
+    chunks = first + second
+    by_stream = {}
+    for i, (s, _) in enumerate(chunks):
+        by_stream.setdefault(s, []).append(i)
+    for s in range(n_streams):
+        o = po.Storage(po.BIT, sizes)
+        for i in by_stream.get(s, []):
+            b = chunks[i][1]
+            _, nn = o.insert_reads(b.seq, b.offsets, K, True, po.MODE_RAW)
+            np.testing.assert_array_equal(tickets[i][1][:b.n_reads], nn, err_msg=f'stream {s} chunk {i}')
+        tabs, ctr = sv.stream_tables(s)
+        assert ctr == [o.n_unique, o.n_occupied], s
+        for x, y in zip(tabs, o.tables()):
+            assert np.array_equal(x, y), s
+        del o, tabs
     sv.close()
 
 

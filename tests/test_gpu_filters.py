@@ -1,49 +1,56 @@
-"""StreamingSolidFilter on the GPU against the sequential oracle: each read's
-counts come from insert_and_query in read order (solidifier.hh:58-73) and the
-pass decision uses the reference's float ratio against 1.0 - min_prop_solid."""
+"""StreamingSolidFilter on the GPU against goetia's compiled filter
+(tests/golden/solid_ref.npz, written by tests/golden/make_solid_golden.py):
+every read's pass decision (solidifier.hh:58-73: insert_and_query in read
+order, then goetia's float ratio of non-solid k-mers against
+1.0 - min_prop_solid; reads shorter than K never pass), and the tables and
+counters goetia's graph was left with -- batched in two calls, and one read
+at a time."""
+import hashlib
+import os
+
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+from conftest import GOLDEN_DIR
 
-from goetia_amd import libgoetia as lg  # noqa: E402
-from goetia_amd import workload  # noqa: E402
-
-
-def expected(oracle, kind, sizes, reads, K, min_prop, threshold):
-    o = oracle.Storage(kind, sizes)
-    out = []
-    for r in reads:
-        if len(r) < K:
-            out.append(False)
-            continue
-        counts = o.insert_and_query_sequence(r, K, True)
-        n_not = int(np.sum(counts.astype(np.int64) < threshold))
-        ratio = np.float32(n_not) / np.float32(len(r) - K + 1)
-        out.append(not (float(ratio) >= 1.0 - float(np.float32(min_prop))))
-    return np.array(out), o
+GOLD = np.load(os.path.join(GOLDEN_DIR, 'solid_ref.npz'), allow_pickle=False)
+CASES = sorted(k[:-len('__params')] for k in GOLD.files if k.endswith('__params'))
 
 
-@pytest.mark.parametrize('kind,threshold,min_prop', [(2, 1, 0.75), (2, 2, 0.75), (2, 3, 0.5), (2, 2, 0.9),
-                                                     (1, 1, 0.75)])
-def test_solid_filter_vs_oracle(oracle, kind, threshold, min_prop):
+def golden_inputs():
+    from goetia_amd import workload
     genome = workload.genome(81, 30_000)
-    gen = workload.reads(82, genome, 2500, 10, 200, 0.02, 0.0)
-    reads = [gen.read(i) for i in range(gen.n_reads)]
-    K = 21
-    sizes = oracle.primes_near_x(4, 200_000)
+    b = workload.reads(82, genome, 2500, 10, 200, 0.02, 0.0)
+    h = hashlib.sha256(np.ascontiguousarray(b.seq, np.uint8).tobytes())
+    h.update(np.ascontiguousarray(b.offsets, '<i8').tobytes())
+    return b, h.digest()
+
+
+def test_golden_inputs_regenerate():
+    _, d = golden_inputs()
+    assert d == GOLD['input_sha256'].tobytes()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', CASES)
+def test_solid_filter_vs_reference(name):
+    from goetia_amd import libgoetia as lg
+    kind, can, K, mt, nt, thr = (int(v) for v in GOLD[f'{name}__params'])
+    prop = float(GOLD[f'{name}__min_prop'][0])
+    exp = GOLD[f'{name}__passed'].astype(bool)
+    batch, _ = golden_inputs()
+    reads = [batch.read(i) for i in range(batch.n_reads)]
     cls = lg.ByteStorage if kind == 2 else lg.BitStorage
-    g = lg.dBG[cls, lg.CanLemireShifter].build(cls(sizes), K)
-    f = lg.StreamingSolidFilter.build(g, min_prop, threshold)
-    got = f.filter_sequences(reads[:1200])
-    got = np.concatenate([got, f.filter_sequences(reads[1200:])])
-    exp, o = expected(oracle, kind, sizes, reads, K, min_prop, threshold)
+    sh = lg.CanLemireShifter if can else lg.FwdLemireShifter
+    g = lg.dBG[cls, sh].build(cls.build(mt, nt), K)
+    f = lg.StreamingSolidFilter.build(g, prop, thr)
+    got = np.concatenate([f.filter_sequences(reads[:1200]), f.filter_sequences(reads[1200:])])
     np.testing.assert_array_equal(got, exp)
-    assert 0 < exp.sum() < len(exp) or kind == 1
-    for a, b in zip(g.get_raw(), o.tables()):
-        np.testing.assert_array_equal(a, b)
+    assert [g.n_unique(), g.n_occupied()] == [int(v) for v in GOLD[f'{name}__counters']]
+    for i, t in enumerate(g.get_raw()):
+        assert hashlib.sha256(t.tobytes()).digest() == GOLD[f'{name}__table_sha'][i].tobytes(), (name, i)
     # one read at a time gives the same decisions
-    g2 = lg.dBG[cls, lg.CanLemireShifter].build(cls(sizes), K)
-    f2 = lg.StreamingSolidFilter.build(g2, min_prop, threshold)
+    g2 = lg.dBG[cls, sh].build(cls.build(mt, nt), K)
+    f2 = lg.StreamingSolidFilter.build(g2, prop, thr)
     one = [f2.filter_sequence(r)[0] if len(r) >= K else False for r in reads[:300]]
     np.testing.assert_array_equal(one, exp[:300])

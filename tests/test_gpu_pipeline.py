@@ -1,5 +1,8 @@
 """The batched pipeline (captured hipGraph replays over double-buffered
-batches) and every tile shape give exactly the plain path's results."""
+batches) gives exactly the oracle's sequential results (per-read n_new of
+every batch, tables or set contents, counters), the plain path's too, and so
+does every tile shape; the uneven-read configuration at 1/100 scale through
+the pipeline equals the oracle."""
 import numpy as np
 import pytest
 
@@ -29,6 +32,24 @@ def split(batch, n_parts):
     return out
 
 
+def oracle_storage(oracle, kind):
+    if kind == 'bit':
+        return oracle.Storage(oracle.BIT, oracle.primes_near_x(4, 2_000_003))
+    if kind == 'byte':
+        return oracle.Storage(oracle.BYTE, oracle.primes_near_x(4, 1_000_003))
+    return oracle.Storage(oracle.EXACT)
+
+
+def check_against_oracle(o, graph, kind):
+    assert graph.n_unique() == o.n_unique
+    assert graph.n_occupied() == o.n_occupied
+    if kind == 'exact':
+        np.testing.assert_array_equal(graph.S.keys(), o.exact_keys())
+    else:
+        for x, y in zip(graph.get_raw(), o.tables()):
+            np.testing.assert_array_equal(x, y)
+
+
 def make_graph(kind, K=25, cap=0):
     if kind == 'bit':
         st = lg.BitStorage.build(2_000_003, 4)
@@ -40,7 +61,7 @@ def make_graph(kind, K=25, cap=0):
 
 
 @pytest.mark.parametrize('kind', ['bit', 'byte', 'exact'])
-def test_pipeline_matches_plain_path(kind):
+def test_pipeline_matches_plain_path(oracle, kind):
     a, b = uneven_batch(31, 60, 3000)
     parts = split(a, 3) + split(b, 4)
     plain = make_graph(kind)
@@ -50,8 +71,12 @@ def test_pipeline_matches_plain_path(kind):
     max_bases = max(p.n_bases for p in parts)
     with Pipeline(piped, max_reads, max_bases) as pipe:
         outs = [pipe.submit(p) for p in parts]
-    for e, o, p in zip(exp, outs, parts):
-        np.testing.assert_array_equal(o[:p.n_reads], e)
+    o = oracle_storage(oracle, kind)
+    for e, out, p in zip(exp, outs, parts):
+        _, o_nn = o.insert_reads(p.seq, p.offsets, 25, True, oracle.MODE_RAW)
+        np.testing.assert_array_equal(out[:p.n_reads], o_nn)
+        np.testing.assert_array_equal(out[:p.n_reads], e)
+    check_against_oracle(o, piped, kind)
     assert piped.n_unique() == plain.n_unique()
     if kind == 'exact':
         np.testing.assert_array_equal(piped.S.keys(), plain.S.keys())
@@ -59,6 +84,25 @@ def test_pipeline_matches_plain_path(kind):
         for x, y in zip(piped.get_raw(), plain.get_raw()):
             np.testing.assert_array_equal(x, y)
         assert piped.n_occupied() == plain.n_occupied()
+
+
+def test_pipeline_uneven_reads_vs_oracle(oracle):
+    """The uneven-read configuration (lognormal long reads, median 8 kb,
+    clipped to 150 bp-100 kb, among 150 bp reads, 5% substitutions, K=25,
+    exact set) at 1/100 scale, in 8 batches through the pipeline: every
+    batch's n_new, the set contents and counters equal the oracle's
+    sequential insert."""
+    cfg, batch = workload.exact_mixed(0.01)
+    K = cfg['K']
+    parts = split(batch, 8)
+    g = make_graph('exact', K=K)
+    with Pipeline(g, max(p.n_reads for p in parts), max(p.n_bases for p in parts)) as pipe:
+        outs = [pipe.submit(p) for p in parts]
+    o = oracle.Storage(oracle.EXACT)
+    for out, p in zip(outs, parts):
+        _, o_nn = o.insert_reads(p.seq, p.offsets, K, True, oracle.MODE_RAW)
+        np.testing.assert_array_equal(out[:p.n_reads], o_nn)
+    check_against_oracle(o, g, 'exact')
 
 
 def test_pipeline_exact_set_grows_between_batches():
