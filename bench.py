@@ -74,6 +74,8 @@ def parse():
                    help='cms_20m is the headline; the others print their own JSON line')
     p.add_argument('--plain-chunk-kmers', type=int, default=500_000,
                    help='chunk size of the plain path in k-mers (goetia IntervalCounter default)')
+    p.add_argument('--serving-chunks', type=int, default=120,
+                   help='chunks of the served-vs-plain comparison (64 streams of BitStorage(2^30, 4)); 0 skips it')
     return p.parse_args()
 
 
@@ -237,6 +239,8 @@ def main():
     roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS,
                 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBPS, 4),
                 'traffic': round(traffic * n_kmers / launches) if traffic else None,
+                'traffic_source': 'profile: rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per k-mer of this kernel on '
+                                  'this workload (profiles/cms_20m_hbm_traffic.md) x k-mers per launch',
                 'algorithmic_bytes_per_launch': round(BYTES_PER_KMER[dom] * n_kmers / launches),
                 'launches_per_step': launches,
                 'bytes_per_kmer': BYTES_PER_KMER[dom],
@@ -295,8 +299,21 @@ def main():
             ra['query_frac_of_random_read'] = round(q_rate / ra['read_ops_per_s_table_size'], 4)
         roofline['random_access'] = ra
 
+    tablesizes = store.get_tablesizes()
+    # The served form against the plain one-chunk-at-a-time path on the same
+    # chunks (the 64-stream serving configuration, fewer chunks): aggregate
+    # rate ratio and chunk latency under Poisson arrivals.
+    served = None
+    if args.serving_chunks:
+        del reads, counts, n_new
+        torch.cuda.empty_cache()
+        from goetia_amd import serving
+        served = serving.serving_64(1.0, n_chunks=args.serving_chunks)
+        log(f"served vs plain: {served['served_over_plain']}x, Poisson p99 "
+            f"{served['poisson']['served']['p99_ms']} ms served, {served['poisson']['plain']['p99_ms']} ms plain")
+
     out = {
-        'metric': 'canonical k-mers/s insert+query (1 GPU)',
+        'metric': 'canonical k-mers/s insert+query (1 GPU) vs plain; p50/p99 chunk latency',
         'value': round(value, 1),
         'unit': 'kmers/s',
         'n_gpus': 1,
@@ -312,10 +329,20 @@ def main():
                                'K=31 canonical, ByteStorage(2^28, 4); step = reset + insert (n_new) + '
                                'query_sequence' % batch.n_reads,
                    'reads': batch.n_reads, 'kmers_per_step': n_kmers, 'K': K,
-                   'storage': 'ByteStorage', 'tablesizes': store.get_tablesizes()},
+                   'storage': 'ByteStorage', 'tablesizes': tablesizes},
         'roofline': roofline,
         'insert_and_query_kmers_per_s': iq_rate,
         'chunk_latency': chunk,
+        'vs_plain': served['served_over_plain'] if served else None,
+        'served': None if served is None else {
+            'workload': 'serving_64: 64 streams x BitStorage(2^30, 4), K=31, 16 GiB budget, '
+                        f"{served['chunks']} lognormal chunks of 100-250 bp reads",
+            'served_kmers_per_s': served['value'], 'plain_kmers_per_s': served['plain_kmers_per_s'],
+            'served_over_plain': served['served_over_plain'],
+            'p50_ms': served['poisson']['served']['p50_ms'], 'p99_ms': served['poisson']['served']['p99_ms'],
+            'plain_p50_ms': served['poisson']['plain']['p50_ms'], 'plain_p99_ms': served['poisson']['plain']['p99_ms'],
+            'arrivals_per_s': served['poisson']['rate_chunks_per_s'],
+            'served_equals_plain': served['served_equals_plain']},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

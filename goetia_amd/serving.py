@@ -119,8 +119,16 @@ def run(server, chunks, arrivals=None):
 
 
 def bench(args):
+    return serving_64(args.scale)
+
+
+def serving_64(scale=1.0, n_chunks=None):
+    """The serving configuration: 64 streams of BitStorage(2^30, 4), K=31, a
+    16 GiB HBM budget (32 resident streams), lognormal chunks; the plain
+    one-chunk-at-a-time path against the served path (4 slots, resident
+    first) on the same chunks, saturated and with Poisson arrivals at 80% of
+    the plain path's capacity.  `n_chunks` (default 400) trims the run."""
     from . import libgoetia as lg
-    scale = args.scale
     n_streams = 64
     K = 31
     sizes = lg.get_n_primes_near_x(4, 2 ** 30 if scale >= 1 else max(2 ** 20, int(2 ** 30 * scale)))
@@ -128,7 +136,8 @@ def bench(args):
     budget = int(16 * 2 ** 30 * min(1.0, scale)) if scale < 1 else 16 * 2 ** 30
     t0 = time.perf_counter()
     genomes = make_streams(n_streams, scale=scale)
-    n_chunks = max(16, int(400 * min(1.0, scale * 10)))
+    if n_chunks is None:
+        n_chunks = max(16, int(400 * min(1.0, scale * 10)))
     chunks = make_chunks(genomes, n_chunks, scale=min(1.0, scale * 10))
     kmers = int(sum(int(b.kmer_counts(K).sum()) for _, b in chunks))
     gen_s = time.perf_counter() - t0
