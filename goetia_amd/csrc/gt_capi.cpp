@@ -1379,8 +1379,8 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         st->own.bin_vq.release();
     }
     if (st->kind == kByte) {
-        // More than four tables or 2^30 bins: hash the batch, then apply it in
-        // k-mer order on one lane.
+        // More than four tables or 2^30 bins: hash the batch, then apply it
+        // with the sort-based exact insert_and_query.
         GT_TRY(prepare_reads(st, R, true, s));
         int64_t nk = 0;
         GT_HIP(hipMemcpyAsync(&nk, st->own.kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
@@ -1392,11 +1392,11 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         GT_TRY(valid.ensure((size_t)nk));
         const Launch L = make_launch(st, s);
         launch_hash(L, R, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>());
-        launch_seq_insert_query(fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), (uint64_t)nk, R.canonical,
-                                st->tables_view(), counts, nullptr, st->counters.as<uint64_t>(), s);
         GT_HIP(hipGetLastError());
-        GT_HIP(hipStreamSynchronize(s));   // scratch is released on return
-        return GT_OK;
+        GT_TRY(sorted_insert_query(st->tables_view(), fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(),
+                                   (uint64_t)nk, R.canonical, counts, nullptr, st->counters.as<uint64_t>(), g_dev.cus,
+                                   s));
+        return GT_OK;   // (synchronised: the hash buffers are released on return)
     }
     GT_TRY(dbg_insert_impl(st, R, nullptr, nullptr, s));
     // Presence storages return 1 after every insert (bitstorage.cc:78-84,
@@ -1825,8 +1825,8 @@ int gt_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t*
 }
 
 // Storage::insert_and_query per hash in index order.  Count-min results
-// depend on every earlier insert of the same bins, so they are applied by one
-// device lane in index order, which is exact.
+// depend on every earlier insert of the same bins: the sort-based exact
+// insert_and_query gives each hash its bins' counts in index order.
 int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, int16_t* counts) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
@@ -1838,9 +1838,8 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
     GT_TRY(st->h_out2.ensure(n));
     GT_TRY(st->h_out1.ensure(n * sizeof(int16_t)));
     if (st->kind == kByte) {
-        launch_seq_insert_query(st->h_out3.as<uint64_t>(), nullptr, nullptr, n, 0, st->tables_view(),
-                                st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), s);
-        GT_HIP(hipGetLastError());
+        GT_TRY(sorted_insert_query(st->tables_view(), st->h_out3.as<uint64_t>(), nullptr, nullptr, n, 0,
+                                   st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), g_dev.cus, s));
         GT_TRY(stage_out(counts, st->h_out1, n, s));
         GT_HIP(hipStreamSynchronize(s));
     } else {

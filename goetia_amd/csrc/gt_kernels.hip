@@ -927,43 +927,6 @@ __global__ __launch_bounds__(kBlock) void hash_query_kernel(const uint64_t* h, u
     }
 }
 
-// Count-min insert_and_query in strict index order (bytestorage.cc:60-150):
-// one lane walks the k-mers, so every result sees exactly the inserts before
-// it.  Used for per-read calls, whose k-mer counts are small.  valid may be
-// null; invalid entries get -1 and no insert.
-__global__ void seq_insert_query_kernel(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n,
-                                        int canonical, Tables T, int16_t* counts, uint8_t* is_new,
-                                        uint64_t* counters) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    uint64_t nn = 0, occ = 0;
-    for (uint64_t k = 0; k < n; ++k) {
-        if (valid && !valid[k]) {
-            if (counts) counts[k] = -1;
-            if (is_new) is_new[k] = 0;
-            continue;
-        }
-        const uint64_t h = rc ? kmer_value(fw[k], rc[k], canonical) : fw[k];
-        bool nw = false;
-        int16_t m = 255;
-        GT_FOR_TABLES(i, T) {
-            uint8_t* cell = T.base[i] + gt_fastmod(h, T.p[i], T.m[i]);
-            const uint8_t c = *cell;
-            if (!nw && c == 0) {
-                nw = true;
-                if (i == 0) occ++;
-            }
-            const uint8_t after = c < 255 ? (uint8_t)(c + 1) : c;
-            *cell = after;
-            m = after < m ? after : m;
-        }
-        nn += nw;
-        if (counts) counts[k] = nw ? 1 : m;
-        if (is_new) is_new[k] = nw;
-    }
-    counters[kNUnique] += nn;
-    counters[kNOccupied] += occ;
-}
-
 // ------------------------------------------------------------ read layout
 
 // tile_read[t] = read holding base t*kTileGrain (reads are [off[r], off[r+1])).
@@ -1219,10 +1182,22 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
     return off + x - v;
 }
 
+// The first pass keeps a tile's per-window values in LDS with one padding
+// word after every 64: thread t writes windows 8t..8t+7, so the 64 lanes of a
+// store touch 64 different banks (unpadded, stride 8 puts them on 8), and the
+// ranking pass, which reads consecutive windows, stays conflict-free too.
+#ifndef GT_BIN_L1_PAD
+#define GT_BIN_L1_PAD 1
+#endif
+constexpr int kBinTileP = kBinTile + (GT_BIN_L1_PAD ? kBinTile / 64 : 0);
+__device__ __forceinline__ int bin_pw(int w) { return GT_BIN_L1_PAD ? w + (w >> 6) : w; }
+// LDS index of event e = table * kBinTile + window
+__device__ __forceinline__ int bin_pe(int e) { return (e / kBinTile) * kBinTileP + bin_pw(e % kBinTile); }
+
 struct BinL1Shared {
-    uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
-    uint32_t ord[kBinTile];
-    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
+    uint32_t g[kBinMaxTables * kBinTileP];  // table-major: [table][window]; kNoBin = no event
+    uint32_t ord[kBinTileP];
+    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index (table * kBinTile + window)
     uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
     uint32_t part[kBlock];
 };
@@ -1250,7 +1225,7 @@ struct BinL1Op {
         t0 = tile_base;
         const int w0 = threadIdx.x * kBinPer;
         for (uint32_t t = 0; t < P.n_tables; ++t)
-            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
+            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTileP + bin_pw(w0 + j)] = kNoBin;
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
@@ -1262,12 +1237,12 @@ struct BinL1Op {
         const uint32_t ord = (uint32_t)(s + kd);
         if (P.hasev) P.hasev[ord] = valid ? 1 : 0;
         if (!valid) return;
-        const int w = (int)(s - t0);
+        const int w = bin_pw((int)(s - t0));
         const uint64_t h = kmer_value(fw, rc, canonical);
         sh->ord[w] = ord;
 #pragma unroll
         for (uint32_t i = 0; i < kBinMaxTables; ++i)
-            if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + gt_fastmod30(h, (uint32_t)T.p[i], T.m[i]);
+            if (i < P.n_tables) sh->g[i * kBinTileP + w] = P.gbase[i] + gt_fastmod30(h, (uint32_t)T.p[i], T.m[i]);
     }
 
     __device__ void flush() {}
@@ -1288,7 +1263,7 @@ struct BinL1Op {
         for (int c = 0; c < per_wave; c += 4 * 64) {        // per_wave is a multiple of 512
             uint32_t d[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[e0 + c + q * 64 + lane], P.shift2);
+            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[bin_pe(e0 + c + q * 64 + lane)], P.shift2);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (d[q] != (uint32_t)kDigits1) atomicAdd(&wc[d[q]], 1u);
@@ -1310,7 +1285,7 @@ struct BinL1Op {
         __syncthreads();
         for (int c = 0; c < per_wave; c += 64) {
             const int e = e0 + c + lane;
-            const uint32_t dd = bin_d1(sh->g[e], P.shift2);
+            const uint32_t dd = bin_d1(sh->g[bin_pe(e)], P.shift2);
             const bool act = dd != (uint32_t)kDigits1;
             const uint64_t m = match_lds(sl, dd, act);
             if (act) sh->perm[wc[dd] + (uint32_t)__popcll(m & lt)] = (uint16_t)e;
@@ -1330,7 +1305,7 @@ struct BinL1Op {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t i = i0 + q * kBlock + threadIdx.x;
-                if (i < n_ev) dst[i] = ((uint64_t)sh->g[e[q]] << 32) | sh->ord[e[q] & (kBinTile - 1)];
+                if (i < n_ev) dst[i] = ((uint64_t)sh->g[bin_pe(e[q])] << 32) | sh->ord[bin_pw(e[q] & (kBinTile - 1))];
             }
         }
     }
@@ -2262,11 +2237,6 @@ void launch_bit_union(uint32_t* me, const uint32_t* other, uint64_t n_words, int
 void launch_exact_rehash(const ExactSet& from, const ExactSet& to, uint64_t* counters, hipStream_t s) {
     const uint64_t g = (from.cap + 255) / 256;
     exact_rehash_kernel<<<(unsigned)(g < 16384 ? g : 16384), 256, 0, s>>>(from, to, counters);
-}
-
-void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n, int canonical,
-                             const Tables& T, int16_t* counts, uint8_t* is_new, uint64_t* counters, hipStream_t s) {
-    seq_insert_query_kernel<<<1, 64, 0, s>>>(fw, rc, valid, n, canonical, T, counts, is_new, counters);
 }
 
 void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s) {

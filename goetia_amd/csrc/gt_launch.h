@@ -51,8 +51,12 @@ void launch_hash_query(const Launch& L, const uint64_t* h, uint64_t n, const Tab
 void launch_bit_union(uint32_t* me, const uint32_t* other, uint64_t n_words, int count_bits, uint64_t* counters,
                       hipStream_t s);
 void launch_exact_rehash(const ExactSet& from, const ExactSet& to, uint64_t* counters, hipStream_t s);
-void launch_seq_insert_query(const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n, int canonical,
-                             const Tables& T, int16_t* counts, uint8_t* is_new, uint64_t* counters, hipStream_t s);
+// Count-min insert_and_query of n values in index order (fw[k], or
+// min(fw, rc) when rc is set and canonical), sort-based (gt_sorted_iq.hip);
+// synchronises the stream before it returns.
+int sorted_insert_query(const Tables& T, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n,
+                        int canonical, int16_t* counts, uint8_t* is_new, uint64_t* counters, int cus,
+                        hipStream_t s);
 void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s);
 
 // Binned insert of Bloom / count-min tables (one sub-batch = kbase, l1,

@@ -348,12 +348,13 @@ def test_binned_insert_vs_oracle(oracle, kind, sub_batch, monkeypatch):
 
 
 @pytest.mark.parametrize('sub_batch', [0, 8192 * 3])
-@pytest.mark.parametrize('n_tables', [1, 3, 4])
+@pytest.mark.parametrize('n_tables', [1, 3, 4, 5, 16])
 def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatch):
-    """Count-min insert_and_query through the binned replay: every window's
-    count just after its own insert (1 when new), with hot k-mers past 255,
-    skip-N windows (-1) and several sub-batches, equals the sequential
-    oracle; counters and tables too."""
+    """Count-min insert_and_query through the binned replay (up to four
+    tables) and the sort-based path (five or more): every window's count just
+    after its own insert (1 when new), with hot k-mers past 255, skip-N
+    windows (-1) and several sub-batches, equals the sequential oracle;
+    counters and tables too."""
     from goetia_amd import workload
     if sub_batch:
         monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', str(sub_batch))
@@ -369,4 +370,45 @@ def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatc
     np.testing.assert_array_equal(got, exp)
     assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
     for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_byte_insert_and_query_large_table_vs_oracle(oracle):
+    """A count-min storage with more than 2^30 bins (one table of 2^30 + 1
+    bins beside a small one) takes the sort-based insert_and_query: results,
+    counters and tables equal the sequential oracle's."""
+    from goetia_amd import workload
+    genome = workload.genome(61, 30_000)
+    gen = workload.reads(62, genome, 2000, 30, 200, 0.01, 0.0)
+    reads = [gen.read(r) for r in range(gen.n_reads)] + [b'ACGTTGCA' * 30] * 40
+    sizes = [(1 << 30) + 1, 1_000_003]
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), 21)
+    got = g.insert_and_query_sequences(ReadBatch.from_strings(reads))
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp = np.concatenate([o.insert_and_query_sequence(r, 21, True) for r in reads])
+    np.testing.assert_array_equal(got, exp)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(g.get_raw(), o.tables()):
+        assert hashlib.sha256(a.tobytes()).digest() == hashlib.sha256(b.tobytes()).digest()
+
+
+@pytest.mark.parametrize('n_tables', [2, 4, 7])
+def test_byte_insert_and_query_hashes_vs_oracle(oracle, n_tables):
+    """Storage::insert_and_query over an array of hashes in index order
+    (sort-based): repeated hashes (runs past 255), colliding bins of small
+    tables, all-ones and zero values; results, counters and tables equal the
+    oracle's one-hash-at-a-time calls, over two calls."""
+    rng = np.random.default_rng(70 + n_tables)
+    pool = np.concatenate([rng.integers(0, 2**64 - 1, 3000, dtype=np.uint64),
+                           np.array([0, 2**64 - 1], np.uint64)])
+    hs = pool[rng.integers(0, len(pool), 20_000)]
+    hs[5000:5400] = pool[7]
+    sizes = oracle.primes_near_x(n_tables, 5_003)
+    st = lg.ByteStorage(sizes)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    for part in (hs[:12_000], hs[12_000:]):
+        got = st.insert_and_query_hashes(part)
+        np.testing.assert_array_equal(got, [o.insert_and_query(int(h)) for h in part])
+    assert (st.n_unique_kmers(), st.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(st.get_raw_tables(), o.tables()):
         np.testing.assert_array_equal(a, b)
