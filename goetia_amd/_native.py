@@ -118,6 +118,7 @@ SIGNATURES = {
     'gt_shifter_shift_left': [u16, u64p, u64p, C.c_uint8, C.c_uint8],
     'gt_dbg_insert_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
     'gt_dbg_insert': [st_p, u16, cint, cint, vp, vp, u64, vp, vp, u64p],
+    'gt_dbg_insert_noalloc_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
     'gt_dbg_query_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp, vp],
     'gt_dbg_query': [st_p, u16, cint, cint, vp, vp, u64, vp, vp],
     'gt_dbg_insert_and_query_dev': [st_p, u16, cint, cint, vp, vp, u64, u64, vp, vp],

@@ -91,15 +91,28 @@ int ensure_device(int device = -1) {
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;             // a view of memory the buffer does not own
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
-    // Grows (never shrinks); contents are not kept.
+    // Makes the buffer a view of n bytes at ptr; it never allocates or frees.
+    void borrow(void* ptr, size_t n) {
+        release();
+        p = ptr;
+        bytes = n;
+        borrowed = true;
+    }
+    // Grows (never shrinks); contents are not kept.  A borrowed buffer only
+    // checks that it is large enough.
     int ensure(size_t need) {
         if (need <= bytes && p) return GT_OK;
+        if (borrowed)
+            return fail(GT_ERR_INVALID_ARG, "the call needs " + std::to_string(need) + " bytes of a " +
+                                                std::to_string(bytes) + "-byte fixed scratch region");
         release();
         size_t sz = std::max<size_t>(need, 256);
         hipError_t e = hipMalloc(&p, sz);
@@ -2164,6 +2177,103 @@ int gt_workspace_free(gt_workspace* ws) {
     (void)hipDeviceSynchronize();
     delete ws;
     return GT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ allocation-free insert
+//
+// gt_dbg_insert_noalloc_dev runs the atomic first-touch insert with its
+// scratch (claim map, per-start fresh flags, tile -> read table, packed k-mer
+// offsets) in one fixed device array instead of library allocations.  Each GPU
+// holds its own copy of the array for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress gives on that GPU.
+// Calls share it: they must not overlap on one GPU.
+
+constexpr uint64_t kNaClaimSlots = 1ULL << 22;          // 2^19 starts per sub-batch at 4 tables
+constexpr uint64_t kNaFreshBytes = 1ULL << 20;          // >= the largest sub-batch (2^21 starts, 1 table)
+constexpr uint64_t kNaMaxBases = 1ULL << 30;
+constexpr uint64_t kNaMaxReadsFlags = 1ULL << 22;       // reads of a call that asks for new_flags
+constexpr uint64_t kNaKeysOff = 0;
+constexpr uint64_t kNaValsOff = kNaKeysOff + kNaClaimSlots * 8;
+constexpr uint64_t kNaGenOff = kNaValsOff + kNaClaimSlots * 8;
+constexpr uint64_t kNaFreshOff = kNaGenOff + 256;
+constexpr uint64_t kNaTileOff = kNaFreshOff + kNaFreshBytes;
+constexpr uint64_t kNaTileBytes = ((kNaMaxBases + kTileGrain - 1) / kTileGrain + 1) * 4 + 256;
+constexpr uint64_t kNaKoffOff = (kNaTileOff + kNaTileBytes + 255) / 256 * 256;
+constexpr uint64_t kNaKoffBytes = (kNaMaxReadsFlags + 1) * 8;
+constexpr uint64_t kNaScanOff = (kNaKoffOff + kNaKoffBytes + 255) / 256 * 256;
+constexpr uint64_t kNaScanBytes = 1ULL << 20;
+constexpr uint64_t kNaBytes = kNaScanOff + kNaScanBytes;   // about 98 MiB
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+struct NoAllocScratch {
+    std::mutex mu;
+    bool ready = false;
+    uint8_t* base = nullptr;   // this GPU's address of `scratch`
+    Scratch sc;
+};
+constexpr int kNaDevices = 64;
+NoAllocScratch g_noalloc[kNaDevices];
+
+// Once per GPU, with that GPU current: take the array's device address and
+// lay the scratch out over it; the claim map starts cleared.
+int noalloc_setup(NoAllocScratch& na) {
+    if (na.ready) return GT_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(GT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    uint8_t* b = static_cast<uint8_t*>(ptr);
+    Scratch& sc = na.sc;
+    sc.claim_keys.borrow(b + kNaKeysOff, kNaClaimSlots * 8);
+    sc.claim_vals.borrow(b + kNaValsOff, kNaClaimSlots * 8);
+    sc.claim_gen.borrow(b + kNaGenOff, 256);
+    sc.fresh.borrow(b + kNaFreshOff, kNaFreshBytes);
+    sc.tile_read.borrow(b + kNaTileOff, kNaTileBytes);
+    sc.kmer_off.borrow(b + kNaKoffOff, kNaKoffBytes);
+    sc.scan_blocks.borrow(b + kNaScanOff, kNaScanBytes);
+    sc.fixed = true;
+    launch_fill_u64(sc.claim_keys.as<uint64_t>(), kNaClaimSlots, 0, nullptr);
+    launch_fill_u64(sc.claim_vals.as<uint64_t>(), kNaClaimSlots, 0, nullptr);
+    launch_fill_u64(sc.claim_gen.as<uint64_t>(), 1, 0, nullptr);
+    GT_HIP(hipGetLastError());
+    GT_HIP(hipStreamSynchronize(nullptr));
+    sc.claim_cap = kNaClaimSlots;
+    sc.claim_gen_issued = 0;
+    na.base = b;
+    na.ready = true;
+    return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gt_dbg_insert_noalloc_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                              const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint64_t* n_new,
+                              uint8_t* new_flags, void* stream) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
+    if (st->kind == kExact)
+        return fail(GT_ERR_UNSUPPORTED, "gt_dbg_insert_noalloc_dev takes Bloom and count-min storages (the exact set "
+                                        "grows by allocating)");
+    if (n_bases > kNaMaxBases)
+        return fail(GT_ERR_INVALID_ARG, "gt_dbg_insert_noalloc_dev: n_bases " + std::to_string(n_bases) +
+                                            " exceeds its fixed scratch (at most 2^30 bases per call)");
+    if (new_flags && n_reads > kNaMaxReadsFlags)
+        return fail(GT_ERR_INVALID_ARG, "gt_dbg_insert_noalloc_dev: with new_flags at most 2^22 reads per call");
+    GT_TRY(ensure_device());
+    if (g_dev.device < 0 || g_dev.device >= kNaDevices) return fail(GT_ERR_DEVICE, "device index out of range");
+    NoAllocScratch& na = g_noalloc[g_dev.device];
+    std::lock_guard<std::mutex> ln(na.mu);
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(noalloc_setup(na));
+    return dbg_insert_impl(st, na.sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
+                           new_flags, (hipStream_t)stream);
 }
 
 int gt_dbg_insert_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canonical, int mode, const uint8_t* seq,

@@ -201,6 +201,19 @@ int gt_dbg_insert_dev(gt_storage* st, uint16_t K, int canonical, int mode, const
 int gt_dbg_insert(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
                   const int64_t* offsets, uint64_t n_reads, uint64_t* n_new, uint8_t* new_flags,
                   uint64_t* n_kmers);
+/* gt_dbg_insert_dev without any allocation: no counterpart in goetia.  For
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  Same arguments and
+ * results as gt_dbg_insert_dev, for Bloom and count-min storages
+ * (GT_ERR_UNSUPPORTED for the exact set, which grows by allocating); the
+ * insert's scratch is one fixed array of about 98 MiB per GPU, kept for the
+ * life of the process, so a call takes at most 2^30 bases (and at most 2^22
+ * reads when new_flags is asked for), GT_ERR_INVALID_ARG beyond.  All calls
+ * on one GPU share that array: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+int gt_dbg_insert_noalloc_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
+                              const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint64_t* n_new,
+                              uint8_t* new_flags, void* stream);
 
 /* dBG::query_sequence(seq, counts, hashes) (dbg.hh:364-376) over every read:
  * counts (count_t = int16, storage.hh:80) and hashes (optional) packed per
