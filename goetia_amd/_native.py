@@ -22,6 +22,7 @@ GT_ERR_NO_DEVICE = 6
 GT_ERR_IO = 7
 GT_ERR_INTERNAL = 8
 GT_ERR_INVALID_CHARACTER = 9
+GT_ERR_PAIRING = 10
 
 BIT, BYTE, EXACT = 1, 2, 3
 MODE_RAW, MODE_SKIP_N = 0, 1
@@ -69,6 +70,7 @@ _ERRORS = {
     GT_ERR_IO: GoetiaFileException,
     GT_ERR_INTERNAL: GoetiaException,
     GT_ERR_INVALID_CHARACTER: InvalidCharacterException,
+    GT_ERR_PAIRING: GoetiaException,
 }
 
 u8p = C.POINTER(C.c_uint8)
@@ -172,6 +174,11 @@ SIGNATURES = {
     'gt_inserter_advance': [vp, vp, u64p, u64p, C.POINTER(cint)],
     'gt_inserter_stats': [vp, u64p, u64p, u64p],
     'gt_inserter_free': [vp],
+    'gt_pairs_open': [C.c_char_p, C.c_char_p, cint, u32, cint, C.POINTER(vp)],
+    'gt_pairs_next': [vp, vp, u64, vp, vp, u64, vp, C.POINTER(cint)],
+    'gt_pairs_stats': [vp, u64p, u64p, C.POINTER(cint)],
+    'gt_pairs_close': [vp],
+    'gt_inserter_advance_pairs': [vp, vp, u64p, u64p, C.POINTER(cint)],
 }
 
 _lib = None

@@ -883,3 +883,285 @@ int gt_inserter_free(gt_inserter* ins) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ paired files
+//
+// SplitPairedReader<FastxParser<DNA_SIMPLE>> (readers.hh:235-334) and
+// FileProcessor::advance over it (processors.hh:180-204).  Every step reads
+// one record attempt from each file.  A side whose attempt is rejected
+// (invalid characters, shorter than min_length) is empty; an attempt that
+// raises (an invalid character from a strict parser, a quality of another
+// length) makes that side empty too, unless the reader is strict, in which
+// case the whole pair is dropped (handle_next, processors.hh:149-171).  With
+// force_name_match, two present records whose names are not a pair
+// (check_is_pair, parsing.cc:56-87) drop the pair and count two skipped, or,
+// strict, stop with "Unpaired reads".  Files that end at different steps
+// stop with "Mismatched split paired files." at the step after the shorter
+// one ended.  A pair's reads are inserted left then right, and the interval
+// is polled once per pair with both sides' k-mers.
+
+struct gt_pairs {
+    gt_fastx* side[2] = {nullptr, nullptr};
+    bool strict = false;
+    bool force_name_match = false;
+    uint64_t n_unpaired = 0;               // SplitPairedReader::_n_skipped
+    // the record of each side of the current pair (present = false when empty)
+    std::string seq[2], name[2], qual[2];
+    bool present[2] = {false, false};
+    bool held = false;                     // the current pair was read but not yet taken
+};
+
+namespace {
+
+// split_on_first (parsing.cc:31-52): the text before the first delimiter
+// run, and the text after it.
+std::pair<std::string, std::string> split_first(const std::string& s, const char* delims) {
+    auto is_delim = [&](char c) { return std::strchr(delims, c) != nullptr && c != '\0'; };
+    std::string left, right;
+    bool have_left = false;
+    for (size_t i = 0; i < s.size(); ++i) {
+        if (!have_left || left.empty()) {
+            if (is_delim(s[i])) {
+                left = s.substr(0, i);
+                have_left = !left.empty();
+            }
+        } else if (!is_delim(s[i])) {
+            right = s.substr(i);
+            break;
+        }
+    }
+    if (left.empty()) left = s;
+    return {left, right};
+}
+
+bool ends_with(const std::string& s, const char* suffix) {
+    const size_t n = std::strlen(suffix);
+    return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
+}
+
+// check_is_pair (parsing.cc:56-87): "<id>/1" and "<id>/2"; or equal first
+// words whose rest ends in "1:" / "2:"; or equal first words whose rest is
+// "<x>/1" / "<x>/2" with the same non-empty <x>.
+bool names_pair(const std::string& l, const std::string& r) {
+    const auto ls = split_first(l, " \t"), rs = split_first(r, " \t");
+    if (ends_with(ls.first, "/1") && ends_with(rs.first, "/2")) {
+        const auto a = split_first(ls.first, "/"), b = split_first(rs.first, "/");
+        return !a.first.empty() && a.first == b.first;
+    }
+    if (ls.first == rs.first && ends_with(ls.second, "1:") && ends_with(rs.second, "2:")) return true;
+    if (ls.first == rs.first && ends_with(ls.second, "/1") && ends_with(rs.second, "/2")) {
+        const auto a = split_first(ls.second, "/"), b = split_first(rs.second, "/");
+        return !a.first.empty() && a.first == b.first;
+    }
+    return false;
+}
+
+// The record-at-a-time reader for a parser opened by gt_pairs_open.
+void drop_blocks(gt_fastx* p) {
+    if (!p->par) return;
+    if (p->map) ::munmap((void*)p->map, p->map_len);
+    p->map = nullptr;
+    p->map_len = 0;
+    p->par = false;
+}
+
+enum PairStep { kPairReady, kPairDropped, kPairsDone };
+
+// One SplitPairedReader::next (with the advance loop's is_complete check and
+// handle_next) into pr->seq / name / qual / present.
+int next_pair(gt_pairs* pr, PairStep* step) {
+    if (pr->held) {
+        pr->held = false;
+        *step = kPairReady;
+        return GT_OK;
+    }
+    gt_fastx* L = pr->side[0];
+    gt_fastx* R = pr->side[1];
+    if (L->complete != R->complete) return set_error(GT_ERR_PAIRING, "Mismatched split paired files.");
+    if (L->complete) {
+        *step = kPairsDone;
+        return GT_OK;
+    }
+    bool threw = false;
+    for (int k = 0; k < 2; ++k) {
+        gt_fastx* p = pr->side[k];
+        pr->present[k] = false;
+        const Next r = next_record(*p);
+        if (r == kIoError) return set_error(GT_ERR_IO, "error reading " + p->path);
+        if (r == kInvalidChar || r == kInvalidRead) threw = true;
+        if (r == kRecord) {
+            pr->present[k] = true;
+            pr->seq[k].swap(p->seq);
+            pr->name[k].swap(p->name);
+            pr->qual[k].swap(p->qual);
+        }
+    }
+    if (threw && pr->strict) {
+        *step = kPairDropped;
+        return GT_OK;
+    }
+    if (pr->force_name_match && pr->present[0] && pr->present[1] && !names_pair(pr->name[0], pr->name[1])) {
+        if (pr->strict) return set_error(GT_ERR_PAIRING, "Unpaired reads");
+        pr->n_unpaired += 2;
+        pr->present[0] = pr->present[1] = false;
+    }
+    *step = kPairReady;
+    return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gt_pairs_open(const char* left, const char* right, int strict, uint32_t min_length, int force_name_match,
+                  gt_pairs** out) {
+    if (!left || !right || !out) return set_error(GT_ERR_INVALID_ARG, "null path or output");
+    *out = nullptr;
+    gt_pairs* pr = new (std::nothrow) gt_pairs();
+    if (!pr) return set_error(GT_ERR_OUT_OF_MEMORY, "gt_pairs_open");
+    int rc = gt_fastx_open(left, strict, min_length, &pr->side[0]);
+    if (rc == GT_OK) rc = gt_fastx_open(right, strict, min_length, &pr->side[1]);
+    if (rc != GT_OK) {
+        gt_pairs_close(pr);
+        return rc;
+    }
+    drop_blocks(pr->side[0]);
+    drop_blocks(pr->side[1]);
+    pr->strict = strict != 0;
+    pr->force_name_match = force_name_match != 0;
+    *out = pr;
+    return GT_OK;
+}
+
+int gt_pairs_next(gt_pairs* pr, uint8_t* seq, uint64_t seq_cap, int64_t* lens, char* names, uint64_t names_cap,
+                  int64_t* name_lens, int* done) {
+    if (!pr || !lens || !done || (!seq && seq_cap)) return set_error(GT_ERR_INVALID_ARG, "null argument");
+    for (;;) {
+        PairStep step = kPairReady;
+        const int rc = next_pair(pr, &step);
+        if (rc != GT_OK) return rc;
+        *done = step == kPairsDone;
+        if (step == kPairsDone) return GT_OK;
+        if (step == kPairDropped) return set_error(GT_ERR_INVALID_CHARACTER, "a record of the pair was rejected "
+                                                                             "by a strict reader");
+        uint64_t at = 0, nat = 0;
+        for (int k = 0; k < 2; ++k) {
+            lens[k] = -1;
+            if (name_lens) name_lens[k] = -1;
+            if (!pr->present[k]) continue;
+            const uint64_t n = pr->seq[k].size();
+            if (at + n > seq_cap) return set_error(GT_ERR_INVALID_ARG, "pair does not fit the buffer");
+            std::memcpy(seq + at, pr->seq[k].data(), n);
+            at += n;
+            lens[k] = (int64_t)n;
+            if (names && name_lens) {
+                const uint64_t m = pr->name[k].size();
+                if (nat + m > names_cap) return set_error(GT_ERR_INVALID_ARG, "names do not fit the buffer");
+                std::memcpy(names + nat, pr->name[k].data(), m);
+                nat += m;
+                name_lens[k] = (int64_t)m;
+            }
+        }
+        return GT_OK;
+    }
+}
+
+int gt_pairs_stats(const gt_pairs* pr, uint64_t* n_parsed, uint64_t* n_skipped, int* complete) {
+    if (!pr) return set_error(GT_ERR_INVALID_ARG, "null reader");
+    if (n_parsed) *n_parsed = pr->side[0]->n_parsed + pr->side[1]->n_parsed;
+    if (n_skipped) *n_skipped = pr->n_unpaired + pr->side[0]->n_skipped + pr->side[1]->n_skipped;
+    if (complete) *complete = pr->side[0]->complete && pr->side[1]->complete;
+    return GT_OK;
+}
+
+int gt_pairs_close(gt_pairs* pr) {
+    if (!pr) return GT_OK;
+    for (gt_fastx* p : pr->side)
+        if (p) gt_fastx_close(p);
+    delete pr;
+    return GT_OK;
+}
+
+// FileProcessor::advance over a SplitPairedReader for an InserterProcessor
+// (processors.hh:180-204, 304-306): pairs are appended to the active host
+// slot, left then right; the slot is inserted when it is full, when a pair
+// brings the interval counter to `interval` (the call then returns, every
+// insert finished) or at the end of the input.
+int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequences, uint64_t* time_total,
+                              int* remaining) {
+    if (!ins || !pr) return set_error(GT_ERR_INVALID_ARG, "null inserter or reader");
+    for (const Slot& s : ins->slot)
+        if (s.cur < s.n) return set_error(GT_ERR_INVALID_ARG, "the inserter holds reads of a single file");
+    for (;;) {
+        Slot& sl = ins->slot[ins->active];
+        GT_FX_HIP(hipEventSynchronize(sl.copied));   // the slot's previous copy is done
+        uint64_t nr = 0, nb = 0;
+        sl.h_sub[0] = 0;
+        bool crossed = false, done = false, full = false;
+        int err = GT_OK;
+        while (nr + 2 <= ins->max_reads) {           // a pair adds at most two reads
+            PairStep step = kPairReady;
+            err = next_pair(pr, &step);
+            if (err != GT_OK) break;
+            if (step == kPairsDone) {
+                done = true;
+                break;
+            }
+            if (step == kPairDropped) continue;
+            uint64_t need = 0;
+            for (int k = 0; k < 2; ++k)
+                if (pr->present[k]) need += pr->seq[k].size();
+            if (nb + need > ins->max_bases) {
+                if (nr == 0) {
+                    err = set_error(GT_ERR_INVALID_ARG, "a read pair does not fit the batch buffers");
+                    break;
+                }
+                pr->held = true;                     // first pair of the next slot
+                full = true;
+                break;
+            }
+            uint64_t t = 0;
+            for (int k = 0; k < 2; ++k) {
+                if (!pr->present[k]) continue;
+                const uint64_t len = pr->seq[k].size();
+                std::memcpy(sl.h_seq + nb, pr->seq[k].data(), len);
+                nb += len;
+                sl.h_sub[++nr] = (int64_t)nb;
+                t += len >= ins->K ? len - ins->K + 1 : 0;
+                ++ins->n_sequences;
+            }
+            ins->total += t;
+            ins->counter += t;
+            if (ins->counter >= ins->interval) {
+                ins->counter = 0;
+                crossed = true;
+                break;
+            }
+        }
+        if (nr) {
+            GT_FX_HIP(hipMemcpyAsync(sl.d_seq, sl.h_seq, nb, hipMemcpyHostToDevice, ins->stream));
+            GT_FX_HIP(hipMemcpyAsync(sl.d_off, sl.h_sub, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                                     ins->stream));
+            GT_FX_HIP(hipEventRecord(sl.copied, ins->stream));
+            const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
+                                             nullptr, nullptr, ins->stream);
+            if (rc != GT_OK) return rc;
+            ins->pending_work = true;
+            ins->active ^= 1;
+        }
+        if (err != GT_OK) {
+            GT_TRY_FX(sync_inserts(ins));
+            return err;
+        }
+        if (crossed || done) {
+            GT_TRY_FX(sync_inserts(ins));
+            if (n_sequences) *n_sequences = ins->n_sequences;
+            if (time_total) *time_total = ins->total;
+            if (remaining) *remaining = crossed ? 1 : 0;
+            return GT_OK;
+        }
+        (void)full;   // a full slot (by reads or bases) was inserted; fill the other one
+    }
+}
+
+}  // extern "C"

@@ -1182,22 +1182,10 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
     return off + x - v;
 }
 
-// The first pass keeps a tile's per-window values in LDS with one padding
-// word after every 64: thread t writes windows 8t..8t+7, so the 64 lanes of a
-// store touch 64 different banks (unpadded, stride 8 puts them on 8), and the
-// ranking pass, which reads consecutive windows, stays conflict-free too.
-#ifndef GT_BIN_L1_PAD
-#define GT_BIN_L1_PAD 1
-#endif
-constexpr int kBinTileP = kBinTile + (GT_BIN_L1_PAD ? kBinTile / 64 : 0);
-__device__ __forceinline__ int bin_pw(int w) { return GT_BIN_L1_PAD ? w + (w >> 6) : w; }
-// LDS index of event e = table * kBinTile + window
-__device__ __forceinline__ int bin_pe(int e) { return (e / kBinTile) * kBinTileP + bin_pw(e % kBinTile); }
-
 struct BinL1Shared {
-    uint32_t g[kBinMaxTables * kBinTileP];  // table-major: [table][window]; kNoBin = no event
-    uint32_t ord[kBinTileP];
-    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index (table * kBinTile + window)
+    uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
+    uint32_t ord[kBinTile];
+    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
     uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
     uint32_t part[kBlock];
 };
@@ -1225,7 +1213,7 @@ struct BinL1Op {
         t0 = tile_base;
         const int w0 = threadIdx.x * kBinPer;
         for (uint32_t t = 0; t < P.n_tables; ++t)
-            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTileP + bin_pw(w0 + j)] = kNoBin;
+            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
@@ -1237,12 +1225,12 @@ struct BinL1Op {
         const uint32_t ord = (uint32_t)(s + kd);
         if (P.hasev) P.hasev[ord] = valid ? 1 : 0;
         if (!valid) return;
-        const int w = bin_pw((int)(s - t0));
+        const int w = (int)(s - t0);
         const uint64_t h = kmer_value(fw, rc, canonical);
         sh->ord[w] = ord;
 #pragma unroll
         for (uint32_t i = 0; i < kBinMaxTables; ++i)
-            if (i < P.n_tables) sh->g[i * kBinTileP + w] = P.gbase[i] + gt_fastmod30(h, (uint32_t)T.p[i], T.m[i]);
+            if (i < P.n_tables) sh->g[i * kBinTile + w] = P.gbase[i] + gt_fastmod30(h, (uint32_t)T.p[i], T.m[i]);
     }
 
     __device__ void flush() {}
@@ -1263,7 +1251,7 @@ struct BinL1Op {
         for (int c = 0; c < per_wave; c += 4 * 64) {        // per_wave is a multiple of 512
             uint32_t d[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[bin_pe(e0 + c + q * 64 + lane)], P.shift2);
+            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[e0 + c + q * 64 + lane], P.shift2);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (d[q] != (uint32_t)kDigits1) atomicAdd(&wc[d[q]], 1u);
@@ -1285,7 +1273,7 @@ struct BinL1Op {
         __syncthreads();
         for (int c = 0; c < per_wave; c += 64) {
             const int e = e0 + c + lane;
-            const uint32_t dd = bin_d1(sh->g[bin_pe(e)], P.shift2);
+            const uint32_t dd = bin_d1(sh->g[e], P.shift2);
             const bool act = dd != (uint32_t)kDigits1;
             const uint64_t m = match_lds(sl, dd, act);
             if (act) sh->perm[wc[dd] + (uint32_t)__popcll(m & lt)] = (uint16_t)e;
@@ -1305,7 +1293,7 @@ struct BinL1Op {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t i = i0 + q * kBlock + threadIdx.x;
-                if (i < n_ev) dst[i] = ((uint64_t)sh->g[bin_pe(e[q])] << 32) | sh->ord[bin_pw(e[q] & (kBinTile - 1))];
+                if (i < n_ev) dst[i] = ((uint64_t)sh->g[e[q]] << 32) | sh->ord[e[q] & (kBinTile - 1)];
             }
         }
     }

@@ -10,7 +10,7 @@ from .storage import (BitStorage, BTreeStorage, ByteStorage, PHMapStorage, Spars
                       StorageTraits, count_t, get_n_primes_near_x)
 from ._native import (GoetiaException, GoetiaFileException, InvalidCharacterException, InvalidSequenceException,
                       SequenceLengthException, UninitializedShifterException)
-from .parsing import FastxParser, NoMoreReadsAvailable, Record
+from .parsing import FastxParser, NoMoreReadsAvailable, Record, SplitPairedReader
 from .processors import FilterProcessor, InserterProcessor, IntervalCounter
 from .filters import DiginormFilter, StreamingSolidFilter
 from .cdbg import StreamingCompactor
@@ -20,6 +20,7 @@ __all__ = ['dBG', 'Kmer', 'Shift', 'CanLemireShifter', 'FwdLemireShifter', 'Cano
            'PHMapStorage', 'BTreeStorage', 'StorageTraits', 'count_t', 'get_n_primes_near_x',
            'GoetiaException', 'InvalidSequenceException', 'SequenceLengthException',
            'UninitializedShifterException', 'GoetiaFileException', 'InvalidCharacterException',
-           'FastxParser', 'NoMoreReadsAvailable', 'Record', 'InserterProcessor', 'IntervalCounter',
+           'FastxParser', 'NoMoreReadsAvailable', 'Record', 'SplitPairedReader', 'InserterProcessor',
+           'IntervalCounter',
            'FilterProcessor',
            'StreamingSolidFilter', 'DiginormFilter', 'StreamingCompactor']

@@ -165,3 +165,72 @@ class FastxParser:
 
     def is_complete(self):
         return self._stats()[3] and self._pos >= len(self._records)
+
+
+class SplitPairedReader:
+    """goetia::SplitPairedReader<FastxParser<DNA_SIMPLE>> (readers.hh:235-334)
+    over the native paired reader (gt_pairs_*): next() gives (left, right)
+    Records, None for an empty side; a pair that a strict reader rejects
+    raises InvalidCharacterException; unpaired names (force_name_match) give
+    (None, None) and count two skipped, or raise GoetiaException when strict;
+    files of different lengths raise GoetiaException."""
+
+    def __init__(self, left, right, strict=False, min_length=0, force_name_match=False):
+        self.left_filename, self.right_filename = str(left), str(right)
+        self._h = N.vp()
+        N.call('gt_pairs_open', self.left_filename.encode(), self.right_filename.encode(), int(bool(strict)),
+               int(min_length), int(bool(force_name_match)), C.byref(self._h))
+        self._buf = np.empty(1 << 20, np.uint8)
+        self._names = C.create_string_buffer(1 << 16)
+
+    @classmethod
+    def build(cls, left, right, strict=False, min_length=0, force_name_match=False):
+        return cls(left, right, strict, min_length, force_name_match)
+
+    def close(self):
+        if getattr(self, '_h', None) and self._h.value:
+            N.load().gt_pairs_close(self._h)
+            self._h = N.vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def next(self):
+        if self.is_complete():
+            raise NoMoreReadsAvailable('No more reads available')
+        lens = np.zeros(2, np.int64)
+        nlens = np.zeros(2, np.int64)
+        done = C.c_int()
+        N.call('gt_pairs_next', self._h, N.ptr(self._buf), len(self._buf), N.ptr(lens), self._names,
+               len(self._names), N.ptr(nlens), C.byref(done))
+        if done.value:
+            return None, None
+        out, at, nat = [], 0, 0
+        raw = self._names.raw
+        for k in range(2):
+            if lens[k] < 0:
+                out.append(None)
+                continue
+            seq = self._buf[at:at + lens[k]].tobytes().decode()
+            name = raw[nat:nat + nlens[k]].decode()
+            at += int(lens[k])
+            nat += int(nlens[k])
+            out.append(Record(name, seq))
+        return tuple(out)
+
+    def _stats(self):
+        parsed, skipped, complete = C.c_uint64(), C.c_uint64(), C.c_int()
+        N.call('gt_pairs_stats', self._h, C.byref(parsed), C.byref(skipped), C.byref(complete))
+        return parsed.value, skipped.value, bool(complete.value)
+
+    def n_parsed(self):
+        return self._stats()[0]
+
+    def n_skipped(self):
+        return self._stats()[1]
+
+    def is_complete(self):
+        return self._stats()[2]

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .parsing import FastxParser, fastx_text
+from .parsing import FastxParser, SplitPairedReader, fastx_text
 from .reads import ReadBatch
 
 
@@ -67,15 +67,22 @@ class InserterProcessor:
             pass
 
     def advance(self, parser):
-        """FileProcessor::advance: (n_sequences, time, remaining)."""
+        """FileProcessor::advance over a FastxParser or a SplitPairedReader
+        (processors.hh:180-231): (n_sequences, time, remaining)."""
         n, t, rem = C.c_uint64(), C.c_uint64(), C.c_int()
-        N.call('gt_inserter_advance', self._h, parser._h, C.byref(n), C.byref(t), C.byref(rem))
+        entry = 'gt_inserter_advance_pairs' if isinstance(parser, SplitPairedReader) else 'gt_inserter_advance'
+        N.call(entry, self._h, parser._h, C.byref(n), C.byref(t), C.byref(rem))
         return n.value, t.value, bool(rem.value)
 
-    def process(self, filename, strict=False, min_length=0):
-        """FileProcessor::process (processors.hh:100-126): the whole file;
-        (n_sequences, time)."""
-        parser = filename if isinstance(filename, FastxParser) else FastxParser(filename, strict, min_length)
+    def process(self, filename, strict=False, min_length=0, right_filename=None, force_name_match=False):
+        """FileProcessor::process (processors.hh:69-126): a whole file, or a
+        split pair of files; (n_sequences, time)."""
+        if isinstance(filename, (FastxParser, SplitPairedReader)):
+            parser = filename
+        elif right_filename is not None:
+            parser = SplitPairedReader(filename, right_filename, strict, min_length, force_name_match)
+        else:
+            parser = FastxParser(filename, strict, min_length)
         remaining = True
         n = t = 0
         while remaining:
@@ -86,9 +93,10 @@ class InserterProcessor:
         """pythonize_processors.py:12-33: advance through a file (or a
         parser), yielding (n_sequences, time, parser.n_skipped()) after every
         advance that consumed reads."""
-        if right_file is not None:
-            raise N.UnsupportedError('paired-end input (SplitPairedReader) is not supported')
-        parser = FastxParser(file) if isinstance(file, (str, bytes)) else file
+        if isinstance(file, (str, bytes)):
+            parser = FastxParser(file) if right_file is None else SplitPairedReader(file, right_file)
+        else:
+            parser = file
         advancing = True
         prev = 0
         while advancing:
@@ -197,8 +205,11 @@ class FilterProcessor:
         return n, t
 
     def chunked_process(self, file, right_file=None):
-        """pythonize_processors.py:12-33."""
-        return InserterProcessor.chunked_process(self, file, right_file)
+        """pythonize_processors.py:12-33 over one file (pairs of files go
+        through InserterProcessor; the filters here read single files)."""
+        if right_file is not None or isinstance(file, SplitPairedReader):
+            raise N.UnsupportedError('FilterProcessor reads single files; paired files go through InserterProcessor')
+        return InserterProcessor.chunked_process(self, file)
 
     def n_passed(self):
         return self._n_passed

@@ -15,6 +15,7 @@
 #include <memory>
 #include <optional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "goetia/dbg.hh"
@@ -42,6 +43,9 @@ struct Record {
             output << ">" << name << '\n' << sequence << '\n';
     }
 };
+
+// parsing.hh:60: one record from each file of a pair, either may be absent.
+typedef std::pair<std::optional<Record>, std::optional<Record>> RecordPair;
 
 template <class Alphabet = DNA_SIMPLE>
 class FastxParser {
@@ -136,6 +140,71 @@ class FastxParser {
         int complete = 0;
         detail::check(gt_fastx_stats(_h, &v[0], &v[1], &v[2], &complete), "FastxParser");
         return which == 3 ? (uint64_t)complete : v[which];
+    }
+};
+
+// SplitPairedReader<FastxParser<DNA_SIMPLE>> (readers.hh:235-334) over the
+// library's paired reader: next() gives (left, right), an empty side where
+// that file's record was rejected; a pair a strict reader rejects throws
+// InvalidCharacterException, files of different lengths and (strict,
+// force_name_match) unpaired names throw GoetiaException.
+template <class ParserType = FastxParser<>>
+class SplitPairedReader {
+    gt_pairs* _h = nullptr;
+
+  public:
+    typedef ParserType parser_type;
+    typedef RecordPair value_type;
+
+    SplitPairedReader(const std::string& left, const std::string& right, bool strict = false,
+                      uint32_t min_length = 0, bool force_name_match = false) {
+        detail::check(gt_pairs_open(left.c_str(), right.c_str(), strict ? 1 : 0, min_length, force_name_match ? 1 : 0,
+                                    &_h),
+                      "SplitPairedReader");
+    }
+    SplitPairedReader(const SplitPairedReader&) = delete;
+    SplitPairedReader& operator=(const SplitPairedReader&) = delete;
+    ~SplitPairedReader() { gt_pairs_close(_h); }
+
+    static std::shared_ptr<SplitPairedReader> build(const std::string& left, const std::string& right,
+                                                    bool strict = false, uint32_t min_length = 0,
+                                                    bool force_name_match = false) {
+        return std::make_shared<SplitPairedReader>(left, right, strict, min_length, force_name_match);
+    }
+
+    gt_pairs* handle() const { return _h; }
+
+    RecordPair next() {
+        if (is_complete()) throw NoMoreReadsAvailable();
+        std::string seq(1 << 20, '\0'), names(1 << 16, '\0');
+        int64_t lens[2], nlens[2];
+        int done = 0;
+        detail::check(gt_pairs_next(_h, reinterpret_cast<uint8_t*>(&seq[0]), seq.size(), lens, &names[0],
+                                    names.size(), nlens, &done),
+                      "SplitPairedReader::next");
+        RecordPair out;
+        if (done) return out;
+        size_t at = 0, nat = 0;
+        std::optional<Record>* side[2] = {&out.first, &out.second};
+        for (int k = 0; k < 2; ++k) {
+            if (lens[k] < 0) continue;
+            *side[k] = Record{names.substr(nat, (size_t)nlens[k]), seq.substr(at, (size_t)lens[k]), std::string()};
+            at += (size_t)lens[k];
+            nat += (size_t)nlens[k];
+        }
+        return out;
+    }
+
+    bool is_complete() const { return stat(2) != 0; }
+    uint64_t n_parsed() const { return stat(0); }
+    uint64_t n_skipped() const { return stat(1); }
+
+  private:
+    uint64_t stat(int which) const {
+        uint64_t v[2] = {0, 0};
+        int complete = 0;
+        detail::check(gt_pairs_stats(_h, &v[0], &v[1], &complete), "SplitPairedReader");
+        return which == 2 ? (uint64_t)complete : v[which];
     }
 };
 

@@ -56,6 +56,15 @@ class InserterProcessor {
         return {n, t, remaining != 0};
     }
 
+    // FileProcessor::advance over a split pair of files (processors.hh:180-204).
+    std::tuple<uint64_t, uint64_t, bool> advance(std::shared_ptr<SplitPairedReader<ParserType>>& reader) {
+        uint64_t n = 0, t = 0;
+        int remaining = 0;
+        detail::check(gt_inserter_advance_pairs(_h, reader->handle(), &n, &t, &remaining),
+                      "InserterProcessor::advance");
+        return {n, t, remaining != 0};
+    }
+
     std::tuple<uint64_t, uint64_t> process(std::shared_ptr<ParserType>& parser) {
         uint64_t n = 0, t = 0;
         bool remaining = true;
@@ -63,10 +72,26 @@ class InserterProcessor {
         return {n, t};
     }
 
+    std::tuple<uint64_t, uint64_t> process(std::shared_ptr<SplitPairedReader<ParserType>>& reader) {
+        uint64_t n = 0, t = 0;
+        bool remaining = true;
+        while (remaining) std::tie(n, t, remaining) = advance(reader);
+        return {n, t};
+    }
+
     std::tuple<uint64_t, uint64_t> process(const std::string& filename, bool strict = false,
                                            uint32_t min_length = 0) {
         auto parser = ParserType::build(filename, strict, min_length);
         return process(parser);
+    }
+
+    // FileProcessor::process(left, right, ...) (processors.hh:69-94).
+    std::tuple<uint64_t, uint64_t> process(const std::string& left_filename, const std::string& right_filename,
+                                           bool strict = false, uint32_t min_length = 0,
+                                           bool force_name_match = false) {
+        auto reader =
+            SplitPairedReader<ParserType>::build(left_filename, right_filename, strict, min_length, force_name_match);
+        return process(reader);
     }
 
     uint64_t n_sequences() const { return stat(0); }

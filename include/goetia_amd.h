@@ -49,6 +49,8 @@ extern "C" {
 #define GT_ERR_IO              7   /* GoetiaFileException, bitstorage.cc:186 */
 #define GT_ERR_INTERNAL        8
 #define GT_ERR_INVALID_CHARACTER 9 /* InvalidCharacterException, alphabets.hh:41-55 */
+#define GT_ERR_PAIRING         10  /* GoetiaException of SplitPairedReader: files of different
+                                      lengths, unpaired names (readers.hh:265-269, 314-317) */
 
 #define GT_BIT_STORAGE   1         /* BitStorage, bitstorage.hh:93 */
 #define GT_BYTE_STORAGE  2         /* ByteStorage, bytestorage.hh:96 */
@@ -520,6 +522,41 @@ int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequence
                         int* remaining);
 /* Running n_sequences, time (k-mers) and the interval counter. */
 int gt_inserter_stats(const gt_inserter* ins, uint64_t* n_sequences, uint64_t* time_total, uint64_t* counter);
+
+/* SplitPairedReader<FastxParser<DNA_SIMPLE>> (readers.hh:235-334) over two
+ * files read record by record: each step takes one record attempt from each
+ * file.  A side whose record is rejected (invalid characters from a
+ * non-strict parser, shorter than min_length, a quality of another length) is
+ * empty; with strict set, a raising side (an invalid character, a quality of
+ * another length) drops the whole pair (FileProcessor::handle_next,
+ * processors.hh:149-171).  With force_name_match, two present records whose
+ * names are not a pair (check_is_pair, parsing.cc:56-87) drop the pair and
+ * count two skipped, or, strict, fail with GT_ERR_PAIRING ("Unpaired reads").
+ * Files ending at different steps fail with GT_ERR_PAIRING ("Mismatched
+ * split paired files.") at the next step.  Host only. */
+typedef struct gt_pairs gt_pairs;
+int gt_pairs_open(const char* left, const char* right, int strict, uint32_t min_length, int force_name_match,
+                  gt_pairs** out);
+/* SplitPairedReader::next: the pair's sequences, left then right, into seq
+ * (seq_cap bytes), lens[2] their lengths (-1 for an empty side) and, when
+ * names is given, their names likewise (name_lens[2]); *done = 1 once both
+ * files are exhausted.  A pair dropped by a strict reader fails with
+ * GT_ERR_INVALID_CHARACTER. */
+int gt_pairs_next(gt_pairs* pr, uint8_t* seq, uint64_t seq_cap, int64_t* lens, char* names, uint64_t names_cap,
+                  int64_t* name_lens, int* done);
+/* n_parsed (both files), n_skipped (both files and unpaired reads),
+ * is_complete (readers.hh:262-270, 327-333). */
+int gt_pairs_stats(const gt_pairs* pr, uint64_t* n_parsed, uint64_t* n_skipped, int* complete);
+int gt_pairs_close(gt_pairs* pr);
+/* FileProcessor::advance over a SplitPairedReader for an InserterProcessor
+ * (processors.hh:180-204, 304-306): each pair's present reads are inserted,
+ * left then right; n_sequences counts present reads and the interval counter
+ * is polled once per pair with both reads' k-mers, so the call returns right
+ * after the pair that reaches `interval` (every insert finished), or at the
+ * end of both files with *remaining = 0.  An inserter drives either single
+ * files (gt_inserter_advance) or pairs. */
+int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequences, uint64_t* time_total,
+                              int* remaining);
 int gt_inserter_free(gt_inserter* ins);
 
 /* ------------------------------------------------------------ workload */

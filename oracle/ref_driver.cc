@@ -20,6 +20,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -126,6 +127,39 @@ uint64_t solid_run(void* graph, float min_prop, uint32_t threshold, const char* 
         } catch (SequenceLengthException&) {
         }
     }
+    return n;
+}
+
+// InserterProcessor<G>::advance over a SplitPairedReader (processors.hh:180-204)
+// until no pairs remain: every advance's (n_sequences, time, remaining) into
+// steps (3 words each, up to cap steps); *err = 1 when the reader raised a
+// GoetiaException (the advance then stops), 0 otherwise.
+template <class G>
+uint64_t paired_run(void* graph, const char* left, const char* right, int strict, uint32_t min_length, int force,
+                    uint64_t interval, uint64_t* steps, uint64_t cap, int* err, uint64_t* n_skipped,
+                    uint64_t* n_parsed) {
+    auto g = static_cast<Holder<G>*>(graph)->g;
+    auto proc = G::Processor::build(g, interval);
+    auto reader = SplitPairedReader<FastxParser<DNA_SIMPLE>>::build(left, right, strict, min_length, force);
+    uint64_t n = 0;
+    *err = 0;
+    try {
+        bool remaining = true;
+        while (remaining) {
+            uint64_t ns, tt;
+            std::tie(ns, tt, remaining) = proc->advance(reader);
+            if (n < cap) {
+                steps[3 * n] = ns;
+                steps[3 * n + 1] = tt;
+                steps[3 * n + 2] = remaining ? 1 : 0;
+            }
+            ++n;
+        }
+    } catch (GoetiaException&) {
+        *err = 1;
+    }
+    *n_skipped = reader->n_skipped();
+    *n_parsed = reader->n_parsed();
     return n;
 }
 
@@ -453,6 +487,57 @@ uint64_t ref_solid_filter_reads(void* hv, float min_prop, uint32_t threshold, co
         return solid_run<Graph<ByteStorage, FwdLemireShifter>>(h->graph, min_prop, threshold, seq, off, n_reads, passed);
     }
     return ~0ULL;
+}
+
+// Paired-file insertion through goetia's own processor and reader (Bloom
+// and count-min graphs); returns the number of advances.
+uint64_t ref_paired_insert(void* hv, const char* left, const char* right, int strict, uint32_t min_length,
+                           int force, uint64_t interval, uint64_t* steps, uint64_t cap, int* err,
+                           uint64_t* n_skipped, uint64_t* n_parsed) {
+    Handle* h = (Handle*)hv;
+    auto run = [&](auto tag) {
+        using G = typename decltype(tag)::type;
+        return paired_run<G>(h->graph, left, right, strict, min_length, force, interval, steps, cap, err, n_skipped,
+                             n_parsed);
+    };
+    if (h->kind == 1)
+        return h->canonical ? run(std::common_type<Graph<BitStorage, CanLemireShifter>>{})
+                            : run(std::common_type<Graph<BitStorage, FwdLemireShifter>>{});
+    return h->canonical ? run(std::common_type<Graph<ByteStorage, CanLemireShifter>>{})
+                        : run(std::common_type<Graph<ByteStorage, FwdLemireShifter>>{});
+}
+
+// SplitPairedReader::next over two files until both end: per step, the
+// lengths of the left and right sequences (-1 for an empty side) or -2 for
+// both when next() raised (strict rejection), and -3 when the reader raised a
+// GoetiaException (the walk stops there).  Returns the steps.
+uint64_t ref_paired_walk(const char* left, const char* right, int strict, uint32_t min_length, int force,
+                         int64_t* lens, uint64_t cap) {
+    auto reader = SplitPairedReader<FastxParser<DNA_SIMPLE>>::build(left, right, strict, min_length, force);
+    uint64_t n = 0;
+    auto put = [&](int64_t a, int64_t b) {
+        if (n < cap) {
+            lens[2 * n] = a;
+            lens[2 * n + 1] = b;
+        }
+        ++n;
+    };
+    try {
+        while (!reader->is_complete()) {
+            try {
+                auto pr = reader->next();
+                put(pr.first ? (int64_t)pr.first->sequence.size() : -1,
+                    pr.second ? (int64_t)pr.second->sequence.size() : -1);
+            } catch (InvalidCharacterException&) {
+                put(-2, -2);
+            } catch (InvalidRead&) {
+                put(-2, -2);
+            }
+        }
+    } catch (GoetiaException&) {
+        put(-3, -3);
+    }
+    return n;
 }
 
 // dBG::save(filename) (dbg.hh:396-399): the graph's storage writes its OXLI
