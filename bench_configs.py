@@ -290,9 +290,10 @@ def fastx(args):
         del gr, proc
     out['value'] = out['large_interval']['kmers_per_s']
     out['unit'] = 'kmers/s'
-    out['note'] = ('plain FASTQ read by the block-parallel parser (host threads) into pinned double buffers; '
-                   'the next batch is parsed while the GPU inserts the current one (PCIe included); '
-                   'advance() waits for its inserts at every interval')
+    out['note'] = ('plain FASTQ read by the block-parallel parser (host threads) into pinned double buffers of '
+                   '256 MiB; the next batch is parsed while the GPU inserts the current one (PCIe included); '
+                   'advance() returns at every interval once its inserts are queued, and the storage\'s later '
+                   'calls wait for them; the clock stops after the last insert has finished')
     os.remove(path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)

@@ -246,6 +246,10 @@ struct gt_storage {
     uint64_t gen_issued = 0;           // generation bumps enqueued so far (device kGen follows it)
     uint64_t epoch = 0;                // bumped when device table pointers change (exact-set growth)
     std::vector<struct gt_pipeline*> pipelines;   // attached pipelines
+    // Work another stream queued on this storage without waiting for it (the
+    // file inserter's): later device calls on the storage wait for this event.
+    hipEvent_t fence = nullptr;
+    bool fence_set = false;
     std::recursive_mutex mu;
 
     Tables tables_view() const {
@@ -351,6 +355,13 @@ int zero_tables(gt_storage* st, hipStream_t s) {
         launch_fill_u64(st->tables.as<uint64_t>(), st->tables.bytes / 8, 0, s);
     }
     GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+// Orders stream s after work another stream queued on the storage without
+// waiting (storage_set_fence).
+int order_after_fence(gt_storage* st, hipStream_t s) {
+    if (st->fence_set) GT_HIP(hipStreamWaitEvent(s, st->fence, 0));
     return GT_OK;
 }
 
@@ -861,6 +872,17 @@ namespace gt {
 // translation units; returns code.
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
+// Marks the end of work queued on stream s for this storage by a caller that
+// does not wait for it (the file inserter); the storage's later device calls
+// wait for it (order_after_fence).
+int storage_set_fence(gt_storage* st, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    if (!st->fence) GT_HIP(hipEventCreateWithFlags(&st->fence, hipEventDisableTiming));
+    GT_HIP(hipEventRecord(st->fence, s));
+    st->fence_set = true;
+    return GT_OK;
+}
+
 // Replaces a Bloom / count-min storage's tables with zeroed tables of the
 // given sizes (the file loaders: goetia's load() takes the file's table
 // sizes, bitstorage.cc:239-263, bytestorage.cc:196-240).  Counters are kept;
@@ -999,6 +1021,7 @@ int gt_storage_free(gt_storage* st) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     (void)hipDeviceSynchronize();
     for (auto* p : st->pipelines) p->st = nullptr;   // their later calls fail cleanly
+    if (st->fence) (void)hipEventDestroy(st->fence);
     delete st;
     return GT_OK;
 }
@@ -1154,6 +1177,7 @@ int gt_storage_save_async(gt_storage* st, uint8_t* const* host_tables, uint64_t*
     if (!host_tables || !counters) return fail(GT_ERR_INVALID_ARG, "null host buffers");
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     for (uint32_t i = 0; i < st->n_tables; ++i) {
         GT_HIP(hipMemcpyAsync(host_tables[i], st->tables.as<uint8_t>() + st->table_off[i], st->nbytes[i],
@@ -1168,6 +1192,7 @@ int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, con
     if (!valid_storage(st) || st->kind == kExact) return fail(GT_ERR_INVALID_ARG, "a Bloom or count-min storage is needed");
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     if (!host_tables) {
         launch_fill_u64(st->tables.as<uint64_t>(), st->tables.bytes / 8, 0, s);
@@ -1284,6 +1309,7 @@ int gt_dbg_insert_dev(gt_storage* st, uint16_t K, int canonical, int mode, const
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return dbg_insert_impl(st, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new, new_flags,
                            (hipStream_t)stream);
 }
@@ -1335,6 +1361,7 @@ int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const 
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(st, R, true, s));
@@ -1378,6 +1405,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     if (st->kind == kByte && bin_buckets(st)) {
@@ -1450,6 +1478,7 @@ int gt_solid_filter_dev(gt_storage* st, uint16_t K, int canonical, int mode, con
     if (n_reads == 0) return GT_OK;
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     GT_TRY(st->solid_counts.ensure(std::max<uint64_t>(n_bases, 1) * sizeof(int16_t)));
     int16_t* counts = st->solid_counts.as<int16_t>();
@@ -1515,6 +1544,7 @@ int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const u
     if (n_reads == 0) return GT_OK;
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     // chunks end at read boundaries: the offsets come to the host once
     std::vector<int64_t> off(n_reads + 1);
@@ -1642,6 +1672,7 @@ int degrees_impl(gt_storage* st, uint16_t K, int canonical, int mode, int min_co
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(st, R, true, s));
@@ -1792,6 +1823,7 @@ int gt_insert_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, uin
     if (n && !hashes) return fail(GT_ERR_INVALID_ARG, "null hashes");
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return hashes_insert_impl(st, hashes, n, is_new, (hipStream_t)stream);
 }
 
@@ -1816,6 +1848,7 @@ int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int1
     GT_TRY(ensure_device());
     if (n == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     launch_hash_query(make_launch(st, s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
     GT_HIP(hipGetLastError());
@@ -2097,6 +2130,7 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
     GT_TRY(ensure_device());
     gt_storage* st = p->st;
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, p->compute));
     gt_pipeline::Slot& sl = p->slot[p->next];
     p->next ^= 1;
     GT_TRY(pipeline_deliver(sl));
@@ -2271,6 +2305,7 @@ int gt_dbg_insert_noalloc_dev(gt_storage* st, uint16_t K, int canonical, int mod
     NoAllocScratch& na = g_noalloc[g_dev.device];
     std::lock_guard<std::mutex> ln(na.mu);
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     GT_TRY(noalloc_setup(na));
     return dbg_insert_impl(st, na.sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
                            new_flags, (hipStream_t)stream);
@@ -2284,6 +2319,7 @@ int gt_dbg_insert_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canon
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lw(ws->mu);
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return dbg_insert_impl(st, ws->sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
                            new_flags, (hipStream_t)stream);
 }
@@ -2297,6 +2333,7 @@ int gt_dbg_query_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canoni
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lw(ws->mu);
     std::lock_guard<std::recursive_mutex> lk(st->mu);
+    GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(ws->sc, R, true, s));

@@ -47,6 +47,7 @@
 
 namespace gt {
 int set_error(int code, const std::string& msg);
+int storage_set_fence(gt_storage* st, hipStream_t s);
 }
 
 namespace {
@@ -778,7 +779,7 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     ins->canonical = canonical;
     ins->mode = mode;
     ins->interval = interval;
-    ins->max_bases = batch_bases ? batch_bases : (64ULL << 20);
+    ins->max_bases = batch_bases ? batch_bases : (256ULL << 20);
     ins->max_reads = std::max<uint64_t>(ins->max_bases / 64, 1024);
     hipError_t e = hipStreamCreate(&ins->stream);
     for (Slot& sl : ins->slot) {
@@ -801,9 +802,10 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
 // with IntervalCounter::poll (metrics.hh:126-135): reads are inserted until
 // the k-mers of the reads taken since the last interval reach `interval`
 // (each read counts len - K + 1, 0 when shorter than K); the call returns
-// right after the read that reaches it, or when the input is exhausted, with
-// every insert finished.  In between, the next batch is parsed into the other
-// slot while the GPU inserts the current one.
+// right after the read that reaches it, with its inserts queued on the
+// inserter's stream (every later call on the storage waits for them), or when
+// the input is exhausted, with every insert finished.  In between, the next
+// batch is parsed into the other slot while the GPU inserts the current one.
 int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequences, uint64_t* time_total,
                         int* remaining) {
     if (!ins || !parser) return set_error(GT_ERR_INVALID_ARG, "null inserter or parser");
@@ -856,11 +858,12 @@ int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequence
         const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
                                          nullptr, nullptr, ins->stream);
         if (rc != GT_OK) return rc;
+        GT_TRY_FX(gt::storage_set_fence(ins->st, ins->stream));
         ins->pending_work = true;
         sl.cur = end;
         if (sl.cur >= sl.n) ins->active ^= 1;   // parse into the other slot while this one is inserted
         if (crossed) {
-            GT_TRY_FX(sync_inserts(ins));
+            // the interval's inserts are queued; later calls on the storage wait for them
             if (n_sequences) *n_sequences = ins->n_sequences;
             if (time_total) *time_total = ins->total;
             if (remaining) *remaining = 1;
@@ -1146,6 +1149,7 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
             const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
                                              nullptr, nullptr, ins->stream);
             if (rc != GT_OK) return rc;
+            GT_TRY_FX(gt::storage_set_fence(ins->st, ins->stream));
             ins->pending_work = true;
             ins->active ^= 1;
         }
@@ -1154,7 +1158,7 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
             return err;
         }
         if (crossed || done) {
-            GT_TRY_FX(sync_inserts(ins));
+            if (done) GT_TRY_FX(sync_inserts(ins));
             if (n_sequences) *n_sequences = ins->n_sequences;
             if (time_total) *time_total = ins->total;
             if (remaining) *remaining = crossed ? 1 : 0;

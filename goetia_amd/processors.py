@@ -46,7 +46,7 @@ class InserterProcessor:
     """goetia::InserterProcessor<dBG<S, Sh>> for one graph."""
     DEFAULT_INTERVAL = IntervalCounter.DEFAULT_INTERVAL
 
-    def __init__(self, graph, interval=DEFAULT_INTERVAL, verbose=False, batch_bases=64 << 20):
+    def __init__(self, graph, interval=DEFAULT_INTERVAL, verbose=False, batch_bases=0):
         self.inserter = graph
         self.verbose = verbose
         self._interval = interval
