@@ -250,6 +250,12 @@ struct gt_storage {
     // file inserter's): later device calls on the storage wait for this event.
     hipEvent_t fence = nullptr;
     bool fence_set = false;
+    // Reads a file inserter has taken but not yet queued (it returned at an
+    // interval and queues them with its next batch): every call on the
+    // storage queues them first (StorageLock), so each call sees the storage
+    // as if they had been inserted when the inserter returned.
+    int (*flush_fn)(void*) = nullptr;
+    void* flush_ctx = nullptr;
     std::recursive_mutex mu;
 
     Tables tables_view() const {
@@ -357,6 +363,25 @@ int zero_tables(gt_storage* st, hipStream_t s) {
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
+
+// Queues the reads a file inserter holds for this storage (see flush_fn); the
+// caller holds st->mu.
+int flush_pending(gt_storage* st) {
+    if (!st->flush_fn) return GT_OK;
+    int (*fn)(void*) = st->flush_fn;
+    void* ctx = st->flush_ctx;
+    st->flush_fn = nullptr;            // the inserter's own calls below take the lock again
+    st->flush_ctx = nullptr;
+    return fn(ctx);
+}
+
+// The lock every call on a storage takes; it queues a file inserter's held
+// reads first.  rc is that flush's status.
+struct StorageLock {
+    std::lock_guard<std::recursive_mutex> g;
+    int rc;
+    explicit StorageLock(gt_storage* st) : g(st->mu), rc(flush_pending(st)) {}
+};
 
 // Orders stream s after work another stream queued on the storage without
 // waiting (storage_set_fence).
@@ -876,12 +901,35 @@ int set_error(int code, const std::string& msg) { return fail(code, msg); }
 // does not wait for it (the file inserter); the storage's later device calls
 // wait for it (order_after_fence).
 int storage_set_fence(gt_storage* st, hipStream_t s) {
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     if (!st->fence) GT_HIP(hipEventCreateWithFlags(&st->fence, hipEventDisableTiming));
     GT_HIP(hipEventRecord(st->fence, s));
     st->fence_set = true;
     return GT_OK;
 }
+
+// Makes fn(ctx) the storage's holder of reads taken but not yet queued (a
+// file inserter); every later call on the storage runs it first
+// (StorageLock).  Another holder's reads are queued first.  fn = nullptr
+// ends ctx's hold.
+int storage_hold(gt_storage* st, int (*fn)(void*), void* ctx) {
+    std::lock_guard<std::recursive_mutex> g(st->mu);
+    if (!fn) {
+        if (st->flush_ctx == ctx) {
+            st->flush_fn = nullptr;
+            st->flush_ctx = nullptr;
+        }
+        return GT_OK;
+    }
+    if (st->flush_fn && st->flush_ctx != ctx) GT_TRY(flush_pending(st));
+    st->flush_fn = fn;
+    st->flush_ctx = ctx;
+    return GT_OK;
+}
+
+// The storage's lock, for a holder that changes what it holds.
+std::recursive_mutex& storage_mutex(gt_storage* st) { return st->mu; }
 
 // Replaces a Bloom / count-min storage's tables with zeroed tables of the
 // given sizes (the file loaders: goetia's load() takes the file's table
@@ -893,7 +941,8 @@ int storage_reshape(gt_storage* st, const uint64_t* sizes, uint32_t n_tables) {
     for (uint32_t i = 0; i < n_tables; ++i)
         if (sizes[i] == 0 || sizes[i] >= kMaxBins) return fail(GT_ERR_INVALID_ARG, "table sizes must be in 1..2^32-1");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     if (st->n_tables == n_tables && std::equal(sizes, sizes + n_tables, st->sizes.begin())) return GT_OK;
     GT_HIP(hipDeviceSynchronize());
     std::vector<uint64_t> sz(sizes, sizes + n_tables), nb, off;
@@ -1019,6 +1068,9 @@ int gt_storage_new(int kind, const uint64_t* tablesizes, uint32_t n_tables, uint
 int gt_storage_free(gt_storage* st) {
     if (!st) return GT_OK;
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    {
+        StorageLock lk(st);            // a file inserter's held reads go in before the storage goes
+    }
     (void)hipDeviceSynchronize();
     for (auto* p : st->pipelines) p->st = nullptr;   // their later calls fail cleanly
     if (st->fence) (void)hipEventDestroy(st->fence);
@@ -1034,7 +1086,8 @@ int gt_storage_clone(const gt_storage* st, gt_storage** out) {
 int gt_storage_reset(gt_storage* st) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(zero_tables(st, nullptr));
     if (st->kind == kExact) {
         launch_fill_u64(st->counters.as<uint64_t>() + kNUnique, 2, 0, nullptr);   // n_unique, last-was-present
@@ -1086,7 +1139,8 @@ static uint64_t set_bucket_count(uint64_t n_unique, bool last_was_present) {
 int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     uint64_t c[kNCounters];
     GT_HIP(hipDeviceSynchronize());
     GT_TRY(read_counters(st, c, nullptr));
@@ -1100,7 +1154,8 @@ int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied
 int gt_storage_set_counters(gt_storage* st, uint64_t n_unique, uint64_t n_occupied) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     uint64_t c[2] = {n_unique, n_occupied};
     GT_HIP(hipMemcpy(st->counters.p, c, sizeof(c), hipMemcpyHostToDevice));
     return GT_OK;
@@ -1111,7 +1166,8 @@ int gt_storage_read_table(gt_storage* st, uint32_t i, uint8_t* host_dst) {
     if (st->kind == kExact) return fail(GT_ERR_UNSUPPORTED, "the exact set has no raw tables (get_raw_tables() is null)");
     if (i >= st->n_tables) return fail(GT_ERR_INVALID_ARG, "table index out of range");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_HIP(hipDeviceSynchronize());
     GT_HIP(hipMemcpy(host_dst, st->tables.as<uint8_t>() + st->table_off[i], st->nbytes[i], hipMemcpyDeviceToHost));
     return GT_OK;
@@ -1122,7 +1178,8 @@ int gt_storage_write_table(gt_storage* st, uint32_t i, const uint8_t* host_src) 
     if (st->kind == kExact) return fail(GT_ERR_UNSUPPORTED, "the exact set has no raw tables");
     if (i >= st->n_tables) return fail(GT_ERR_INVALID_ARG, "table index out of range");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_HIP(hipDeviceSynchronize());
     GT_HIP(hipMemcpy(st->tables.as<uint8_t>() + st->table_off[i], host_src, st->nbytes[i], hipMemcpyHostToDevice));
     return GT_OK;
@@ -1133,7 +1190,8 @@ int gt_storage_update_from(gt_storage* st, const gt_storage* other) {
     if (st->kind != kBit || other->kind != kBit) return fail(GT_ERR_UNSUPPORTED, "update_from is defined for BitStorage");
     if (st->sizes != other->sizes) return fail(GT_ERR_INVALID_ARG, "both nodegraphs must have same table sizes");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_HIP(hipDeviceSynchronize());
     for (uint32_t i = 0; i < st->n_tables; ++i) {
         const uint64_t words = (st->nbytes[i] + 3) / 4;
@@ -1150,7 +1208,8 @@ int gt_storage_exact_keys(gt_storage* st, uint64_t* host_out, uint64_t cap, uint
     if (!valid_storage(st) || !n) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
     if (st->kind != kExact) return fail(GT_ERR_UNSUPPORTED, "exact_keys is defined for the exact set");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_HIP(hipDeviceSynchronize());
     DevBuf out, cnt;
     GT_TRY(out.ensure(std::max<uint64_t>(cap, 1) * sizeof(uint64_t)));
@@ -1176,7 +1235,8 @@ int gt_storage_save_async(gt_storage* st, uint8_t* const* host_tables, uint64_t*
     if (!valid_storage(st) || st->kind == kExact) return fail(GT_ERR_INVALID_ARG, "a Bloom or count-min storage is needed");
     if (!host_tables || !counters) return fail(GT_ERR_INVALID_ARG, "null host buffers");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     for (uint32_t i = 0; i < st->n_tables; ++i) {
@@ -1191,7 +1251,8 @@ int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, con
                           void* stream) {
     if (!valid_storage(st) || st->kind == kExact) return fail(GT_ERR_INVALID_ARG, "a Bloom or count-min storage is needed");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     if (!host_tables) {
@@ -1308,7 +1369,8 @@ int gt_dbg_insert_dev(gt_storage* st, uint16_t K, int canonical, int mode, const
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return dbg_insert_impl(st, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new, new_flags,
                            (hipStream_t)stream);
@@ -1321,7 +1383,8 @@ int gt_dbg_insert(gt_storage* st, uint16_t K, int canonical, int mode, const uin
     const uint64_t n_bases = n_reads ? (uint64_t)offsets[n_reads] : 0;
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     hipStream_t s = nullptr;
     const uint64_t nk = packed_kmers(offsets, n_reads, K);
     GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
@@ -1360,7 +1423,8 @@ int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const 
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
@@ -1383,7 +1447,8 @@ int gt_dbg_query(gt_storage* st, uint16_t K, int canonical, int mode, const uint
     const uint64_t nk = packed_kmers(offsets, n_reads, K);
     if (nk == 0) return GT_OK;
     hipStream_t s = nullptr;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
     GT_TRY(stage_in(st->h_off, offsets, n_reads + 1, s));
     GT_TRY(st->h_out1.ensure(nk * sizeof(int16_t)));
@@ -1404,7 +1469,8 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
@@ -1477,7 +1543,8 @@ int gt_solid_filter_dev(gt_storage* st, uint16_t K, int canonical, int mode, con
     GT_TRY(ensure_device());
     if (n_reads == 0) return GT_OK;
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     GT_TRY(st->solid_counts.ensure(std::max<uint64_t>(n_bases, 1) * sizeof(int16_t)));
@@ -1543,7 +1610,8 @@ int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const u
     GT_TRY(ensure_device());
     if (n_reads == 0) return GT_OK;
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     // chunks end at read boundaries: the offsets come to the host once
@@ -1671,7 +1739,8 @@ int degrees_impl(gt_storage* st, uint16_t K, int canonical, int mode, int min_co
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
@@ -1776,7 +1845,8 @@ int segment_view(gt_storage* st, uint16_t K, const uint8_t* seq, const int64_t* 
     const uint64_t nk = packed_kmers(offsets, n_reads, K);
     if (nk == 0) return GT_OK;
     hipStream_t s = nullptr;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(stage_in(st->h_seq, seq, n_bases, s));
     GT_TRY(stage_in(st->h_off, offsets, n_reads + 1, s));
     GT_TRY(st->h_out1.ensure(nk * sizeof(int16_t)));
@@ -1822,7 +1892,8 @@ int gt_insert_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, uin
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     if (n && !hashes) return fail(GT_ERR_INVALID_ARG, "null hashes");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return hashes_insert_impl(st, hashes, n, is_new, (hipStream_t)stream);
 }
@@ -1832,7 +1903,8 @@ int gt_insert_hashes(gt_storage* st, const uint64_t* hashes, uint64_t n, uint8_t
     if (n && !hashes) return fail(GT_ERR_INVALID_ARG, "null hashes");
     GT_TRY(ensure_device());
     if (n == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     hipStream_t s = nullptr;
     GT_TRY(stage_in(st->h_out3, hashes, n, s));
     GT_TRY(st->h_out2.ensure(n));
@@ -1847,7 +1919,8 @@ int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int1
     if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
     GT_TRY(ensure_device());
     if (n == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     launch_hash_query(make_launch(st, s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
@@ -1878,7 +1951,8 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
     if (n && (!hashes || !counts)) return fail(GT_ERR_INVALID_ARG, "null pointer");
     GT_TRY(ensure_device());
     if (n == 0) return GT_OK;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     hipStream_t s = nullptr;
     GT_TRY(stage_in(st->h_out3, hashes, n, s));
     GT_TRY(st->h_out2.ensure(n));
@@ -1922,7 +1996,8 @@ int gt_autotune(gt_storage* st, uint16_t K, int canonical, int mode, const uint8
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return fail(GT_ERR_INVALID_ARG, "autotune needs a non-empty batch");
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
     GT_TRY(prepare_reads(st, R, true, s));
@@ -2072,7 +2147,8 @@ int gt_pipeline_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     if (max_reads == 0 || max_bases == 0 || max_reads >= 0xFFFFFFFFULL)
         return fail(GT_ERR_INVALID_ARG, "invalid batch bounds");
     GT_TRY(ensure_device());
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     const uint64_t per = std::max<uint32_t>(st->n_tables, 1);
     const uint64_t cap = next_pow2(2 * per * (max_bases + kTileMax));
     if (st->kind != kExact && cap > (1ULL << 29))
@@ -2129,7 +2205,8 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
     if (n_bases && !seq) return fail(GT_ERR_INVALID_ARG, "null reads");
     GT_TRY(ensure_device());
     gt_storage* st = p->st;
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, p->compute));
     gt_pipeline::Slot& sl = p->slot[p->next];
     p->next ^= 1;
@@ -2174,7 +2251,8 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
 
 int gt_pipeline_flush(gt_pipeline* p) {
     if (!p || !p->st) return fail(GT_ERR_INVALID_ARG, "invalid pipeline");
-    std::lock_guard<std::recursive_mutex> lk(p->st->mu);
+    StorageLock lk(p->st);
+    GT_TRY(lk.rc);
     // deliver in submission order
     GT_TRY(pipeline_deliver(p->slot[p->next]));
     GT_TRY(pipeline_deliver(p->slot[p->next ^ 1]));
@@ -2185,7 +2263,7 @@ int gt_pipeline_flush(gt_pipeline* p) {
 int gt_pipeline_free(gt_pipeline* p) {
     if (!p) return GT_OK;
     if (p->st) {
-        std::lock_guard<std::recursive_mutex> lk(p->st->mu);
+        StorageLock lk(p->st);   // a failed flush of held reads does not keep the pipeline alive
         (void)pipeline_deliver(p->slot[0]);
         (void)pipeline_deliver(p->slot[1]);
         auto& v = p->st->pipelines;
@@ -2304,7 +2382,8 @@ int gt_dbg_insert_noalloc_dev(gt_storage* st, uint16_t K, int canonical, int mod
     if (g_dev.device < 0 || g_dev.device >= kNaDevices) return fail(GT_ERR_DEVICE, "device index out of range");
     NoAllocScratch& na = g_noalloc[g_dev.device];
     std::lock_guard<std::mutex> ln(na.mu);
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     GT_TRY(noalloc_setup(na));
     return dbg_insert_impl(st, na.sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
@@ -2318,7 +2397,8 @@ int gt_dbg_insert_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canon
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     GT_TRY(ensure_device());
     std::lock_guard<std::recursive_mutex> lw(ws->mu);
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     return dbg_insert_impl(st, ws->sc, make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical), n_new,
                            new_flags, (hipStream_t)stream);
@@ -2332,7 +2412,8 @@ int gt_dbg_query_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canoni
     GT_TRY(ensure_device());
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     std::lock_guard<std::recursive_mutex> lw(ws->mu);
-    std::lock_guard<std::recursive_mutex> lk(st->mu);
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);

@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <future>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -48,6 +49,8 @@
 namespace gt {
 int set_error(int code, const std::string& msg);
 int storage_set_fence(gt_storage* st, hipStream_t s);
+int storage_hold(gt_storage* st, int (*fn)(void*), void* ctx);
+std::recursive_mutex& storage_mutex(gt_storage* st);
 }
 
 namespace {
@@ -428,6 +431,44 @@ size_t block_bytes() {
     return 64u << 20;
 }
 
+// A run of consecutive kept records of one parsed part going into a batch:
+// their bases in one piece, and their end offsets shifted by `delta`.
+struct RunCopy {
+    uint8_t* dst;
+    const char* src;
+    int64_t* dst_off;
+    const int64_t* src_off;
+    size_t count;
+    int64_t delta;
+};
+
+void copy_run(const RunCopy& r) {
+    const int64_t nb = r.src_off[r.count - 1] - (r.src_off[-1]);
+    std::memcpy(r.dst, r.src, (size_t)nb);
+    for (size_t k = 0; k < r.count; ++k) r.dst_off[k] = r.src_off[k] + r.delta;
+}
+
+// Copies the runs (on several threads when there is much to copy) and empties
+// the list.
+void copy_runs(std::vector<RunCopy>& runs) {
+    if (runs.empty()) return;
+    size_t total = 0;
+    for (const RunCopy& r : runs) total += (size_t)(r.src_off[r.count - 1] - r.src_off[-1]);
+    const int T = (int)std::min<size_t>({runs.size(), (size_t)std::max(1, block_threads() / 2), total >> 20});
+    if (T <= 1) {
+        for (const RunCopy& r : runs) copy_run(r);
+    } else {
+        std::vector<std::thread> pool;
+        auto work = [&](int t) {
+            for (size_t i = (size_t)t; i < runs.size(); i += (size_t)T) copy_run(runs[i]);
+        };
+        for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+    }
+    runs.clear();
+}
+
 // Parses the block of the mapping starting at `pos` (a record start): up to
 // the last verifiable record start within `block` bytes, growing the block
 // while it holds less than one record.
@@ -575,9 +616,13 @@ int gt_fastx_next_batch_qual(gt_fastx* p, uint64_t max_reads, uint64_t max_bases
     offsets[0] = 0;
     if (names) name_offsets[0] = 0;
     *n_reads = 0;
+    // Runs of records taken from the current block, copied together before
+    // the block is released (next_block) and before returning.
+    std::vector<RunCopy> runs;
     for (;;) {
         if (p->par) {
             if (p->cur_part >= p->cur.parts.size()) {
+                copy_runs(runs);
                 const int r = next_block(*p);
                 if (r == 0) {
                     p->complete = true;
@@ -598,6 +643,25 @@ int gt_fastx_next_batch_qual(gt_fastx* p, uint64_t max_reads, uint64_t max_bases
                 ++p->cur_part;
                 p->cur_rec = 0;
                 continue;
+            }
+            if (!qual && !names) {
+                // every record of the part from cur_rec that fits, as one run
+                const size_t i = p->cur_rec;
+                const size_t jmax = std::min<uint64_t>(B.count(), i + (max_reads - n));
+                const int64_t lim = B.off[i] + (int64_t)(max_bases - bases);
+                const size_t j =
+                    (size_t)(std::upper_bound(B.off.begin() + i, B.off.begin() + jmax + 1, lim) - B.off.begin()) - 1;
+                if (j > i) {
+                    runs.push_back(RunCopy{seq + bases, B.seq.data() + B.off[i], offsets + n + 1, B.off.data() + i + 1,
+                                           j - i, (int64_t)bases - B.off[i]});
+                    bases += (uint64_t)(B.off[j] - B.off[i]);
+                    n += j - i;
+                    p->cur_rec = j;
+                    p->n_parsed = p->base_parsed + p->part_parsed + B.cum_parsed[j - 1];
+                    p->n_skipped = p->base_skipped + p->part_skipped + B.cum_skipped[j - 1];
+                    continue;
+                }
+                // the record at cur_rec does not fit: the record-by-record step below stops
             }
             const size_t i = p->cur_rec;
             const uint64_t len = (uint64_t)(B.off[i + 1] - B.off[i]);
@@ -669,6 +733,7 @@ int gt_fastx_next_batch_qual(gt_fastx* p, uint64_t max_reads, uint64_t max_bases
         ++n;
         p->pending = false;
     }
+    copy_runs(runs);
     *n_reads = n;
     return GT_OK;
 }
@@ -698,15 +763,19 @@ int gt_fastx_close(gt_fastx* p) {
 
 namespace {
 
-// One batch slot: parsed reads in pinned host memory and their device copy.
+// One batch slot: reads in pinned host memory and their device copy.  Reads
+// [0, cur) are queued on the device, [cur, end) are taken (counted by the
+// interval) but held, [end, n) are parsed and not yet taken.
 struct Slot {
     uint8_t* h_seq = nullptr;
-    int64_t* h_off = nullptr;              // offsets as parsed
-    int64_t* h_sub = nullptr;              // offsets of the segment being inserted, rebased
+    int64_t* h_off = nullptr;              // offsets of the slot's reads (h_off[0] = 0)
+    int64_t* h_sub = nullptr;              // offsets of a range that starts past read 0, rebased
     uint8_t* d_seq = nullptr;
     int64_t* d_off = nullptr;
-    hipEvent_t copied = nullptr;           // the host buffers may be refilled after this
-    uint64_t n = 0, cur = 0;               // parsed reads, next one to insert
+    int64_t* d_sub = nullptr;
+    hipEvent_t copied = nullptr;           // the host buffers' last copy (copy stream)
+    hipEvent_t inserted = nullptr;         // the device buffers' last insert (insert stream)
+    uint64_t n = 0, cur = 0, end = 0;
 };
 
 }  // namespace
@@ -719,21 +788,27 @@ struct gt_inserter {
     uint64_t counter = 0, total = 0, n_sequences = 0;
     uint64_t max_reads = 0, max_bases = 0;
     Slot slot[2];
-    int active = 0;                        // slot holding the reads being inserted
-    hipStream_t stream = nullptr;
-    bool pending_work = false;             // inserts enqueued since the last synchronisation
+    int active = 0;                        // the slot reads are taken from (or paired reads appended to)
+    hipStream_t stream = nullptr;          // inserts
+    hipStream_t copy = nullptr;            // host-to-device copies
+    bool pending_work = false;             // inserts queued since the last synchronisation
+    bool holding = false;                  // registered with the storage as holder of taken reads
 
     ~gt_inserter() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (copy) (void)hipStreamSynchronize(copy);
         for (Slot& sl : slot) {
             if (sl.h_seq) (void)hipHostFree(sl.h_seq);
             if (sl.h_off) (void)hipHostFree(sl.h_off);
             if (sl.h_sub) (void)hipHostFree(sl.h_sub);
             if (sl.d_seq) (void)hipFree(sl.d_seq);
             if (sl.d_off) (void)hipFree(sl.d_off);
+            if (sl.d_sub) (void)hipFree(sl.d_sub);
             if (sl.copied) (void)hipEventDestroy(sl.copied);
+            if (sl.inserted) (void)hipEventDestroy(sl.inserted);
         }
         if (stream) (void)hipStreamDestroy(stream);
+        if (copy) (void)hipStreamDestroy(copy);
     }
 };
 
@@ -755,11 +830,80 @@ int hip_fail(hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(e_, #call);     \
     } while (0)
 
+using StorageGuard = std::lock_guard<std::recursive_mutex>;
+
 int sync_inserts(gt_inserter* ins) {
     if (ins->pending_work) {
         GT_FX_HIP(hipStreamSynchronize(ins->stream));
         ins->pending_work = false;
     }
+    return GT_OK;
+}
+
+// Queues the slot's taken reads [cur, end) as one insert: their bases and
+// offsets go to the device on the copy stream (after the slot's previous
+// insert has stopped reading the device buffers), the insert stream waits
+// for the copy, and the storage's fence follows the insert.  The caller
+// holds the storage's lock.
+int queue_taken(gt_inserter* ins, Slot& sl) {
+    if (ins->holding) {
+        ins->holding = false;
+        GT_TRY_FX(gt::storage_hold(ins->st, nullptr, ins));
+    }
+    if (sl.cur >= sl.end) return GT_OK;
+    const uint64_t nr = sl.end - sl.cur;
+    const int64_t b0 = sl.h_off[sl.cur];
+    const uint64_t nb = (uint64_t)(sl.h_off[sl.end] - b0);
+    GT_FX_HIP(hipStreamWaitEvent(ins->copy, sl.inserted, 0));
+    const int64_t* d_offsets = sl.d_off;
+    if (sl.cur == 0) {
+        GT_FX_HIP(hipMemcpyAsync(sl.d_off, sl.h_off, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ins->copy));
+    } else {
+        // a range after a flush: rebase its offsets (h_sub's previous copy must be done)
+        GT_FX_HIP(hipEventSynchronize(sl.copied));
+        for (uint64_t i = 0; i <= nr; ++i) sl.h_sub[i] = sl.h_off[sl.cur + i] - b0;
+        GT_FX_HIP(hipMemcpyAsync(sl.d_sub, sl.h_sub, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ins->copy));
+        d_offsets = sl.d_sub;
+    }
+    if (nb) GT_FX_HIP(hipMemcpyAsync(sl.d_seq + b0, sl.h_seq + b0, nb, hipMemcpyHostToDevice, ins->copy));
+    GT_FX_HIP(hipEventRecord(sl.copied, ins->copy));
+    GT_FX_HIP(hipStreamWaitEvent(ins->stream, sl.copied, 0));
+    sl.cur = sl.end;
+    const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq + b0, d_offsets, nr, nb,
+                                     nullptr, nullptr, ins->stream);
+    if (rc != GT_OK) return rc;
+    GT_FX_HIP(hipEventRecord(sl.inserted, ins->stream));
+    GT_TRY_FX(gt::storage_set_fence(ins->st, ins->stream));
+    ins->pending_work = true;
+    return GT_OK;
+}
+
+// The storage's flush of an inserter's held reads (they are in the active slot).
+int flush_held(void* ctx) {
+    gt_inserter* ins = static_cast<gt_inserter*>(ctx);
+    ins->holding = false;              // the storage has already dropped the hold
+    return queue_taken(ins, ins->slot[ins->active]);
+}
+
+// Holds the active slot's taken reads until the next batch or the storage's
+// next call (caller holds the storage's lock).
+int hold_taken(gt_inserter* ins) {
+    const Slot& sl = ins->slot[ins->active];
+    if (sl.cur >= sl.end || ins->holding) return GT_OK;
+    GT_TRY_FX(gt::storage_hold(ins->st, flush_held, ins));
+    ins->holding = true;
+    return GT_OK;
+}
+
+// Queues the active slot's taken reads and makes the other slot active and
+// empty (caller holds the storage's lock); the caller then waits for the new
+// slot's host buffers (hipEventSynchronize of its `copied`) before filling it.
+int rotate(gt_inserter* ins) {
+    GT_TRY_FX(queue_taken(ins, ins->slot[ins->active]));
+    ins->active ^= 1;
+    Slot& nx = ins->slot[ins->active];
+    nx.n = nx.cur = nx.end = 0;
+    nx.h_off[0] = 0;
     return GT_OK;
 }
 
@@ -782,13 +926,21 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     ins->max_bases = batch_bases ? batch_bases : (256ULL << 20);
     ins->max_reads = std::max<uint64_t>(ins->max_bases / 64, 1024);
     hipError_t e = hipStreamCreate(&ins->stream);
+    if (e == hipSuccess) e = hipStreamCreate(&ins->copy);
     for (Slot& sl : ins->slot) {
+        const size_t ob = (ins->max_reads + 1) * sizeof(int64_t);
         if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_seq, ins->max_bases);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_off, (ins->max_reads + 1) * sizeof(int64_t));
-        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_sub, (ins->max_reads + 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_off, ob);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_sub, ob);
         if (e == hipSuccess) e = hipMalloc((void**)&sl.d_seq, ins->max_bases);
-        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_off, (ins->max_reads + 1) * sizeof(int64_t));
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_off, ob);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.d_sub, ob);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.inserted, hipEventDisableTiming);
+        // recorded once, so the first waits on them return at once
+        if (e == hipSuccess) e = hipEventRecord(sl.copied, ins->copy);
+        if (e == hipSuccess) e = hipEventRecord(sl.inserted, ins->stream);
+        if (e == hipSuccess) sl.h_off[0] = 0;
     }
     if (e != hipSuccess) {
         delete ins;
@@ -799,29 +951,39 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
 }
 
 // InserterProcessor / FileProcessor::advance (processors.hh:206-231, 304-306)
-// with IntervalCounter::poll (metrics.hh:126-135): reads are inserted until
-// the k-mers of the reads taken since the last interval reach `interval`
-// (each read counts len - K + 1, 0 when shorter than K); the call returns
-// right after the read that reaches it, with its inserts queued on the
-// inserter's stream (every later call on the storage waits for them), or when
-// the input is exhausted, with every insert finished.  In between, the next
-// batch is parsed into the other slot while the GPU inserts the current one.
+// with IntervalCounter::poll (metrics.hh:126-135): reads are taken until the
+// k-mers of the reads taken since the last interval reach `interval` (each
+// read counts len - K + 1, 0 when shorter than K); the call returns right
+// after the read that reaches it, or, with every insert finished, at the end
+// of the input.  Taken reads are inserted in batches of up to the slot size:
+// those taken when the call returns are held, and go in with the next batch
+// or before the storage's next call, whichever comes first, so every call on
+// the storage sees every read taken so far.  While the GPU inserts one slot
+// the next batch is parsed into the other.
 int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequences, uint64_t* time_total,
                         int* remaining) {
     if (!ins || !parser) return set_error(GT_ERR_INVALID_ARG, "null inserter or parser");
+    std::recursive_mutex& mu = gt::storage_mutex(ins->st);
     for (;;) {
         Slot& sl = ins->slot[ins->active];
-        if (sl.cur >= sl.n) {
-            // refill this slot once its previous copy to the device is done
-            GT_FX_HIP(hipEventSynchronize(sl.copied));
-            sl.n = sl.cur = 0;
+        if (sl.end >= sl.n) {
+            // every parsed read is taken: queue them, parse the next batch into the other slot
+            {
+                StorageGuard g(mu);
+                GT_TRY_FX(rotate(ins));
+            }
+            Slot& nx = ins->slot[ins->active];
+            GT_FX_HIP(hipEventSynchronize(nx.copied));
             uint64_t got = 0;
-            const int rc = gt_fastx_next_batch(parser, ins->max_reads, ins->max_bases, sl.h_seq, sl.h_off, nullptr, 0,
+            const int rc = gt_fastx_next_batch(parser, ins->max_reads, ins->max_bases, nx.h_seq, nx.h_off, nullptr, 0,
                                                nullptr, &got);
             // FileProcessor::handle_next (processors.hh:149-171) skips a record
             // with an invalid character even from a strict parser
             if (rc != GT_OK && rc != GT_ERR_INVALID_CHARACTER) return rc;
-            sl.n = got;
+            {
+                StorageGuard g(mu);
+                nx.n = got;
+            }
             if (got == 0 && rc == GT_ERR_INVALID_CHARACTER) continue;
             if (got == 0) {
                 GT_TRY_FX(sync_inserts(ins));
@@ -830,8 +992,9 @@ int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequence
                 if (remaining) *remaining = 0;
                 return GT_OK;
             }
+            continue;
         }
-        uint64_t end = sl.cur;
+        uint64_t end = sl.end;
         bool crossed = false;
         while (end < sl.n) {
             const int64_t len = sl.h_off[end + 1] - sl.h_off[end];
@@ -846,24 +1009,12 @@ int gt_inserter_advance(gt_inserter* ins, gt_fastx* parser, uint64_t* n_sequence
                 break;
             }
         }
-        const uint64_t nr = end - sl.cur;
-        const int64_t b0 = sl.h_off[sl.cur];
-        const uint64_t nb = (uint64_t)(sl.h_off[end] - b0);
-        // the rebased offsets of this slot's previous segment may still be in flight
-        GT_FX_HIP(hipEventSynchronize(sl.copied));
-        for (uint64_t i = 0; i <= nr; ++i) sl.h_sub[i] = sl.h_off[sl.cur + i] - b0;
-        if (nb) GT_FX_HIP(hipMemcpyAsync(sl.d_seq, sl.h_seq + b0, nb, hipMemcpyHostToDevice, ins->stream));
-        GT_FX_HIP(hipMemcpyAsync(sl.d_off, sl.h_sub, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ins->stream));
-        GT_FX_HIP(hipEventRecord(sl.copied, ins->stream));
-        const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
-                                         nullptr, nullptr, ins->stream);
-        if (rc != GT_OK) return rc;
-        GT_TRY_FX(gt::storage_set_fence(ins->st, ins->stream));
-        ins->pending_work = true;
-        sl.cur = end;
-        if (sl.cur >= sl.n) ins->active ^= 1;   // parse into the other slot while this one is inserted
+        {
+            StorageGuard g(mu);
+            sl.end = end;
+            if (crossed) GT_TRY_FX(hold_taken(ins));
+        }
         if (crossed) {
-            // the interval's inserts are queued; later calls on the storage wait for them
             if (n_sequences) *n_sequences = ins->n_sequences;
             if (time_total) *time_total = ins->total;
             if (remaining) *remaining = 1;
@@ -880,9 +1031,17 @@ int gt_inserter_stats(const gt_inserter* ins, uint64_t* n_sequences, uint64_t* t
     return GT_OK;
 }
 
+// Held reads go in before the inserter goes (the storage must outlive its
+// inserters; a storage freed first has already taken them).
 int gt_inserter_free(gt_inserter* ins) {
+    if (!ins) return GT_OK;
+    int rc = GT_OK;
+    if (ins->holding) {
+        StorageGuard g(gt::storage_mutex(ins->st));
+        rc = queue_taken(ins, ins->slot[ins->active]);
+    }
     delete ins;
-    return GT_OK;
+    return rc;
 }
 
 }  // extern "C"
@@ -1086,23 +1245,28 @@ int gt_pairs_close(gt_pairs* pr) {
 }
 
 // FileProcessor::advance over a SplitPairedReader for an InserterProcessor
-// (processors.hh:180-204, 304-306): pairs are appended to the active host
-// slot, left then right; the slot is inserted when it is full, when a pair
-// brings the interval counter to `interval` (the call then returns, every
-// insert finished) or at the end of the input.
+// (processors.hh:180-204, 304-306): pairs are appended to the active slot,
+// left then right, and taken as they come; a full slot is queued and the
+// other one filled.  The call returns when a pair brings the interval
+// counter to `interval` (its reads held as in gt_inserter_advance), or, with
+// every insert finished, at the end of the input or when the reader raises.
 int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequences, uint64_t* time_total,
                               int* remaining) {
     if (!ins || !pr) return set_error(GT_ERR_INVALID_ARG, "null inserter or reader");
     for (const Slot& s : ins->slot)
-        if (s.cur < s.n) return set_error(GT_ERR_INVALID_ARG, "the inserter holds reads of a single file");
+        if (s.end < s.n) return set_error(GT_ERR_INVALID_ARG, "the inserter holds reads of a single file");
+    std::recursive_mutex& mu = gt::storage_mutex(ins->st);
     for (;;) {
         Slot& sl = ins->slot[ins->active];
-        GT_FX_HIP(hipEventSynchronize(sl.copied));   // the slot's previous copy is done
-        uint64_t nr = 0, nb = 0;
-        sl.h_sub[0] = 0;
+        uint64_t nr = sl.n;
+        int64_t nb = sl.h_off[nr];
         bool crossed = false, done = false, full = false;
         int err = GT_OK;
-        while (nr + 2 <= ins->max_reads) {           // a pair adds at most two reads
+        for (;;) {
+            if (nr + 2 > ins->max_reads) {             // a pair adds at most two reads
+                full = true;
+                break;
+            }
             PairStep step = kPairReady;
             err = next_pair(pr, &step);
             if (err != GT_OK) break;
@@ -1114,7 +1278,7 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
             uint64_t need = 0;
             for (int k = 0; k < 2; ++k)
                 if (pr->present[k]) need += pr->seq[k].size();
-            if (nb + need > ins->max_bases) {
+            if ((uint64_t)nb + need > ins->max_bases) {
                 if (nr == 0) {
                     err = set_error(GT_ERR_INVALID_ARG, "a read pair does not fit the batch buffers");
                     break;
@@ -1128,8 +1292,8 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
                 if (!pr->present[k]) continue;
                 const uint64_t len = pr->seq[k].size();
                 std::memcpy(sl.h_seq + nb, pr->seq[k].data(), len);
-                nb += len;
-                sl.h_sub[++nr] = (int64_t)nb;
+                nb += (int64_t)len;
+                sl.h_off[++nr] = nb;
                 t += len >= ins->K ? len - ins->K + 1 : 0;
                 ++ins->n_sequences;
             }
@@ -1141,17 +1305,15 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
                 break;
             }
         }
-        if (nr) {
-            GT_FX_HIP(hipMemcpyAsync(sl.d_seq, sl.h_seq, nb, hipMemcpyHostToDevice, ins->stream));
-            GT_FX_HIP(hipMemcpyAsync(sl.d_off, sl.h_sub, (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
-                                     ins->stream));
-            GT_FX_HIP(hipEventRecord(sl.copied, ins->stream));
-            const int rc = gt_dbg_insert_dev(ins->st, ins->K, ins->canonical, ins->mode, sl.d_seq, sl.d_off, nr, nb,
-                                             nullptr, nullptr, ins->stream);
-            if (rc != GT_OK) return rc;
-            GT_TRY_FX(gt::storage_set_fence(ins->st, ins->stream));
-            ins->pending_work = true;
-            ins->active ^= 1;
+        {
+            StorageGuard g(mu);
+            sl.n = sl.end = nr;
+            if (full)
+                GT_TRY_FX(rotate(ins));
+            else if (done || err != GT_OK)
+                GT_TRY_FX(queue_taken(ins, sl));
+            else if (crossed)
+                GT_TRY_FX(hold_taken(ins));
         }
         if (err != GT_OK) {
             GT_TRY_FX(sync_inserts(ins));
@@ -1164,7 +1326,8 @@ int gt_inserter_advance_pairs(gt_inserter* ins, gt_pairs* pr, uint64_t* n_sequen
             if (remaining) *remaining = crossed ? 1 : 0;
             return GT_OK;
         }
-        (void)full;   // a full slot (by reads or bases) was inserted; fill the other one
+        // full: fill the other slot once its host buffers are free
+        GT_FX_HIP(hipEventSynchronize(ins->slot[ins->active].copied));
     }
 }
 

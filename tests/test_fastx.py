@@ -220,3 +220,56 @@ def test_record_iteration_has_qualities():
     recs = list(FastxParser(os.path.join(DATA, 'test-fastq-reads.fq')))
     exp = ofx.parse(os.path.join(DATA, 'test-fastq-reads.fq'), with_quality=True)
     assert [r.quality for r in recs] == exp['qualities'] and all(len(r.quality) == len(r.sequence) for r in recs)
+
+
+def _batches(path, min_length, max_reads, max_bases, names):
+    """Each batch's reads and the parser's (n_parsed, n_skipped) after it."""
+    p = FastxParser(path, False, min_length)
+    out = []
+    while True:
+        r = p.next_batch(max_reads, max_bases, names=names)
+        batch = r[0] if names else r
+        if batch.n_reads == 0:
+            break
+        out.append(([batch.read(i) for i in range(batch.n_reads)], p._stats()[:2]))
+    p.close()
+    return out
+
+
+@pytest.mark.parametrize('fastq', [False, True])
+@pytest.mark.parametrize('block,threads', [(200, 3), (4096, 5), (1 << 20, 16)])
+def test_bulk_batches_vs_record_path(tmp_path, monkeypatch, fastq, block, threads):
+    """Batches without names or qualities (copied a run of records at a time
+    from each parsed part) hold the same reads, cut at the same records, with
+    the same parsed / skipped counts after every batch, as batches with names
+    (copied record by record), and their concatenation is the oracle's."""
+    monkeypatch.setenv('GOETIA_AMD_FASTX_BLOCK', str(block))
+    monkeypatch.setenv('GOETIA_AMD_FASTX_THREADS', str(threads))
+    for k, maker in enumerate([_clean_file, lambda p, s, fq: _random_file(p, s, fq, False)]):
+        path = str(tmp_path / f'b{k}.{"fq" if fastq else "fa"}')
+        maker(path, 70 + k + block % 11, fastq)
+        for min_length in (0, 30):
+            exp = [s.encode() for s in ofx.parse(path, False, min_length)['sequences']]
+            for max_reads, max_bases in ((1, 1 << 20), (7, 2100), (13, 5000), (1 << 16, 1 << 24), (50, 2001)):
+                bulk = _batches(path, min_length, max_reads, max_bases, False)
+                rec = _batches(path, min_length, max_reads, max_bases, True)
+                assert bulk == rec, (k, min_length, max_reads, max_bases)
+                assert [s for b, _ in bulk for s in b] == exp
+
+
+def test_bulk_batches_threaded_copy(tmp_path, monkeypatch):
+    """A block large enough that its runs are copied on several threads."""
+    monkeypatch.setenv('GOETIA_AMD_FASTX_BLOCK', str(1 << 23))
+    monkeypatch.setenv('GOETIA_AMD_FASTX_THREADS', '8')
+    rng = np.random.default_rng(91)
+    path = str(tmp_path / 'big.fq')
+    lens = rng.integers(20, 300, 30000)
+    with open(path, 'w') as f:
+        for i, n in enumerate(lens):
+            s = ''.join(rng.choice(list('ACGTN' if i % 97 == 0 else 'ACGT'), n))
+            f.write(f'@r{i}\n{s}\n+\n{"I" * n}\n')
+    exp = [s.encode() for s in ofx.parse(path, False, 40)['sequences']]
+    for max_reads, max_bases in ((1 << 16, 1 << 24), (5000, 1 << 24), (1 << 16, 1 << 20)):
+        bulk = _batches(path, 40, max_reads, max_bases, False)
+        assert bulk == _batches(path, 40, max_reads, max_bases, True)
+        assert [s for b, _ in bulk for s in b] == exp

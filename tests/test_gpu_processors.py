@@ -182,3 +182,95 @@ def test_async_driver_over_samples(oracle):
                 o.insert_sequence(s, K, True)
     for a, b in zip(g.get_raw(), o.tables()):
         np.testing.assert_array_equal(a, b)
+
+
+def _oracle_after(oracle, sizes, K, seqs):
+    o = oracle.Storage(oracle.BYTE, sizes)
+    for s in seqs:
+        if len(s) >= K:
+            o.insert_sequence(s, K, True)
+    return o
+
+
+def _assert_storage(g, o):
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+
+
+@pytest.mark.parametrize('batch_bases', [0, 4096])
+def test_storage_sees_taken_reads_at_every_interval(oracle, batch_bases):
+    """advance() returns with the reads it took held (queued with the next
+    batch); every call on the storage queues them first, so counters, tables
+    and queries right after each return equal inserting exactly the reads
+    taken so far, as goetia's advance leaves them."""
+    path = os.path.join(DATA, 'test-fastq-reads.fq')
+    K = 21
+    seqs = ofx.parse(path)['sequences']
+    sizes = oracle.primes_near_x(3, 60_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    proc = lg.InserterProcessor(g, 300, batch_bases=batch_bases)
+    parser = lg.FastxParser.build(path)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    done, steps = 0, 0
+    while True:
+        n, _, rem = proc.advance(parser)
+        for s in seqs[done:n]:
+            if len(s) >= K:
+                o.insert_sequence(s, K, True)
+        done = n
+        if steps % 3 == 0:
+            _assert_storage(g, o)
+        else:
+            probe = seqs[max(0, n - 1)]
+            if len(probe) >= K:
+                np.testing.assert_array_equal(np.array(g.query_sequence(probe)), o.query_sequence(probe, K, True)[0])
+        steps += 1
+        if not rem:
+            break
+    assert steps > 8
+    _assert_storage(g, o)
+
+
+def test_two_inserters_on_one_storage(oracle):
+    """Two inserters advancing in turn over one storage: each advance's reads
+    go in after those of every earlier advance, whichever inserter made it."""
+    K = 21
+    paths = [os.path.join(DATA, 'test-fastq-reads.fq'), os.path.join(DATA, 'fastx', 'random_wrapped.fa')]
+    seqs = [ofx.parse(p)['sequences'] for p in paths]
+    sizes = oracle.primes_near_x(3, 60_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    procs = [lg.InserterProcessor(g, 700), lg.InserterProcessor(g, 1300, batch_bases=8192)]
+    parsers = [lg.FastxParser.build(p) for p in paths]
+    o = oracle.Storage(oracle.BYTE, sizes)
+    done, live = [0, 0], [True, True]
+    turn = 0
+    while any(live):
+        i = turn % 2
+        turn += 1
+        if not live[i]:
+            continue
+        n, _, rem = procs[i].advance(parsers[i])
+        for s in seqs[i][done[i]:n]:
+            if len(s) >= K:
+                o.insert_sequence(s, K, True)
+        done[i] = n
+        live[i] = rem
+        if turn % 4 == 0:
+            assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    _assert_storage(g, o)
+
+
+def test_freed_inserter_leaves_its_held_reads(oracle):
+    """Reads an inserter holds when it is freed go into the storage."""
+    path = os.path.join(DATA, 'test-fastq-reads.fq')
+    K = 21
+    seqs = ofx.parse(path)['sequences']
+    sizes = oracle.primes_near_x(3, 60_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    proc = lg.InserterProcessor(g, 2000)
+    parser = lg.FastxParser.build(path)
+    n, _, rem = proc.advance(parser)
+    assert rem and 0 < n < len(seqs)
+    del proc
+    _assert_storage(g, _oracle_after(oracle, sizes, K, seqs[:n]))
