@@ -74,7 +74,7 @@ def parse():
                    help='cms_20m is the headline; the others print their own JSON line')
     p.add_argument('--plain-chunk-kmers', type=int, default=500_000,
                    help='chunk size of the plain path in k-mers (goetia IntervalCounter default)')
-    p.add_argument('--serving-chunks', type=int, default=120,
+    p.add_argument('--serving-chunks', type=int, default=400,
                    help='chunks of the served-vs-plain comparison (64 streams of BitStorage(2^30, 4)); 0 skips it')
     return p.parse_args()
 

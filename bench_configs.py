@@ -291,9 +291,9 @@ def fastx(args):
     out['value'] = out['large_interval']['kmers_per_s']
     out['unit'] = 'kmers/s'
     out['note'] = ('plain FASTQ read by the block-parallel parser (host threads) into pinned double buffers of '
-                   '256 MiB; the next batch is parsed while the GPU inserts the current one (PCIe included); '
-                   'advance() returns at every interval once its inserts are queued, and the storage\'s later '
-                   'calls wait for them; the clock stops after the last insert has finished')
+                   '128 MiB; the next batch is parsed while the GPU inserts the current one (PCIe included); '
+                   'advance() returns at every interval with the reads it took held for the next batch (any call '
+                   'on the storage inserts them first); the clock stops after the last insert has finished')
     os.remove(path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)

@@ -218,7 +218,7 @@ struct Scratch {
     // binned insert: events of the two partition passes, first-pass digit
     // offsets, second-pass counts / starts and their scan sums, new-window
     // flags and the sub-batch's first packed window index
-    DevBuf bin_ent1, bin_ent2, bin_off1, bin_cnt2, bin_sums, bin_flags, bin_kbase, bin_vq;
+    DevBuf bin_ent1, bin_ent2, bin_dig2, bin_off1, bin_cnt2, bin_sums, bin_flags, bin_kbase, bin_vq;
     bool fixed = false;                // buffers preallocated and never moved (captured graphs)
 };
 
@@ -650,15 +650,17 @@ uint32_t bin_buckets(const gt_storage* st) {
 
 // Whether an insert goes through the binned path.  GOETIA_AMD_BINNED=0 / 1
 // forces it off / on (where the tables allow it); by default it is used for
-// batches of at least 2^20 bases with at least one base per 8 table bins,
-// where replaying buckets in LDS beats random atomics on the tables.
+// batches of at least 2^20 bases with at least one base per 256 table bins,
+// where replaying buckets in LDS beats random atomics on the tables (on
+// 4 x 2^28 count-min bins, 4.5M-base batches insert 3.7e9 k-mers/s binned
+// against 2.2e9 with atomics, 1.2M-base batches 1.4e9 against 1.8e9).
 bool use_binned(const gt_storage* st, const Scratch& sc, const Reads& R) {
     if (R.dyn || sc.fixed || !bin_buckets(st)) return false;
     const char* v = std::getenv("GOETIA_AMD_BINNED");
     if (v && *v) return v[0] != '0';
     uint64_t bins = 0;
     for (uint64_t p : st->sizes) bins += p;
-    return (uint64_t)R.n_bases >= (1ULL << 20) && (uint64_t)R.n_bases * 8 >= bins;
+    return (uint64_t)R.n_bases >= (1ULL << 20) && (uint64_t)R.n_bases * 256 >= bins;
 }
 
 // Binned sub-batch in start positions, a multiple of kTileMax: 2^27 for an
@@ -700,6 +702,7 @@ int bin_prepare(gt_storage* st, Scratch& sc, uint64_t tiles_max, BinPlan& P) {
     const uint64_t cnt_max = ((uint64_t)P.n_d1 << P.shift2) * groups_max;
     int rc = sc.bin_ent1.ensure(n_ent * sizeof(uint64_t));
     if (rc == GT_OK) rc = sc.bin_ent2.ensure(n_ent * sizeof(uint64_t));
+    if (rc == GT_OK) rc = sc.bin_dig2.ensure(n_ent * sizeof(uint16_t));
     if (rc == GT_OK) rc = sc.bin_off1.ensure((kDigits1 + 1) * tiles_max * sizeof(uint16_t));
     if (rc == GT_OK) rc = sc.bin_cnt2.ensure((cnt_max + 1) * sizeof(uint32_t));
     if (rc == GT_OK) rc = sc.bin_sums.ensure((scan_u32_block_count(cnt_max) + 1) * sizeof(uint32_t));
@@ -707,6 +710,7 @@ int bin_prepare(gt_storage* st, Scratch& sc, uint64_t tiles_max, BinPlan& P) {
     if (rc != GT_OK) return rc;
     P.ent1 = sc.bin_ent1.as<uint64_t>();
     P.ent2 = sc.bin_ent2.as<uint64_t>();
+    P.dig2 = sc.bin_dig2.as<uint16_t>();
     P.off1 = sc.bin_off1.as<uint16_t>();
     P.cnt2 = sc.bin_cnt2.as<uint32_t>();
     P.kbase = sc.bin_kbase.as<uint64_t>();
@@ -768,6 +772,7 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     }
     P.flags = flags;
     P.vq = iq ? sc.bin_vq.as<uint8_t>() : nullptr;
+    P.unordered = iq ? 0u : 1u;        // a plain insert's replay does not depend on event order
 
     const Tables T = st->tables_view();
     uint64_t* ctr = st->counters.as<uint64_t>();
@@ -807,6 +812,7 @@ int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8
         // insert with atomics on the tables instead (same results)
         sc.bin_ent1.release();
         sc.bin_ent2.release();
+        sc.bin_dig2.release();
         sc.bin_flags.release();
         sc.bin_vq.release();
     }
@@ -1483,6 +1489,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
         if (rc != GT_ERR_OUT_OF_MEMORY) return rc;
         st->own.bin_ent1.release();     // no room for the event buffers: strict order below
         st->own.bin_ent2.release();
+        st->own.bin_dig2.release();
         st->own.bin_vq.release();
     }
     if (st->kind == kByte) {

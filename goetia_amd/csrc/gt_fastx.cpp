@@ -923,7 +923,7 @@ int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     ins->canonical = canonical;
     ins->mode = mode;
     ins->interval = interval;
-    ins->max_bases = batch_bases ? batch_bases : (256ULL << 20);
+    ins->max_bases = batch_bases ? batch_bases : (128ULL << 20);
     ins->max_reads = std::max<uint64_t>(ins->max_bases / 64, 1024);
     hipError_t e = hipStreamCreate(&ins->stream);
     if (e == hipSuccess) e = hipStreamCreate(&ins->copy);

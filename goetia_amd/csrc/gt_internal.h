@@ -108,6 +108,7 @@ struct BinPlan {
     uint32_t tiles1;                      // first-pass tiles of the sub-batch
     uint32_t n_groups;                    // ceil(tiles1 / kBinGroup)
     uint64_t* ent1;                       // per tile a region of n_tables * kBinTile events, by d1
+    uint16_t* dig2;                       // second digit of each ent1 event (same index)
     uint64_t* ent2;                       // events by bucket, event order within a bucket
     uint16_t* off1;                       // (kDigits1 + 1) x tiles1, digit-major: start of d1 in its tile
     uint32_t* cnt2;                       // (n_d1 << shift2) x n_groups (+1): counts, then exclusive starts
@@ -122,6 +123,8 @@ struct BinPlan {
     uint32_t r_end;
     uint8_t* hasev;                       // optional, per window of the sub-batch: the first pass
                                           // writes 1 for windows with events, 0 for skipped ones
+    uint32_t unordered;                   // 1: a bucket's events may be in any order (plain
+                                          // inserts); 0: window order (insert_and_query, diginorm)
 };
 
 // One call's view of the reads.  Starts s in [b0, b1) are processed; the

@@ -208,7 +208,7 @@ struct TileSharedT {
     uint64_t tf[256];                 // T[c]
     uint64_t tr[256];                 // T[complement(c)]
     int64_t roff[kReadCache + 1];
-    uint8_t bytes[TILE + kMaxK];
+    alignas(8) uint8_t bytes[TILE + kMaxK];
     uint32_t r_lo, r_hi;
     int cached;
 };
@@ -245,6 +245,66 @@ __device__ __forceinline__ auto call_end_tile(Op& op, int64_t tile, int) -> decl
 }
 template <class Op>
 __device__ __forceinline__ void call_end_tile(Op&, int64_t, long) {}
+
+// Window starts stitched from segments.  Segment u is bytes [u*PER, u*PER+PER)
+// of the tile.  With m = ceil(K / PER) and L = K - (m-1)*PER, the window at
+// u*PER covers segments u .. u+m-2 whole and the first L bytes of u+m-1, and
+// the cyclic hash is linear over XOR, so
+//   fw = XOR_j<m-1 rotl(pf[u+j], K - (j+1)*PER)  ^  qf[u+m-1]
+//   rc = XOR_j<m-1 rotl(pr[u+j], j*PER)          ^  rotl(qr[u+m-1], (m-1)*PER)
+// with pf / qf the forward hash of a whole segment / of its first L bytes
+// (CyclicHash::hash over them) and pr / qr the reverse-complement terms
+// (XOR_i rotl(T[comp(b_i)], i) over them).  Every thread computes its own
+// segment (the first m-1 threads also the segments past the tile), the
+// values meet in LDS, and each thread combines m of them instead of rolling
+// K bytes twice.  kSegExtra bounds m - 1 (K <= 255, PER >= 8).
+constexpr int kSegExtra = 32;
+struct SegScratch {
+    uint64_t pf[kBlock + kSegExtra], qf[kBlock + kSegExtra];
+    uint64_t pr[kBlock + kSegExtra], qr[kBlock + kSegExtra];
+    uint8_t bf[kBlock + kSegExtra], bq[kBlock + kSegExtra];   // non-ACGT bytes in the segment / its prefix
+};
+
+// The op's LDS for stitched starts (free while the windows are visited), or
+// null for ops without it (those roll K bytes per start).
+template <class Op>
+__device__ __forceinline__ auto seg_scratch(Op& op, int) -> decltype(op.seg_scratch()) {
+    return op.seg_scratch();
+}
+template <class Op>
+__device__ __forceinline__ SegScratch* seg_scratch(Op&, long) {
+    return nullptr;
+}
+
+template <int PER, class Sh>
+__device__ __forceinline__ void seg_values(const Sh& sh, SegScratch* sg, int u, int L) {
+    static_assert(PER % 8 == 0, "segments are read 8 bytes at a time");
+    uint64_t pf = 0, pr = 0, qf = 0, qr = 0;
+    uint32_t bf = 0, bq = 0;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += 8) {
+        const uint64_t w = *reinterpret_cast<const uint64_t*>(sh.bytes + u * PER + i0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = i0 + k;
+            const uint32_t c = (uint32_t)(w >> (8 * k)) & 0xFFu;
+            pf = gt_rotl(pf, 1) ^ sh.tf[c];
+            pr ^= gt_rotl(sh.tr[c], (unsigned)i);
+            bf += (uint32_t)!gt_is_acgt((unsigned char)c);
+            if (i == L - 1) {
+                qf = pf;
+                qr = pr;
+                bq = bf;
+            }
+        }
+    }
+    sg->pf[u] = pf;
+    sg->qf[u] = qf;
+    sg->pr[u] = pr;
+    sg->qr[u] = qr;
+    sg->bf[u] = (uint8_t)bf;
+    sg->bq[u] = (uint8_t)bq;
+}
 
 template <int PER, class Op, class Sh>
 __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
@@ -298,6 +358,14 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
         __syncthreads();
 
         op.begin_tile(t0);
+        SegScratch* sg = seg_scratch(op, 0);
+        const int n_seg = (K + PER - 1) / PER;
+        if (sg) {
+            const int L = K - (n_seg - 1) * PER;
+            seg_values<PER>(sh, sg, tid, L);
+            if (tid < n_seg - 1) seg_values<PER>(sh, sg, kBlock + tid, L);
+            __syncthreads();
+        }
         const int64_t s0 = t0 + (int64_t)tid * PER;
         int64_t s_end = s0 + PER;
         if (s_end > b1) s_end = b1;
@@ -330,12 +398,23 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
             const int li = (int)(s0 - t0);
             uint64_t fw = 0, rc = 0;
             int bad = 0;
-            for (int k = 0; k < K; ++k) {
-                const uint8_t c = B[li + k];
-                fw = gt_rotl(fw, 1) ^ sh.tf[c];
-                bad += !gt_is_acgt(c);
+            if (sg) {
+                for (int j = 0; j + 1 < n_seg; ++j) {
+                    fw ^= gt_rotl(sg->pf[tid + j], (unsigned)(K - (j + 1) * PER));
+                    rc ^= gt_rotl(sg->pr[tid + j], (unsigned)(j * PER));
+                    bad += sg->bf[tid + j];
+                }
+                fw ^= sg->qf[tid + n_seg - 1];
+                rc ^= gt_rotl(sg->qr[tid + n_seg - 1], (unsigned)((n_seg - 1) * PER));
+                bad += sg->bq[tid + n_seg - 1];
+            } else {
+                for (int k = 0; k < K; ++k) {
+                    const uint8_t c = B[li + k];
+                    fw = gt_rotl(fw, 1) ^ sh.tf[c];
+                    bad += !gt_is_acgt(c);
+                }
+                for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
             }
-            for (int k = K - 1; k >= 0; --k) rc = gt_rotl(rc, 1) ^ sh.tr[B[li + k]];
 
             for (int j = 0; j < PER; ++j) {
                 const int64_t s = s0 + j;
@@ -1185,7 +1264,10 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
 struct BinL1Shared {
     uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
     uint32_t ord[kBinTile];
-    uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
+    union {
+        uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
+        SegScratch seg;                           // stitched window starts (before end_tile)
+    };
     uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
     uint32_t part[kBlock];
 };
@@ -1196,6 +1278,11 @@ __device__ __forceinline__ uint32_t bin_d1(uint32_t g, uint32_t shift2) {
     return g == kNoBin ? (uint32_t)kDigits1 : (g >> kBinBits) >> shift2;
 }
 
+// First pass over one tile.  kU: the events of a digit may leave the tile in
+// any order (plain inserts, whose replay depends only on each bin's event
+// count and first window); otherwise they keep window order (insert_and_query
+// and diginorm replay events in order).
+template <bool kU>
 struct BinL1Op {
     Tables T;
     BinPlan P;
@@ -1214,6 +1301,7 @@ struct BinL1Op {
         const int w0 = threadIdx.x * kBinPer;
         for (uint32_t t = 0; t < P.n_tables; ++t)
             for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
+        if (kU) sh->wpos[0][threadIdx.x] = 0;      // digit counters (kDigits1 == kBlock)
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
@@ -1235,10 +1323,81 @@ struct BinL1Op {
 
     __device__ void flush() {}
 
+    __device__ SegScratch* seg_scratch() { return &sh->seg; }
+
+    // Sort of the tile's events by first digit, built in LDS and written out
+    // as one contiguous region.  Order-free: an LDS counter per digit gives
+    // each event its slot.
+    __device__ void end_tile_unordered(int64_t tile) {
+        __syncthreads();
+        constexpr int kQ = (int)(kBinMaxTables * kBinTile / kBlock);
+        const uint32_t ne = P.n_tables * kBinTile;
+        uint32_t* cnt = sh->wpos[0];
+        uint32_t di[kQ];                                   // digit << 16 | slot in the digit, or ~0
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const uint32_t e = (uint32_t)q * kBlock + threadIdx.x;
+            di[q] = 0xFFFFFFFFu;
+            if (e < ne) {
+                const uint32_t d = bin_d1(sh->g[e], P.shift2);
+                if (d != (uint32_t)kDigits1) di[q] = (d << 16) | atomicAdd(&cnt[d], 1u);
+            }
+        }
+        __syncthreads();
+        const uint32_t lt1 = (uint32_t)(tile - P.tile0);
+        const int d = threadIdx.x;                         // kDigits1 == kBlock
+        uint32_t n_ev;
+        const uint32_t start = block_exclusive_u32(cnt[d], sh->part, &n_ev);
+        P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
+        if (d == kBlock - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
+        cnt[d] = start;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kQ; ++q)
+            if (di[q] != 0xFFFFFFFFu)
+                sh->perm[cnt[di[q] >> 16] + (di[q] & 0xFFFFu)] = (uint16_t)((uint32_t)q * kBlock + threadIdx.x);
+        __syncthreads();
+        write_sorted(lt1, n_ev);
+    }
+
+    // Writes the tile's events in perm order to its region of ent1, and their
+    // second digits to the same positions of dig2 (the second-pass count
+    // reads only those).
+    __device__ void write_sorted(uint32_t lt1, uint32_t n_ev) {
+        const size_t r0 = (size_t)lt1 * P.n_tables * kBinTile;
+        uint64_t* dst = P.ent1 + r0;
+        uint16_t* dst2 = P.dig2 + r0;
+        const uint32_t mask2 = (1u << P.shift2) - 1u;
+        for (uint32_t i0 = 0; i0 < n_ev; i0 += 4 * kBlock) {
+            uint32_t e[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                e[q] = i < n_ev ? sh->perm[i] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                if (i < n_ev) {
+                    const uint32_t g = sh->g[e[q]];
+                    dst[i] = ((uint64_t)g << 32) | sh->ord[e[q] & (kBinTile - 1)];
+                    dst2[i] = (uint16_t)((g >> kBinBits) & mask2);
+                }
+            }
+        }
+    }
+
+    __device__ void end_tile(int64_t tile) {
+        if constexpr (kU)
+            end_tile_unordered(tile);
+        else
+            end_tile_stable(tile);
+    }
+
     // Stable sort of the tile's events by first digit: each wave ranks a
     // quarter of the events, the sorted order is built in LDS and the region
     // is written out contiguously.
-    __device__ void end_tile(int64_t tile) {
+    __device__ void end_tile_stable(int64_t tile) {
         __syncthreads();
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const uint64_t lt = lanes_below();
@@ -1282,27 +1441,15 @@ struct BinL1Op {
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        uint64_t* dst = P.ent1 + (size_t)lt1 * P.n_tables * kBinTile;
-        for (uint32_t i0 = 0; i0 < n_ev; i0 += 4 * kBlock) {
-            uint32_t e[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t i = i0 + q * kBlock + threadIdx.x;
-                e[q] = i < n_ev ? sh->perm[i] : 0u;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t i = i0 + q * kBlock + threadIdx.x;
-                if (i < n_ev) dst[i] = ((uint64_t)sh->g[e[q]] << 32) | sh->ord[e[q] & (kBinTile - 1)];
-            }
-        }
+        write_sorted(lt1, n_ev);
     }
 };
 
+template <bool kU>
 __global__ __launch_bounds__(kBlock) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
     __shared__ TileSharedT<kBinTile> sh;
     __shared__ BinL1Shared bs;
-    BinL1Op op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
+    BinL1Op<kU> op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
     walk_tiles<kBinPer>(R, sh, op);
 }
 
@@ -1380,40 +1527,66 @@ __device__ __forceinline__ void bin_chunk_load(const BinPlan& P, const BinRuns& 
     }
 }
 
-// f(i - c0, event i) for the events of a chunk, a lane per event.
-template <class F>
-__device__ __forceinline__ void bin_for_events(const BinPlan& P, const BinRuns& sh, uint32_t* first, uint32_t c0,
-                                               uint32_t m, F&& f) {
-    bin_chunk_first(sh, first, c0, m);
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kBlock) {
-        const uint32_t i = c0 + j;
-        uint32_t k = first[j >> 6];
-        while (sh.rpre[k + 1] <= i) ++k;
-        f(j, P.ent1[sh.rstart[k] + (i - sh.rpre[k])]);
-    }
-}
-
 __device__ __forceinline__ uint32_t bin_d2(uint64_t e, uint32_t mask2) {
     return ((uint32_t)(e >> 32) >> kBinBits) & mask2;
 }
 
+// Second digits of chunk events [c0, c0 + m) into d (kNoDigit past m): the
+// loads are issued together and stay in flight until d is read.
+constexpr uint32_t kNoDigit = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void bin_chunk_load_d2(const BinPlan& P, const BinRuns& sh, const uint32_t* first,
+                                                  uint32_t c0, uint32_t m, uint32_t* d) {
+#pragma unroll
+    for (int q = 0; q < kBinChunk / kBlock; ++q) {
+        const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+        d[q] = kNoDigit;
+        if (j < m) {
+            const uint32_t i = c0 + j;
+            uint32_t k = first[j >> 6];
+            while (sh.rpre[k + 1] <= i) ++k;
+            d[q] = P.dig2[sh.rstart[k] + (i - sh.rpre[k])];
+        }
+    }
+}
+
+// For each (first digit, group) item: events per second digit, from the
+// first pass's digit array; a chunk's loads are issued two chunks ahead.
 __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
     __shared__ BinRuns sh;
     __shared__ uint32_t hist[kDigits2];
     __shared__ uint32_t first[kBinChunk / 64];
+    constexpr int kQ = kBinChunk / kBlock;
     const uint32_t n_items = P.n_d1 * P.n_groups;
     const uint32_t mask2 = (1u << P.shift2) - 1u;
+    uint32_t da[kQ], db[kQ];
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
         __syncthreads();
         for (int d = threadIdx.x; d < kDigits2; d += kBlock) hist[d] = 0;
         const uint32_t n = bin_load_runs(P, d1, grp, sh);
-        for (uint32_t c0 = 0; c0 < n; c0 += kBinChunk) {
-            const uint32_t m = n - c0 < (uint32_t)kBinChunk ? n - c0 : (uint32_t)kBinChunk;
-            bin_for_events(P, sh, first, c0, m, [&](uint32_t, uint64_t e) { atomicAdd(&hist[bin_d2(e, mask2)], 1u); });
+        auto chunk_len = [&](uint32_t c) {
+            return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
+        };
+        auto prefetch = [&](uint32_t c, uint32_t (&d)[kQ]) {
+            bin_chunk_first(sh, first, c, chunk_len(c));
             __syncthreads();
+            bin_chunk_load_d2(P, sh, first, c, chunk_len(c), d);
+            __syncthreads();                               // `first` is free again
+        };
+        prefetch(0, da);
+        prefetch(kBinChunk, db);
+        auto step = [&](uint32_t c0, uint32_t (&d)[kQ]) {
+#pragma unroll
+            for (int q = 0; q < kQ; ++q)
+                if (d[q] != kNoDigit) atomicAdd(&hist[d[q]], 1u);
+            prefetch(c0 + 2 * kBinChunk, d);
+        };
+        for (uint32_t c0 = 0; c0 < n; c0 += 2 * kBinChunk) {
+            step(c0, da);
+            if (c0 + kBinChunk < n) step(c0 + kBinChunk, db);
         }
+        __syncthreads();
         for (uint32_t d2 = threadIdx.x; d2 <= mask2; d2 += kBlock)
             P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] = hist[d2];
     }
@@ -1515,6 +1688,92 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
             __syncthreads();
             for (uint32_t dd = threadIdx.x; dd < (uint32_t)kDigits2; dd += kBlock)
                 sh.cur[dd] += sh.lstart[dd + 1] - sh.lstart[dd];
+        }
+    }
+}
+
+struct BinL2SharedU {
+    BinRuns runs;
+    uint64_t sorted[kBinChunk];             // the chunk's events by second digit
+    uint32_t cnt[kDigits2];                 // chunk-local events per digit, then their starts
+    uint32_t lstart[kDigits2 + 1];          // chunk-local start of each second digit
+    uint32_t cur[kDigits2];                 // next ent2 position of each second digit for this item
+    uint32_t first[kBinChunk / 64];
+};
+
+// Scatter of each item by second digit for order-free replays, kBinChunk
+// events at a time: an LDS counter per digit gives each event of the chunk
+// its slot in the sorted chunk, which is written out as one run per digit.
+// The events of a chunk are loaded two chunks ahead, into two register sets
+// used in turn.
+__global__ __launch_bounds__(kBlock) void bin_l2_scatter_u_kernel(BinPlan P) {
+    __shared__ BinL2SharedU sh;
+    constexpr int kQ = kBinChunk / kBlock;
+    const uint32_t n_items = P.n_d1 * P.n_groups;
+    const uint32_t mask2 = (1u << P.shift2) - 1u;
+    uint64_t va[kQ], vb[kQ];
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
+        __syncthreads();
+        for (uint32_t d2 = threadIdx.x; d2 < (uint32_t)kDigits2; d2 += kBlock)
+            sh.cur[d2] = d2 <= mask2 ? P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] : 0u;
+        const uint32_t n = bin_load_runs(P, d1, grp, sh.runs);
+        auto chunk_len = [&](uint32_t c) {
+            return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
+        };
+        // issues the loads of chunk c into v (every thread has finished reading `first`)
+        auto prefetch = [&](uint32_t c, uint64_t (&v)[kQ]) {
+            bin_chunk_first(sh.runs, sh.first, c, chunk_len(c));
+            __syncthreads();
+            bin_chunk_load(P, sh.runs, sh.first, c, chunk_len(c), v);
+        };
+        prefetch(0, va);
+        __syncthreads();
+        prefetch(kBinChunk, vb);
+        auto step = [&](uint32_t c0, uint64_t (&v)[kQ]) {
+            const uint32_t m = chunk_len(c0);
+            for (uint32_t d = threadIdx.x; d < (uint32_t)kDigits2; d += kBlock) sh.cnt[d] = 0;
+            __syncthreads();
+            uint32_t di[kQ];                                 // digit << 16 | slot in the digit, or ~0
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+                di[q] = 0xFFFFFFFFu;
+                if (j < m) {
+                    const uint32_t d = bin_d2(v[q], mask2);
+                    di[q] = (d << 16) | atomicAdd(&sh.cnt[d], 1u);
+                }
+            }
+            __syncthreads();
+            // chunk-local digit starts (two digits per thread)
+            uint32_t a[2], tot = 0;
+            for (int k = 0; k < 2; ++k) {
+                a[k] = sh.cnt[2 * threadIdx.x + k];
+                tot += a[k];
+            }
+            uint32_t run = block_exclusive_u32(tot, sh.runs.part);
+            for (int k = 0; k < 2; ++k) {
+                sh.lstart[2 * threadIdx.x + k] = run;
+                run += a[k];
+            }
+            if (threadIdx.x == kBlock - 1) sh.lstart[kDigits2] = run;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kQ; ++q)
+                if (di[q] != 0xFFFFFFFFu) sh.sorted[sh.lstart[di[q] >> 16] + (di[q] & 0xFFFFu)] = v[q];
+            prefetch(c0 + 2 * kBinChunk, v);                 // stays in flight for two chunks
+            for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
+                const uint64_t e = sh.sorted[i];
+                const uint32_t d = bin_d2(e, mask2);
+                P.ent2[sh.cur[d] + (i - sh.lstart[d])] = e;
+            }
+            __syncthreads();
+            for (uint32_t dd = threadIdx.x; dd < (uint32_t)kDigits2; dd += kBlock)
+                sh.cur[dd] += sh.lstart[dd + 1] - sh.lstart[dd];
+        };
+        for (uint32_t c0 = 0; c0 < n; c0 += 2 * kBinChunk) {
+            step(c0, va);
+            if (c0 + kBinChunk < n) step(c0 + kBinChunk, vb);
         }
     }
 }
@@ -1680,6 +1939,134 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
     __shared__ uint32_t slice[4][kBucketBins / 4];
     __shared__ uint64_t slots[4][256];
     bin_apply_body<false>(P, T, counters, slice, slots);
+}
+
+// Order-free replay of plain inserts (bitstorage.hh:195-219, bytestorage.cc:60-90
+// in batch order), one workgroup per bucket.  Within a batch a bin's final
+// value depends only on how many events hit it (count-min: min(255, before +
+// hits); Bloom: set when hit), and a k-mer is new through a bin exactly when
+// the bin was empty before the batch and the k-mer is the bin's first event,
+// i.e. its smallest window ordinal (a bucket belongs to one table, so its
+// events have distinct ordinals).  So the events only need, per bin, a hit
+// count (count-min) and the smallest ordinal; the event holding that ordinal
+// then settles the bin and resets both for the next bucket.  The next
+// bucket's slice and first events are loaded while this one is replayed.
+template <bool kBytes>
+struct ApplyNext {
+    static constexpr int kQ = 16;                                         // events per thread in registers
+    static constexpr int kW = (kBytes ? kBucketBins / 4 : kBucketBins / 32) / kBlock;   // slice words per thread
+    uint32_t b, lo, hi, t, nw;
+    uint32_t* src;
+    uint32_t w[kW];
+    uint64_t ev[kQ];
+};
+
+// The first bucket at or after b (stepping by `step`) with events, or n_buckets.
+__device__ __forceinline__ uint32_t next_busy_bucket(const BinPlan& P, uint32_t b, uint32_t step) {
+    for (; b < P.n_buckets; b += step)
+        if (P.cnt2[(size_t)b * P.n_groups] != P.cnt2[(size_t)(b + 1) * P.n_groups]) break;
+    return b < P.n_buckets ? b : P.n_buckets;
+}
+
+template <bool kBytes>
+__device__ __forceinline__ void apply_fetch(const BinPlan& P, const Tables& T, uint32_t b, ApplyNext<kBytes>& x) {
+    x.b = b;
+    if (b >= P.n_buckets) return;
+    x.lo = P.cnt2[(size_t)b * P.n_groups];
+    x.hi = P.cnt2[(size_t)(b + 1) * P.n_groups];
+    uint32_t t = 0;
+    uint64_t pt = T.p[0];
+    uint8_t* base = T.base[0];
+#pragma unroll
+    for (uint32_t i = 1; i < kBinMaxTables; ++i) {
+        if (i < P.n_tables && b >= P.bfirst[i]) {
+            t = i;
+            pt = T.p[i];
+            base = T.base[i];
+        }
+    }
+    const uint64_t bin0 = (uint64_t)(b - P.bfirst[t]) << kBinBits;
+    const uint64_t left = pt - bin0;
+    const uint32_t nb = (uint32_t)(left < (uint64_t)kBucketBins ? left : (uint64_t)kBucketBins);
+    x.t = t;
+    x.nw = kBytes ? (nb + 3u) / 4u : (nb + 31u) / 32u;
+    x.src = reinterpret_cast<uint32_t*>(kBytes ? base + bin0 : base + bin0 / 8);
+#pragma unroll
+    for (int k = 0; k < ApplyNext<kBytes>::kW; ++k) {
+        const uint32_t w = (uint32_t)k * kBlock + threadIdx.x;
+        x.w[k] = w < x.nw ? x.src[w] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < ApplyNext<kBytes>::kQ; ++q) {
+        const uint32_t i = x.lo + (uint32_t)q * kBlock + threadIdx.x;
+        x.ev[q] = i < x.hi ? P.ent2[i] : 0;
+    }
+}
+
+template <bool kBytes>
+__global__ __launch_bounds__(kBlock) void bin_apply_u_kernel(BinPlan P, Tables T, uint64_t* counters) {
+    using Next = ApplyNext<kBytes>;
+    constexpr int kQ = Next::kQ;
+    __shared__ uint32_t slice[kBytes ? kBucketBins / 4 : kBucketBins / 32];
+    __shared__ uint32_t first[kBucketBins];               // smallest ordinal per bin, ~0 = none
+    __shared__ uint32_t hits[kBytes ? kBucketBins : 1];   // events per bin
+    const uint64_t kb = *P.kbase;
+    uint8_t* sb = reinterpret_cast<uint8_t*>(slice);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBucketBins; i += kBlock) {
+        first[i] = 0xFFFFFFFFu;
+        if (kBytes) hits[i] = 0;
+    }
+    uint64_t occ = 0;
+    Next cur, nxt;
+    apply_fetch<kBytes>(P, T, next_busy_bucket(P, blockIdx.x, gridDim.x), cur);
+    while (cur.b < P.n_buckets) {
+        apply_fetch<kBytes>(P, T, next_busy_bucket(P, cur.b + gridDim.x, gridDim.x), nxt);
+        __syncthreads();                                  // the previous bucket is written back
+#pragma unroll
+        for (int k = 0; k < Next::kW; ++k) {
+            const uint32_t w = (uint32_t)k * kBlock + threadIdx.x;
+            if (w < cur.nw) slice[w] = cur.w[k];
+        }
+        auto hit = [&](uint64_t e) {
+            const uint32_t key = (uint32_t)(e >> 32) & (kBucketBins - 1);
+            atomicMin(&first[key], (uint32_t)e);
+            if (kBytes) atomicAdd(&hits[key], 1u);
+        };
+#pragma unroll
+        for (int q = 0; q < kQ; ++q)
+            if (cur.lo + (uint32_t)q * kBlock + threadIdx.x < cur.hi) hit(cur.ev[q]);
+        for (uint32_t i = cur.lo + (uint32_t)kQ * kBlock + threadIdx.x; i < cur.hi; i += kBlock) hit(P.ent2[i]);
+        __syncthreads();
+        auto settle = [&](uint64_t e) {
+            const uint32_t key = (uint32_t)(e >> 32) & (kBucketBins - 1);
+            const uint32_t o = (uint32_t)e;
+            if (first[key] != o) return;                  // not the bin's first event
+            first[key] = 0xFFFFFFFFu;
+            bool fresh;
+            if (kBytes) {
+                const uint32_t h = atomicExch(&hits[key], 0u);
+                const uint32_t pre = sb[key];
+                const uint32_t post = pre + (h < 255u ? h : 255u);
+                sb[key] = (uint8_t)(post < 255u ? post : 255u);
+                fresh = pre == 0;
+            } else {
+                const uint32_t bit = 1u << (key & 31u);
+                fresh = (atomicOr(&slice[key >> 5], bit) & bit) == 0;
+            }
+            if (fresh) {
+                P.flags[kb + o] = 1;
+                occ += cur.t == 0;
+            }
+        };
+#pragma unroll
+        for (int q = 0; q < kQ; ++q)
+            if (cur.lo + (uint32_t)q * kBlock + threadIdx.x < cur.hi) settle(cur.ev[q]);
+        for (uint32_t i = cur.lo + (uint32_t)kQ * kBlock + threadIdx.x; i < cur.hi; i += kBlock) settle(P.ent2[i]);
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < cur.nw; w += kBlock) cur.src[w] = slice[w];
+        cur = nxt;
+    }
+    add_counter(counters + kNOccupied, occ);
 }
 
 __global__ __launch_bounds__(kBlock) void dn_apply_kernel(BinPlan P, Tables T, uint64_t* counters) {
@@ -2244,7 +2631,10 @@ void launch_bin_kbase(const Reads& R, int64_t b0, uint64_t* out, hipStream_t s) 
 
 void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, hipStream_t s) {
     const int g = (int)std::min<int64_t>(std::max<int64_t>(P.tiles1, 1), grid);
-    bin_l1_kernel<<<g, kBlock, 0, s>>>(R, T, P);
+    if (P.unordered)
+        bin_l1_kernel<true><<<g, kBlock, 0, s>>>(R, T, P);
+    else
+        bin_l1_kernel<false><<<g, kBlock, 0, s>>>(R, T, P);
 }
 
 void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
@@ -2254,10 +2644,21 @@ void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
 
 void launch_bin_l2_scatter(const BinPlan& P, int grid, hipStream_t s) {
     const int g = (int)std::min<int64_t>(std::max<int64_t>((int64_t)P.n_d1 * P.n_groups, 1), grid);
-    bin_l2_scatter_kernel<<<g, kBlock, 0, s>>>(P);
+    if (P.unordered)
+        bin_l2_scatter_u_kernel<<<g, kBlock, 0, s>>>(P);
+    else
+        bin_l2_scatter_kernel<<<g, kBlock, 0, s>>>(P);
 }
 
 void launch_bin_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int grid, hipStream_t s) {
+    if (P.unordered && !P.vq) {
+        const int g = (int)std::min<int64_t>(std::max<int64_t>(P.n_buckets, 1), grid);
+        if (T.kind == kByte)
+            bin_apply_u_kernel<true><<<g, kBlock, 0, s>>>(P, T, counters);
+        else
+            bin_apply_u_kernel<false><<<g, kBlock, 0, s>>>(P, T, counters);
+        return;
+    }
     const int g = (int)std::min<int64_t>(std::max<int64_t>((P.n_buckets + 3) / 4, 1), grid);
     bin_apply_kernel<<<g, kBlock, 0, s>>>(P, T, counters);
 }

@@ -512,13 +512,14 @@ int gt_fastx_close(gt_fastx* p);
  * gt_dbg_insert, and returns right after the read whose k-mers (len - K + 1,
  * 0 below K) bring the interval counter to `interval`, as FileProcessor::advance
  * does: *n_sequences and *time_total are the running totals and *remaining is
- * 0 once the parser is exhausted.  At an interval the call returns once the
- * inserts up to that read are queued on the inserter's stream: every later
- * call on the storage (any entry point, any stream) waits for them, so what
- * the storage shows is exactly the reads up to the interval; at the end of
- * the input every insert is finished.  Records with invalid characters are
+ * 0 once the parser is exhausted.  At an interval the call returns with the
+ * reads taken since the last batch held by the inserter: they are inserted
+ * with its next batch, or before any other call on the storage (every entry
+ * point inserts them first), so what the storage shows is exactly the reads
+ * up to the interval; at the end of the input every insert is finished.  The
+ * storage must outlive the inserter (gt_inserter_free inserts held reads).  Records with invalid characters are
  * skipped even from a strict parser (FileProcessor::handle_next,
- * processors.hh:149-171).  batch_bases sizes the host batch (0: 256 MiB). */
+ * processors.hh:149-171).  batch_bases sizes the host batch (0: 128 MiB). */
 typedef struct gt_inserter gt_inserter;
 int gt_inserter_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_t interval, uint64_t batch_bases,
                     gt_inserter** out);
