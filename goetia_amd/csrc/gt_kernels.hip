@@ -1551,44 +1551,56 @@ __device__ __forceinline__ void bin_chunk_load_d2(const BinPlan& P, const BinRun
 }
 
 // For each (first digit, group) item: events per second digit, from the
-// first pass's digit array; a chunk's loads are issued two chunks ahead.
+// first pass's digit array; a chunk's loads are issued two chunks ahead.  A
+// workgroup counts kCountGroups consecutive groups of one first digit, so
+// each second digit's counts for them (adjacent in cnt2) go out as one write.
+constexpr uint32_t kCountGroups = 8;
+
 __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
     __shared__ BinRuns sh;
-    __shared__ uint32_t hist[kDigits2];
+    __shared__ uint32_t hist[kCountGroups][kDigits2];
     __shared__ uint32_t first[kBinChunk / 64];
     constexpr int kQ = kBinChunk / kBlock;
-    const uint32_t n_items = P.n_d1 * P.n_groups;
+    const uint32_t g_blocks = (P.n_groups + kCountGroups - 1) / kCountGroups;
+    const uint32_t n_blocks = P.n_d1 * g_blocks;
     const uint32_t mask2 = (1u << P.shift2) - 1u;
     uint32_t da[kQ], db[kQ];
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
+    for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+        const uint32_t d1 = blk / g_blocks, grp0 = (blk % g_blocks) * kCountGroups;
+        const uint32_t ng = P.n_groups - grp0 < kCountGroups ? P.n_groups - grp0 : kCountGroups;
         __syncthreads();
-        for (int d = threadIdx.x; d < kDigits2; d += kBlock) hist[d] = 0;
-        const uint32_t n = bin_load_runs(P, d1, grp, sh);
-        auto chunk_len = [&](uint32_t c) {
-            return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
-        };
-        auto prefetch = [&](uint32_t c, uint32_t (&d)[kQ]) {
-            bin_chunk_first(sh, first, c, chunk_len(c));
-            __syncthreads();
-            bin_chunk_load_d2(P, sh, first, c, chunk_len(c), d);
-            __syncthreads();                               // `first` is free again
-        };
-        prefetch(0, da);
-        prefetch(kBinChunk, db);
-        auto step = [&](uint32_t c0, uint32_t (&d)[kQ]) {
+        for (uint32_t x = threadIdx.x; x < kCountGroups * (uint32_t)kDigits2; x += kBlock) (&hist[0][0])[x] = 0;
+        for (uint32_t gi = 0; gi < ng; ++gi) {
+            uint32_t* h = hist[gi];
+            const uint32_t n = bin_load_runs(P, d1, grp0 + gi, sh);
+            auto chunk_len = [&](uint32_t c) {
+                return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
+            };
+            auto prefetch = [&](uint32_t c, uint32_t (&d)[kQ]) {
+                bin_chunk_first(sh, first, c, chunk_len(c));
+                __syncthreads();
+                bin_chunk_load_d2(P, sh, first, c, chunk_len(c), d);
+                __syncthreads();                           // `first` is free again
+            };
+            prefetch(0, da);
+            prefetch(kBinChunk, db);
+            auto step = [&](uint32_t c0, uint32_t (&d)[kQ]) {
 #pragma unroll
-            for (int q = 0; q < kQ; ++q)
-                if (d[q] != kNoDigit) atomicAdd(&hist[d[q]], 1u);
-            prefetch(c0 + 2 * kBinChunk, d);
-        };
-        for (uint32_t c0 = 0; c0 < n; c0 += 2 * kBinChunk) {
-            step(c0, da);
-            if (c0 + kBinChunk < n) step(c0 + kBinChunk, db);
+                for (int q = 0; q < kQ; ++q)
+                    if (d[q] != kNoDigit) atomicAdd(&h[d[q]], 1u);
+                prefetch(c0 + 2 * kBinChunk, d);
+            };
+            for (uint32_t c0 = 0; c0 < n; c0 += 2 * kBinChunk) {
+                step(c0, da);
+                if (c0 + kBinChunk < n) step(c0 + kBinChunk, db);
+            }
+            __syncthreads();                               // bin_load_runs of the next group reuses sh
         }
-        __syncthreads();
-        for (uint32_t d2 = threadIdx.x; d2 <= mask2; d2 += kBlock)
-            P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] = hist[d2];
+        // lanes take (digit, group) pairs group-fastest: a digit's groups are adjacent words of cnt2
+        for (uint32_t x = threadIdx.x; x < kCountGroups * (mask2 + 1u); x += kBlock) {
+            const uint32_t d2 = x / kCountGroups, gi = x % kCountGroups;
+            if (gi < ng) P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp0 + gi] = hist[gi][d2];
+        }
     }
 }
 
@@ -2638,7 +2650,8 @@ void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, 
 }
 
 void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
-    const int g = (int)std::min<int64_t>(std::max<int64_t>((int64_t)P.n_d1 * P.n_groups, 1), grid);
+    const int64_t blocks = (int64_t)P.n_d1 * ((P.n_groups + kCountGroups - 1) / kCountGroups);
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), grid);
     bin_l2_count_kernel<<<g, kBlock, 0, s>>>(P);
 }
 
