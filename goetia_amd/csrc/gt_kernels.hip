@@ -1506,16 +1506,19 @@ __device__ __forceinline__ uint32_t bin_run_of(const BinRuns& sh, uint32_t i) {
 // Events [c0, c0 + m) of the loaded item, m <= kBinChunk, are found with one
 // binary search per 64 events (into `first`, by threads 0..63) and a forward
 // step from there per lane; lanes read consecutive events.
+template <int CH = kBinChunk>
 __device__ __forceinline__ void bin_chunk_first(const BinRuns& sh, uint32_t* first, uint32_t c0, uint32_t m) {
-    if (threadIdx.x < (unsigned)(kBinChunk / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
+    static_assert(CH / 64 <= kBlock, "one thread per 64 events of a chunk");
+    if (threadIdx.x < (unsigned)(CH / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
 }
 
 // Issues a thread's kBinChunk / kBlock loads of the chunk into v (first set
 // and visible); none is used here, so they stay in flight.
+template <int CH = kBinChunk>
 __device__ __forceinline__ void bin_chunk_load(const BinPlan& P, const BinRuns& sh, const uint32_t* first, uint32_t c0,
                                                uint32_t m, uint64_t* v) {
 #pragma unroll
-    for (int q = 0; q < kBinChunk / kBlock; ++q) {
+    for (int q = 0; q < CH / kBlock; ++q) {
         const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
         v[q] = 0;
         if (j < m) {
@@ -1704,26 +1707,29 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_kernel(BinPlan P) {
     }
 }
 
+// Events per chunk of the order-free scatter: 8192 (64 KB in LDS, two
+// workgroups per CU) puts about 16 events of a digit in each run it writes.
+constexpr int kScatChunk = 8192;
+
 struct BinL2SharedU {
     BinRuns runs;
-    uint64_t sorted[kBinChunk];             // the chunk's events by second digit
+    uint64_t sorted[kScatChunk];            // the chunk's events by second digit
     uint32_t cnt[kDigits2];                 // chunk-local events per digit, then their starts
     uint32_t lstart[kDigits2 + 1];          // chunk-local start of each second digit
     uint32_t cur[kDigits2];                 // next ent2 position of each second digit for this item
-    uint32_t first[kBinChunk / 64];
+    uint32_t first[kScatChunk / 64];
 };
 
-// Scatter of each item by second digit for order-free replays, kBinChunk
+// Scatter of each item by second digit for order-free replays, kScatChunk
 // events at a time: an LDS counter per digit gives each event of the chunk
 // its slot in the sorted chunk, which is written out as one run per digit.
-// The events of a chunk are loaded two chunks ahead, into two register sets
-// used in turn.
+// The next chunk's loads are issued as soon as this chunk is in LDS.
 __global__ __launch_bounds__(kBlock) void bin_l2_scatter_u_kernel(BinPlan P) {
     __shared__ BinL2SharedU sh;
-    constexpr int kQ = kBinChunk / kBlock;
+    constexpr int kQ = kScatChunk / kBlock;
     const uint32_t n_items = P.n_d1 * P.n_groups;
     const uint32_t mask2 = (1u << P.shift2) - 1u;
-    uint64_t va[kQ], vb[kQ];
+    uint64_t v[kQ];
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t d1 = item / P.n_groups, grp = item % P.n_groups;
         __syncthreads();
@@ -1731,18 +1737,12 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_u_kernel(BinPlan P) {
             sh.cur[d2] = d2 <= mask2 ? P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp] : 0u;
         const uint32_t n = bin_load_runs(P, d1, grp, sh.runs);
         auto chunk_len = [&](uint32_t c) {
-            return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
+            return c < n ? (n - c < (uint32_t)kScatChunk ? n - c : (uint32_t)kScatChunk) : 0u;
         };
-        // issues the loads of chunk c into v (every thread has finished reading `first`)
-        auto prefetch = [&](uint32_t c, uint64_t (&v)[kQ]) {
-            bin_chunk_first(sh.runs, sh.first, c, chunk_len(c));
-            __syncthreads();
-            bin_chunk_load(P, sh.runs, sh.first, c, chunk_len(c), v);
-        };
-        prefetch(0, va);
+        bin_chunk_first<kScatChunk>(sh.runs, sh.first, 0, chunk_len(0));
         __syncthreads();
-        prefetch(kBinChunk, vb);
-        auto step = [&](uint32_t c0, uint64_t (&v)[kQ]) {
+        bin_chunk_load<kScatChunk>(P, sh.runs, sh.first, 0, chunk_len(0), v);
+        for (uint32_t c0 = 0; c0 < n; c0 += kScatChunk) {
             const uint32_t m = chunk_len(c0);
             for (uint32_t d = threadIdx.x; d < (uint32_t)kDigits2; d += kBlock) sh.cnt[d] = 0;
             __syncthreads();
@@ -1773,7 +1773,11 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_u_kernel(BinPlan P) {
 #pragma unroll
             for (int q = 0; q < kQ; ++q)
                 if (di[q] != 0xFFFFFFFFu) sh.sorted[sh.lstart[di[q] >> 16] + (di[q] & 0xFFFFu)] = v[q];
-            prefetch(c0 + 2 * kBinChunk, v);                 // stays in flight for two chunks
+            // the next chunk's loads stay in flight while this one is written out
+            const uint32_t c1 = c0 + kScatChunk;
+            bin_chunk_first<kScatChunk>(sh.runs, sh.first, c1, chunk_len(c1));
+            __syncthreads();
+            bin_chunk_load<kScatChunk>(P, sh.runs, sh.first, c1, chunk_len(c1), v);
             for (uint32_t i = threadIdx.x; i < m; i += kBlock) {
                 const uint64_t e = sh.sorted[i];
                 const uint32_t d = bin_d2(e, mask2);
@@ -1782,10 +1786,6 @@ __global__ __launch_bounds__(kBlock) void bin_l2_scatter_u_kernel(BinPlan P) {
             __syncthreads();
             for (uint32_t dd = threadIdx.x; dd < (uint32_t)kDigits2; dd += kBlock)
                 sh.cur[dd] += sh.lstart[dd + 1] - sh.lstart[dd];
-        };
-        for (uint32_t c0 = 0; c0 < n; c0 += 2 * kBinChunk) {
-            step(c0, va);
-            if (c0 + kBinChunk < n) step(c0 + kBinChunk, vb);
         }
     }
 }
