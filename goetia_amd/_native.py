@@ -101,6 +101,10 @@ SIGNATURES = {
     'gt_storage_table_bytes': [st_p, u32, u64p],
     'gt_storage_counters': [st_p, u64p, u64p],
     'gt_storage_set_counters': [st_p, u64, u64],
+    'gt_storage_set_use_bigcount': [st_p, C.c_int],
+    'gt_storage_get_use_bigcount': [st_p, C.POINTER(C.c_int)],
+    'gt_storage_bigcounts': [st_p, vp, vp, u64, u64p],
+    'gt_storage_set_bigcounts': [st_p, vp, vp, u64],
     'gt_storage_read_table': [st_p, u32, vp],
     'gt_storage_write_table': [st_p, u32, vp],
     'gt_storage_update_from': [st_p, st_p],

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, 'libgoetia_amd.so')
 ARCH = os.environ.get('GOETIA_AMD_ARCH', 'gfx950')
 
 SOURCES = ['gt_kernels.hip', 'gt_capi.cpp', 'gt_serving.cpp', 'gt_fastx.cpp', 'gt_membench.hip', 'gt_segments.cpp',
-           'gt_oxli.cpp', 'gt_sorted_iq.hip']
+           'gt_oxli.cpp', 'gt_sorted_iq.hip', 'gt_bigcount.hip']
 HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-Wall', '-Wno-unused-function',

@@ -256,6 +256,10 @@ struct gt_storage {
     // as if they had been inserted when the inserter returned.
     int (*flush_fn)(void*) = nullptr;
     void* flush_ctx = nullptr;
+    // count-min bigcount (storage.hh:129-135): off by default; the map lives
+    // on the device once bigcount has been used
+    bool use_bigcount = false;
+    BigMapDev big;
     std::recursive_mutex mu;
 
     Tables tables_view() const {
@@ -799,7 +803,62 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
     return GT_OK;
 }
 
+// Count-min insert (or insert_and_query, with iq_counts) with bigcount on:
+// the sort-based exact insert gives every window its tables' counts just
+// before it, so the windows that found every table full are known in window
+// order; insert_and_query takes its bigcount values from them before the map
+// takes the batch (bytestorage.cc:60-150).  Synchronises the stream.
+int bigcount_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, int16_t* iq_counts,
+                         hipStream_t s) {
+    if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
+    if (R.n_reads == 0 || R.n_bases == 0) return GT_OK;
+    GT_TRY(prepare_reads(st, R, true, s));
+    int64_t nk = 0;
+    GT_HIP(hipMemcpyAsync(&nk, st->own.kmer_off.as<int64_t>() + R.n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    if (nk <= 0) return GT_OK;
+    DevBuf fw, rc, valid, fresh, full, junk;
+    GT_TRY(fw.ensure((size_t)nk * 8));
+    GT_TRY(rc.ensure((size_t)nk * 8));
+    GT_TRY(valid.ensure((size_t)nk));
+    GT_TRY(full.ensure((size_t)nk));
+    uint8_t* flags = new_flags;
+    if (!flags && n_new) {
+        GT_TRY(fresh.ensure((size_t)nk));
+        flags = fresh.as<uint8_t>();
+    }
+    launch_hash(make_launch(st, s), R, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>());
+    GT_HIP(hipGetLastError());
+    GT_TRY(sorted_insert_query(st->tables_view(), fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(),
+                               (uint64_t)nk, R.canonical, iq_counts, flags, st->counters.as<uint64_t>(), g_dev.cus, s,
+                               full.as<uint8_t>()));
+    if (iq_counts)
+        GT_TRY(big_insert_query(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), R.canonical,
+                                full.as<uint8_t>(), iq_counts, (uint64_t)nk, g_dev.cus, s));
+    GT_TRY(big_insert(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), R.canonical, full.as<uint8_t>(), (uint64_t)nk,
+                      g_dev.cus, s));
+    if (n_new) {
+        // per-read sums of the flags (n_unique was counted by the sort-based pass)
+        GT_TRY(junk.ensure(kNCounters * sizeof(uint64_t)));
+        launch_bin_read_new(R.kmer_off, R.n_reads, flags, n_new, junk.as<uint64_t>(), s);
+        GT_HIP(hipGetLastError());
+    }
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+// The entry points that cannot keep the bigcount map refuse such storages.
+int refuse_bigcount(const gt_storage* st, const char* what) {
+    if (st->use_bigcount)
+        return fail(GT_ERR_UNSUPPORTED, std::string(what) + " is not available on a storage with bigcount on");
+    return GT_OK;
+}
+
 int dbg_insert_impl(gt_storage* st, Scratch& sc, Reads R, uint64_t* n_new, uint8_t* new_flags, hipStream_t s) {
+    if (st->use_bigcount) {
+        if (&sc != &st->own) return refuse_bigcount(st, "an insert with outside scratch");
+        return bigcount_insert_impl(st, R, n_new, new_flags, nullptr, s);
+    }
     if (R.n_reads == 0 || R.n_bases == 0) {
         if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
         return GT_OK;
@@ -836,6 +895,13 @@ int dbg_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags
 
 int hashes_insert_impl(gt_storage* st, const uint64_t* h, uint64_t n, uint8_t* is_new, hipStream_t s) {
     if (n == 0) return GT_OK;
+    if (st->use_bigcount) {
+        DevBuf full;
+        GT_TRY(full.ensure(n));
+        GT_TRY(sorted_insert_query(st->tables_view(), h, nullptr, nullptr, n, 0, nullptr, is_new,
+                                   st->counters.as<uint64_t>(), g_dev.cus, s, full.as<uint8_t>()));
+        return big_insert(st->big, h, nullptr, 0, full.as<uint8_t>(), n, g_dev.cus, s);
+    }
     const Launch L = make_launch(st, s);
     uint64_t* ctr = st->counters.as<uint64_t>();
     uint64_t sub = 0;
@@ -1082,6 +1148,7 @@ int gt_storage_free(gt_storage* st) {
     (void)hipDeviceSynchronize();
     for (auto* p : st->pipelines) p->st = nullptr;   // their later calls fail cleanly
     if (st->fence) (void)hipEventDestroy(st->fence);
+    big_free(st->big);
     delete st;
     return GT_OK;
 }
@@ -1144,6 +1211,54 @@ static uint64_t set_bucket_count(uint64_t n_unique, bool last_was_present) {
     return b;
 }
 
+// Storage::set_use_bigcount (storage.hh:129-135): count-min only.
+int gt_storage_set_use_bigcount(gt_storage* st, int use) {
+    if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
+    if (st->kind != kByte) return fail(GT_ERR_UNSUPPORTED, "bigcount is not supported for this storage.");
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
+    st->use_bigcount = use != 0;
+    return GT_OK;
+}
+
+int gt_storage_get_use_bigcount(gt_storage* st, int* use) {
+    if (!valid_storage(st) || !use) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    *use = st->use_bigcount ? 1 : 0;
+    return GT_OK;
+}
+
+// The bigcount map's entries, sorted by k-mer; *n gets their number, and
+// keys / vals (cap entries each) get them when they fit.
+int gt_storage_bigcounts(gt_storage* st, uint64_t* keys, int16_t* vals, uint64_t cap, uint64_t* n) {
+    if (!valid_storage(st) || !n) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    if (st->kind != kByte) return fail(GT_ERR_UNSUPPORTED, "bigcount is not supported for this storage.");
+    GT_TRY(ensure_device());
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
+    hipStream_t s = nullptr;
+    std::vector<uint64_t> k;
+    std::vector<int16_t> v;
+    GT_TRY(big_pairs(st->big, k, v, s));
+    *n = k.size();
+    if (keys && vals && cap >= k.size()) {
+        std::copy(k.begin(), k.end(), keys);
+        std::copy(v.begin(), v.end(), vals);
+    }
+    return GT_OK;
+}
+
+// Replaces the bigcount map's entries (the count-min loader,
+// bytestorage.cc:271-285); a later pair of one k-mer wins.
+int gt_storage_set_bigcounts(gt_storage* st, const uint64_t* keys, const int16_t* vals, uint64_t n) {
+    if (!valid_storage(st) || (n && (!keys || !vals))) return fail(GT_ERR_INVALID_ARG, "invalid arguments");
+    if (st->kind != kByte) return fail(GT_ERR_UNSUPPORTED, "bigcount is not supported for this storage.");
+    GT_TRY(ensure_device());
+    StorageLock lk(st);
+    GT_TRY(lk.rc);
+    GT_HIP(hipDeviceSynchronize());
+    return big_set_pairs(st->big, keys, vals, n, nullptr);
+}
+
 int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied) {
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(ensure_device());
@@ -1200,6 +1315,8 @@ int gt_storage_update_from(gt_storage* st, const gt_storage* other) {
     GT_TRY(ensure_device());
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "update_from"));
+    GT_TRY(refuse_bigcount(other, "update_from"));
     GT_HIP(hipDeviceSynchronize());
     for (uint32_t i = 0; i < st->n_tables; ++i) {
         const uint64_t words = (st->nbytes[i] + 3) / 4;
@@ -1442,6 +1559,22 @@ int gt_dbg_query_dev(gt_storage* st, uint16_t K, int canonical, int mode, const 
         launch_query(make_launch(st, s), R, st->tables_view(), st->exact_or_empty(), counts, hashes);
     }
     GT_HIP(hipGetLastError());
+    if (st->use_bigcount && st->big.aux) {
+        // ByteStorage::query's bigcount step for counts of 255 (bytestorage.cc:130-137)
+        int64_t nk = 0;
+        GT_HIP(hipMemcpyAsync(&nk, st->own.kmer_off.as<int64_t>() + n_reads, sizeof(nk), hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        if (nk > 0) {
+            DevBuf fw, rc, valid;
+            GT_TRY(fw.ensure((size_t)nk * 8));
+            GT_TRY(rc.ensure((size_t)nk * 8));
+            GT_TRY(valid.ensure((size_t)nk));
+            launch_hash(make_launch(st, s), R, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>());
+            GT_TRY(big_query(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), R.canonical, counts,
+                             (uint64_t)nk, g_dev.cus, s));
+            GT_HIP(hipStreamSynchronize(s));
+        }
+    }
     return GT_OK;
 }
 
@@ -1482,6 +1615,7 @@ int gt_dbg_insert_and_query_dev(gt_storage* st, uint16_t K, int canonical, int m
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
+    if (st->use_bigcount) return bigcount_insert_impl(st, R, nullptr, nullptr, counts, s);
     if (st->kind == kByte && bin_buckets(st)) {
         // Count-min results depend on the exact order of every earlier insert
         // of the same bins: the binned replay gives each event its bins'
@@ -1554,6 +1688,7 @@ int gt_solid_filter_dev(gt_storage* st, uint16_t K, int canonical, int mode, con
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "the batched solid filter"));
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     GT_TRY(st->solid_counts.ensure(std::max<uint64_t>(n_bases, 1) * sizeof(int16_t)));
@@ -1621,6 +1756,7 @@ int gt_diginorm_dev(gt_storage* st, uint16_t K, int canonical, int mode, const u
     if (!passed) return fail(GT_ERR_INVALID_ARG, "null passed");
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "the batched diginorm filter"));
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     // chunks end at read boundaries: the offsets come to the host once
@@ -1750,6 +1886,7 @@ int degrees_impl(gt_storage* st, uint16_t K, int canonical, int mode, int min_co
     if (n_reads == 0 || n_bases == 0) return GT_OK;
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "the neighbour / degree query"));
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);
@@ -1934,6 +2071,7 @@ int gt_query_hashes_dev(gt_storage* st, const uint64_t* hashes, uint64_t n, int1
     hipStream_t s = (hipStream_t)stream;
     launch_hash_query(make_launch(st, s), hashes, n, st->tables_view(), st->exact_or_empty(), counts);
     GT_HIP(hipGetLastError());
+    if (st->use_bigcount) GT_TRY(big_query(st->big, hashes, nullptr, nullptr, 0, counts, n, g_dev.cus, s));
     return GT_OK;
 }
 
@@ -1967,8 +2105,19 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
     GT_TRY(st->h_out2.ensure(n));
     GT_TRY(st->h_out1.ensure(n * sizeof(int16_t)));
     if (st->kind == kByte) {
+        DevBuf full;
+        uint8_t* fl = nullptr;
+        if (st->use_bigcount) {
+            GT_TRY(full.ensure(n));
+            fl = full.as<uint8_t>();
+        }
         GT_TRY(sorted_insert_query(st->tables_view(), st->h_out3.as<uint64_t>(), nullptr, nullptr, n, 0,
-                                   st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), g_dev.cus, s));
+                                   st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), g_dev.cus, s, fl));
+        if (fl) {
+            GT_TRY(big_insert_query(st->big, st->h_out3.as<uint64_t>(), nullptr, nullptr, 0, fl,
+                                    st->h_out1.as<int16_t>(), n, g_dev.cus, s));
+            GT_TRY(big_insert(st->big, st->h_out3.as<uint64_t>(), nullptr, 0, fl, n, g_dev.cus, s));
+        }
         GT_TRY(stage_out(counts, st->h_out1, n, s));
         GT_HIP(hipStreamSynchronize(s));
     } else {
@@ -2158,6 +2307,7 @@ int gt_pipeline_new(gt_storage* st, uint16_t K, int canonical, int mode, uint64_
     GT_TRY(ensure_device());
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "the captured pipeline"));
     const uint64_t per = std::max<uint32_t>(st->n_tables, 1);
     const uint64_t cap = next_pow2(2 * per * (max_bases + kTileMax));
     if (st->kind != kExact && cap > (1ULL << 29))
@@ -2216,6 +2366,7 @@ int gt_pipeline_submit(gt_pipeline* p, const uint8_t* seq, const int64_t* offset
     gt_storage* st = p->st;
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "the captured pipeline"));
     GT_TRY(order_after_fence(st, p->compute));
     gt_pipeline::Slot& sl = p->slot[p->next];
     p->next ^= 1;
@@ -2377,6 +2528,7 @@ extern "C" {
 int gt_dbg_insert_noalloc_dev(gt_storage* st, uint16_t K, int canonical, int mode, const uint8_t* seq,
                               const int64_t* offsets, uint64_t n_reads, uint64_t n_bases, uint64_t* n_new,
                               uint8_t* new_flags, void* stream) {
+    if (st && st->use_bigcount) return refuse_bigcount(st, "the allocation-free insert");
     if (!valid_storage(st)) return fail(GT_ERR_INVALID_ARG, "invalid storage");
     GT_TRY(check_read_args(K, mode, seq, offsets, n_reads, n_bases));
     if (st->kind == kExact)
@@ -2423,6 +2575,7 @@ int gt_dbg_query_ws_dev(gt_storage* st, gt_workspace* ws, uint16_t K, int canoni
     std::lock_guard<std::recursive_mutex> lw(ws->mu);
     StorageLock lk(st);
     GT_TRY(lk.rc);
+    GT_TRY(refuse_bigcount(st, "a query with outside scratch"));
     GT_TRY(order_after_fence(st, (hipStream_t)stream));
     hipStream_t s = (hipStream_t)stream;
     Reads R = make_reads(seq, offsets, n_reads, n_bases, K, mode, canonical);

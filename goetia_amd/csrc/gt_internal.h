@@ -145,4 +145,16 @@ struct Reads {
     int canonical;
 };
 
+// Count-min bigcount map (bytestorage.cc:95-106, 130-137) on the device:
+// open addressing over 64-bit keys, ~0 = free, the all-ones key at index
+// cap; values are int16 in the low half of a u32.
+enum BigAux : int { kBigOnes = 0, kBigKeys = 1, kBigAux = 2 };
+struct BigMapDev {
+    uint64_t* keys = nullptr;   // cap entries
+    uint32_t* vals = nullptr;   // cap + 1 entries
+    uint32_t* pend = nullptr;   // cap + 1: inserts of the current batch that found every table full
+    uint32_t* aux = nullptr;    // [kBigOnes] the all-ones key is present, [kBigKeys] keys in the table
+    uint64_t cap = 0;           // a power of two
+};
+
 }  // namespace gt

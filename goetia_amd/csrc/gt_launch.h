@@ -1,6 +1,8 @@
 // gt_launch.h -- host-side launchers of the kernels in gt_kernels.hip.
 #pragma once
 
+#include <vector>
+
 #include "gt_internal.h"
 
 namespace gt {
@@ -53,10 +55,23 @@ void launch_bit_union(uint32_t* me, const uint32_t* other, uint64_t n_words, int
 void launch_exact_rehash(const ExactSet& from, const ExactSet& to, uint64_t* counters, hipStream_t s);
 // Count-min insert_and_query of n values in index order (fw[k], or
 // min(fw, rc) when rc is set and canonical), sort-based (gt_sorted_iq.hip);
-// synchronises the stream before it returns.
+// full (optional) gets 1 for the values that found every table at 255.
+// Synchronises the stream before it returns.
 int sorted_insert_query(const Tables& T, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n,
                         int canonical, int16_t* counts, uint8_t* is_new, uint64_t* counters, int cus,
-                        hipStream_t s);
+                        hipStream_t s, uint8_t* full = nullptr);
+
+// The count-min bigcount map (gt_bigcount.hip).  Window values as above.
+int big_init(BigMapDev& M, hipStream_t s);
+void big_free(BigMapDev& M);
+int big_insert(BigMapDev& M, const uint64_t* fw, const uint64_t* rc, int canonical, const uint8_t* full, uint64_t n,
+               int cus, hipStream_t s);
+int big_query(const BigMapDev& M, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, int canonical,
+              int16_t* counts, uint64_t n, int cus, hipStream_t s);
+int big_insert_query(const BigMapDev& M, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, int canonical,
+                     const uint8_t* full, int16_t* counts, uint64_t n, int cus, hipStream_t s);
+int big_pairs(const BigMapDev& M, std::vector<uint64_t>& keys, std::vector<int16_t>& vals, hipStream_t s);
+int big_set_pairs(BigMapDev& M, const uint64_t* keys, const int16_t* vals, uint64_t n, hipStream_t s);
 void launch_exact_keys(const ExactSet& E, uint64_t* out, uint64_t cap_out, unsigned long long* n, hipStream_t s);
 
 // Binned insert of Bloom / count-min tables (one sub-batch = kbase, l1,

@@ -10,8 +10,8 @@
 // 1 counting) | [counting: use_bigcount u8] | ksize u32 | n_tables u8 |
 // occupied_bins u64 | per table: size u64, then size/8+1 (Bloom) or size
 // (count-min) bytes | [counting: n_bigcounts u64, then (k-mer u64, count i16)
-// pairs].  The device storages keep no bigcount map, so they write use_bigcount
-// 0 and no pairs, and refuse files that hold any.
+// pairs].  Count-min storages write their bigcount flag and map (pairs in
+// k-mer order; goetia writes its unordered_map's order) and load both.
 //
 // Both formats go through one sink / source interface; the gzip one uses
 // zlib's gzip file functions with the same open modes goetia uses ("wb",
@@ -139,6 +139,8 @@ struct Header {
     uint64_t sizes[gt::kMaxTables] = {};
     uint64_t occupied = 0;
     unsigned char use_bigcount = 0;
+    std::vector<uint64_t> big_keys;   // the bigcount pairs (read with the tables)
+    std::vector<int16_t> big_vals;
 };
 
 uint64_t table_bytes(int kind, uint64_t size) { return kind == GT_BIT_STORAGE ? size / 8 + 1 : size; }
@@ -170,8 +172,6 @@ int read_file(int kind, const std::string& path, Header* h, uint8_t* const* tabl
     h->ksize = (uint16_t)k;
     h->n_tables = nt;
     if (nt == 0 || nt > gt::kMaxTables) return io_fail(path + ": " + std::to_string(nt) + " tables (1..16 supported)");
-    if (h->use_bigcount) return gt::set_error(GT_ERR_UNSUPPORTED, path + ": a bigcount counting file (use_bigcount "
-                                                                        "= 1) cannot be loaded into a device storage");
     std::vector<uint8_t> skip;
     for (uint32_t i = 0; i < nt; ++i) {
         if (!src->get(&h->sizes[i], 8)) return io_fail("K-mer file header read error: " + path);
@@ -186,12 +186,52 @@ int read_file(int kind, const std::string& path, Header* h, uint8_t* const* tabl
         }
         if (!src->get(dst, nb)) return io_fail("Unexpected end of k-mer file: " + path);
     }
-    if (type == kCounting) {
+    if (type == kCounting && tables_out) {
+        // ByteStorageFileReader (bytestorage.cc:271-285): a pair count, then
+        // (k-mer, count) pairs
         uint64_t n_big = 0;
         if (!src->get(&n_big, 8)) return io_fail("K-mer count header read error: " + path);
-        if (n_big) return gt::set_error(GT_ERR_UNSUPPORTED, path + ": " + std::to_string(n_big) +
-                                                                " bigcount entries cannot be loaded into a device storage");
+        h->big_keys.clear();
+        h->big_vals.clear();
+        for (uint64_t j = 0; j < n_big; ++j) {
+            uint64_t key = 0;
+            int16_t val = 0;
+            if (!src->get(&key, 8) || !src->get(&val, 2)) return io_fail("Unexpected end of k-mer count file: " + path);
+            h->big_keys.push_back(key);
+            h->big_vals.push_back(val);
+        }
     }
+    return GT_OK;
+}
+
+// Writes one OXLI file; counting files carry use_bigcount and the pairs.
+int write_file(const char* path, int kind, uint16_t ksize, uint32_t n_tables, const uint64_t* tablesizes,
+               uint64_t occupied, const uint8_t* const* tables, unsigned char use_bigcount, uint64_t n_big,
+               const uint64_t* big_keys, const int16_t* big_vals) {
+    std::unique_ptr<Sink> out;
+    if (use_gz(kind, path)) {
+        gzFile f = gzopen(path, "wb");
+        if (!f) return io_fail(std::string("cannot open ") + path);
+        out.reset(new GzSink(f));
+    } else {
+        FILE* f = fopen(path, "wb");
+        if (!f) return io_fail(std::string("cannot open ") + path + ": " + strerror(errno));
+        out.reset(new FileSink(f));
+    }
+    const unsigned char version = kVersion, type = kind == GT_BIT_STORAGE ? kHashbits : kCounting;
+    const uint32_t k = ksize;
+    const unsigned char nt = (unsigned char)n_tables;
+    bool ok = out->put("OXLI", 4) && out->put(&version, 1) && out->put(&type, 1);
+    if (type == kCounting) ok = ok && out->put(&use_bigcount, 1);
+    ok = ok && out->put(&k, 4) && out->put(&nt, 1) && out->put(&occupied, 8);
+    for (uint32_t i = 0; ok && i < n_tables; ++i)
+        ok = out->put(&tablesizes[i], 8) && out->put(tables[i], table_bytes(kind, tablesizes[i]));
+    if (type == kCounting) {
+        ok = ok && out->put(&n_big, 8);
+        for (uint64_t j = 0; ok && j < n_big; ++j) ok = out->put(&big_keys[j], 8) && out->put(&big_vals[j], 2);
+    }
+    const bool closed = out->close();
+    if (!ok || !closed) return io_fail(std::string("write failed: ") + path);
     return GT_OK;
 }
 
@@ -205,31 +245,7 @@ int gt_oxli_write(const char* path, int kind, uint16_t ksize, uint32_t n_tables,
         return gt::set_error(GT_ERR_INVALID_ARG, "invalid arguments");
     if (kind != GT_BIT_STORAGE && kind != GT_BYTE_STORAGE)
         return gt::set_error(GT_ERR_INVALID_ARG, "OXLI files hold Bloom and count-min tables");
-    std::unique_ptr<Sink> out;
-    if (use_gz(kind, path)) {
-        gzFile f = gzopen(path, "wb");
-        if (!f) return io_fail(std::string("cannot open ") + path);
-        out.reset(new GzSink(f));
-    } else {
-        FILE* f = fopen(path, "wb");
-        if (!f) return io_fail(std::string("cannot open ") + path + ": " + strerror(errno));
-        out.reset(new FileSink(f));
-    }
-    const unsigned char version = kVersion, type = kind == GT_BIT_STORAGE ? kHashbits : kCounting, big = 0;
-    const uint32_t k = ksize;
-    const unsigned char nt = (unsigned char)n_tables;
-    bool ok = out->put("OXLI", 4) && out->put(&version, 1) && out->put(&type, 1);
-    if (type == kCounting) ok = ok && out->put(&big, 1);
-    ok = ok && out->put(&k, 4) && out->put(&nt, 1) && out->put(&occupied, 8);
-    for (uint32_t i = 0; ok && i < n_tables; ++i)
-        ok = out->put(&tablesizes[i], 8) && out->put(tables[i], table_bytes(kind, tablesizes[i]));
-    if (type == kCounting) {
-        const uint64_t n_big = 0;
-        ok = ok && out->put(&n_big, 8);
-    }
-    const bool closed = out->close();
-    if (!ok || !closed) return io_fail(std::string("write failed: ") + path);
-    return GT_OK;
+    return write_file(path, kind, ksize, n_tables, tablesizes, occupied, tables, 0, 0, nullptr, nullptr);
 }
 
 int gt_oxli_read_header(const char* path, int kind, uint16_t* ksize, uint32_t* n_tables, uint64_t* tablesizes,
@@ -270,7 +286,19 @@ int gt_storage_save(gt_storage* st, const char* path, uint16_t ksize) {
         GT_TRY(gt_storage_read_table(st, i, host[i].data()));
         ptrs[i] = host[i].data();
     }
-    return gt_oxli_write(path, kind, ksize, nt, sizes.data(), occ, ptrs.data());
+    int use_big = 0;
+    std::vector<uint64_t> bk;
+    std::vector<int16_t> bv;
+    if (kind == GT_BYTE_STORAGE) {
+        GT_TRY(gt_storage_get_use_bigcount(st, &use_big));
+        uint64_t n_big = 0;
+        GT_TRY(gt_storage_bigcounts(st, nullptr, nullptr, 0, &n_big));
+        bk.resize(n_big);
+        bv.resize(n_big);
+        GT_TRY(gt_storage_bigcounts(st, bk.data(), bv.data(), n_big, &n_big));
+    }
+    return write_file(path, kind, ksize, nt, sizes.data(), occ, ptrs.data(), (unsigned char)(use_big ? 1 : 0),
+                      bk.size(), bk.data(), bv.data());
 }
 
 int gt_storage_load(gt_storage* st, const char* path, uint16_t* ksize) {
@@ -294,6 +322,12 @@ int gt_storage_load(gt_storage* st, const char* path, uint16_t* ksize) {
     uint64_t n_unique = 0;
     GT_TRY(gt_storage_counters(st, &n_unique, nullptr));
     GT_TRY(gt_storage_set_counters(st, n_unique, h.occupied));
+    if (kind == GT_BYTE_STORAGE) {
+        // the file's flag; its pairs replace the map only when there are any
+        GT_TRY(gt_storage_set_use_bigcount(st, h.use_bigcount ? 1 : 0));
+        if (!h.big_keys.empty())
+            GT_TRY(gt_storage_set_bigcounts(st, h.big_keys.data(), h.big_vals.data(), h.big_keys.size()));
+    }
     if (ksize) *ksize = h.ksize;
     return GT_OK;
 }

@@ -3,15 +3,16 @@
 // any number of tables and any table size, by sorting instead of replaying.
 //
 // A k-mer's result is the minimum over tables of its bin's count just after
-// its own insert, and that count is min(255, c0 + r + 1): c0 the bin's count
-// before the batch and r the number of earlier events of the batch on the
-// same (table, bin).  So every (window, table) event becomes a key
+// its own insert.  The count just before it is min(255, c0 + r): c0 the bin's
+// count before the batch and r the number of earlier events of the batch on
+// the same (table, bin).  So every (window, table) event becomes a key
 // table << 32 | bin with its window index as value; a stable radix sort puts
 // each bin's events together in window order, a max-scan of run starts gives
-// r, every event reads c0 and writes its post-insert count, and then the last
-// event of each run stores its count in the table.  A k-mer is new exactly
-// when its minimum is 1 (some bin was 0 just before it), and table 0's bin
-// became occupied exactly when table 0's count is 1.  Windows go through in
+// r, every event reads c0 and writes its count just before, and then the
+// last event of each run stores one more (at most 255) in the table.  A
+// k-mer is new exactly when some table was 0 just before it, table 0's bin
+// became occupied exactly when table 0 was 0, and every table was full (the
+// bigcount condition, bytestorage.cc:95-96) when the smallest was 255.  Windows go through in
 // chunks, each applied before the next, which keeps the order.
 //
 // Used where the binned replay does not apply: more than four tables, tables
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kSqBlock) void sq_post_kernel(const uint64_t* keys,
         const uint64_t key = keys[j];
         if (key == kSentinel) continue;
         const uint32_t t = (uint32_t)(key >> 32), bin = (uint32_t)key;
-        const uint32_t c = (uint32_t)T.base[t][bin] + (uint32_t)(j - start[j]) + 1u;
+        const uint32_t c = (uint32_t)T.base[t][bin] + (uint32_t)(j - start[j]);   // count just before
         vq[(uint64_t)vals[j] * T.n + t] = (uint8_t)(c < 255u ? c : 255u);
     }
 }
@@ -81,19 +82,24 @@ __global__ __launch_bounds__(kSqBlock) void sq_store_kernel(const uint64_t* keys
         const uint64_t key = keys[j];
         if (key == kSentinel || (j + 1 < n && keys[j + 1] == key)) continue;   // last event of its run
         const uint32_t t = (uint32_t)(key >> 32);
-        T.base[t][(uint32_t)key] = vq[(uint64_t)vals[j] * T.n + t];
+        const uint32_t before = vq[(uint64_t)vals[j] * T.n + t];
+        T.base[t][(uint32_t)key] = (uint8_t)(before < 255u ? before + 1u : 255u);
     }
 }
 
+// Per window, from its tables' counts just before it: the minimum just
+// after it (its insert_and_query result), whether it is new (a table was 0),
+// whether table 0's bin became occupied, and whether every table was full.
 __global__ __launch_bounds__(kSqBlock) void sq_combine_kernel(const uint8_t* valid, uint64_t w0, uint32_t m,
                                                               uint32_t nt, const uint8_t* vq, int16_t* counts,
-                                                              uint8_t* is_new, uint64_t* counters) {
+                                                              uint8_t* is_new, uint8_t* full, uint64_t* counters) {
     uint64_t nn = 0, occ = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)kSqBlock + threadIdx.x; i < m; i += (uint64_t)gridDim.x * kSqBlock) {
         const uint64_t k = w0 + i;
         if (valid && !valid[k]) {
             if (counts) counts[k] = -1;
             if (is_new) is_new[k] = 0;
+            if (full) full[k] = 0;
             continue;
         }
         uint32_t lo = 255;
@@ -101,10 +107,12 @@ __global__ __launch_bounds__(kSqBlock) void sq_combine_kernel(const uint8_t* val
             const uint32_t c = vq[i * nt + t];
             lo = c < lo ? c : lo;
         }
-        nn += lo == 1;
-        occ += vq[i * nt] == 1;
-        if (counts) counts[k] = (int16_t)lo;
-        if (is_new) is_new[k] = lo == 1;
+        const uint32_t after = lo < 255u ? lo + 1u : 255u;
+        nn += lo == 0;
+        occ += vq[i * nt] == 0;
+        if (counts) counts[k] = (int16_t)after;
+        if (is_new) is_new[k] = lo == 0;
+        if (full) full[k] = lo == 255u;
     }
     for (int o = 32; o > 0; o >>= 1) {
         nn += __shfl_xor(nn, o, 64);
@@ -152,7 +160,7 @@ unsigned grid_for(uint64_t n, int cus) {
 
 int sorted_insert_query(const Tables& T, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, uint64_t n,
                         int canonical, int16_t* counts, uint8_t* is_new, uint64_t* counters, int cus,
-                        hipStream_t s) {
+                        hipStream_t s, uint8_t* full) {
     if (n == 0) return GT_OK;
     if (T.n == 0 || T.n > (uint32_t)kMaxTables) return set_error(GT_ERR_INVALID_ARG, "1..16 tables");
     const uint32_t nt = T.n;
@@ -201,7 +209,7 @@ int sorted_insert_query(const Tables& T, const uint64_t* fw, const uint64_t* rc,
         sq_store_kernel<<<grid_for(np, cus), kSqBlock, 0, s>>>(keys_out.as<uint64_t>(), vals_out.as<uint32_t>(), np, T,
                                                                vq.as<uint8_t>());
         sq_combine_kernel<<<grid_for(m, cus), kSqBlock, 0, s>>>(valid, w0, m, nt, vq.as<uint8_t>(), counts, is_new,
-                                                                counters);
+                                                                full, counters);
         SQ_TRY(hip_check(hipGetLastError(), "sorted insert_and_query launch"));
     }
     // the buffers are released on return

@@ -7,8 +7,9 @@ The files are written and read by libgoetia_amd (gt_storage_save /
 gt_storage_load, and the host-only gt_oxli_* for tables in host memory), so
 the bytes -- compressed ones included -- are goetia's.  A load replaces the
 storage's tables, sizes included, and restores n_occupied (n_unique is not
-in the format).  Files holding bigcount entries are refused: the device
-storages keep no bigcount map.
+in the format).  Count-min files carry the bigcount flag and pairs: a
+storage's save writes its map (pairs sorted by k-mer) and a load takes the
+file's; the host-only readers skip the pairs.
 """
 import ctypes as C
 

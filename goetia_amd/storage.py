@@ -36,6 +36,15 @@ class _Storage:
     kind = None
     Traits = None
 
+    def set_use_bigcount(self, b):
+        """Storage::set_use_bigcount (storage.hh:129-135): only the count-min
+        storage supports bigcount."""
+        if b:
+            raise N.GoetiaException('bigcount is not supported for this storage.')
+
+    def get_use_bigcount(self):
+        return False
+
     def __init__(self, *args, capacity_hint=0):
         self._h = C.c_void_p()
         self.build_params = args
@@ -221,13 +230,29 @@ class ByteStorage(_Storage):
     params_type = tuple
 
     def set_use_bigcount(self, b):
-        """storage.hh:129-135.  The device tables saturate at 255; counts above
-        that (the reference's optional bigcount map) are not kept."""
-        if b:
-            raise N.GoetiaException('bigcount is not supported by the device ByteStorage')
+        """storage.hh:129-135: with bigcount on, counts past 255 go to a map of
+        k-mer -> int16 count on the device (bytestorage.cc:95-106, 130-137)."""
+        N.call('gt_storage_set_use_bigcount', self._h, int(bool(b)))
 
     def get_use_bigcount(self):
-        return False
+        v = C.c_int()
+        N.call('gt_storage_get_use_bigcount', self._h, C.byref(v))
+        return bool(v.value)
+
+    def bigcounts(self):
+        """The bigcount map as (k-mers, counts) numpy arrays, sorted by k-mer."""
+        n = C.c_uint64()
+        N.call('gt_storage_bigcounts', self._h, None, None, 0, C.byref(n))
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        vals = np.zeros(max(n.value, 1), np.int16)
+        N.call('gt_storage_bigcounts', self._h, N.ptr(keys), N.ptr(vals), n.value, C.byref(n))
+        return keys[:n.value], vals[:n.value]
+
+    def set_bigcounts(self, keys, vals):
+        """Replaces the bigcount map (a later pair of one k-mer wins)."""
+        keys = np.ascontiguousarray(keys, np.uint64)
+        vals = np.ascontiguousarray(vals, np.int16)
+        N.call('gt_storage_set_bigcounts', self._h, N.ptr(keys), N.ptr(vals), len(keys))
 
 
 class SparseppSetStorage(_Storage):

@@ -1,7 +1,7 @@
 /**
  * bytestorage.hh -- ByteStorage (goetia bytestorage.hh:96-243): a count-min
- * sketch of saturating 8-bit counters (MAX_KCOUNT 255) in HBM.  The optional
- * bigcount map of the reference (counts above 255) is not kept.
+ * sketch of saturating 8-bit counters (MAX_KCOUNT 255) in HBM, with goetia's
+ * optional bigcount map (counts past 255, int16) on the device.
  */
 #ifndef GOETIA_AMD_BYTESTORAGE_HH
 #define GOETIA_AMD_BYTESTORAGE_HH
@@ -54,10 +54,12 @@ class ByteStorage : public Storage<uint64_t>, public detail::DeviceStorage {
     const count_t query(value_type h) const override { return query_(h); }
     byte_t** get_raw_tables() override { return raw_tables_(); }
     void reset() override { reset_(); }
-    void set_use_bigcount(bool b) {
-        if (b) throw GoetiaException("bigcount is not supported by the device ByteStorage");
+    void set_use_bigcount(bool b) { detail::check(gt_storage_set_use_bigcount(handle(), b ? 1 : 0), "set_use_bigcount"); }
+    bool get_use_bigcount() {
+        int v = 0;
+        detail::check(gt_storage_get_use_bigcount(handle(), &v), "get_use_bigcount");
+        return v != 0;
     }
-    bool get_use_bigcount() { return false; }
     void save(std::string filename, uint16_t ksize) override { oxli::save(*this, filename, ksize, oxli::kCounting); }
     void load(std::string filename, uint16_t& ksize) override { ksize = oxli::load(*this, filename, oxli::kCounting); }
 };

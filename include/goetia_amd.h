@@ -114,6 +114,25 @@ int gt_storage_table_bytes(const gt_storage* st, uint32_t i, uint64_t* nbytes);
 int gt_storage_counters(gt_storage* st, uint64_t* n_unique, uint64_t* n_occupied);
 /* Restores counters after load() (bitstorage.cc:196-307). */
 int gt_storage_set_counters(gt_storage* st, uint64_t n_unique, uint64_t n_occupied);
+/* Storage::set_use_bigcount / get_use_bigcount (storage.hh:129-140); count-min
+ * only, else GT_ERR_UNSUPPORTED ("bigcount is not supported for this
+ * storage.").  With it on, a count-min insert that finds every table at 255
+ * adds one to its k-mer's bigcount entry (bytestorage.cc:95-106: 0 -> 256,
+ * then +1 while the int16 value is below 65535 as an unsigned int, so 32767
+ * wraps to -32768 and stays), and a query whose minimum is 255 returns the
+ * entry when there is one (:130-137); insert_and_query sees each k-mer's
+ * entry just after its own insert.  The filters, degree queries, captured
+ * pipelines, workspace and allocation-free entry points refuse a storage with
+ * bigcount on (GT_ERR_UNSUPPORTED).  reset() keeps the map, as goetia's
+ * does. */
+int gt_storage_set_use_bigcount(gt_storage* st, int use);
+int gt_storage_get_use_bigcount(gt_storage* st, int* use);
+/* The bigcount map's entries sorted by k-mer: *n their number; keys and vals
+ * (cap entries each) get them when they fit.  Count-min only. */
+int gt_storage_bigcounts(gt_storage* st, uint64_t* keys, int16_t* vals, uint64_t cap, uint64_t* n);
+/* Replaces the bigcount map (the count-min loader, bytestorage.cc:271-285;
+ * a later pair of one k-mer wins).  Count-min only. */
+int gt_storage_set_bigcounts(gt_storage* st, const uint64_t* keys, const int16_t* vals, uint64_t n);
 /* get_raw_tables() bitstorage.hh:228-231: copies table i (reference layout)
  * to host memory of gt_storage_table_bytes() bytes. */
 int gt_storage_read_table(gt_storage* st, uint32_t i, uint8_t* host_dst);
@@ -142,9 +161,11 @@ int gt_storage_load_async(gt_storage* st, const uint8_t* const* host_tables, con
  * 538-618) when the name's text after its last '.' is "gz" and plain files
  * otherwise (:170-302, 480-536).  load replaces the storage's tables, sizes
  * included, with the file's and restores n_occupied; n_unique is kept (the
- * format does not hold it).  The device storages keep no bigcount map: save
- * writes use_bigcount 0 and no pairs, load refuses (GT_ERR_UNSUPPORTED) a file
- * with use_bigcount set or bigcount pairs.  I/O failures are GT_ERR_IO
+ * format does not hold it).  Count-min files carry the bigcount flag and
+ * pairs: save writes the storage's (pairs sorted by k-mer; goetia writes its
+ * hash map's order), load sets the flag and, when the file holds pairs,
+ * replaces the map with them (bytestorage.cc:247, 271-285).  The host-only
+ * readers below skip the pairs.  I/O failures are GT_ERR_IO
  * (GoetiaFileException).  The exact set has no file format
  * (sparseppstorage.cc:52-58 are empty): GT_ERR_UNSUPPORTED. */
 int gt_storage_save(gt_storage* st, const char* path, uint16_t ksize);

@@ -281,6 +281,8 @@ int or_storage_init(or_storage* st, int kind, const uint64_t* sizes, uint32_t n_
 void or_storage_free(or_storage* st) {
     for (uint32_t i = 0; i < st->n_tables; ++i) free(st->tables[i]);
     free(st->keys);
+    free(st->big_keys);
+    free(st->big_vals);
     memset(st, 0, sizeof(*st));
 }
 
@@ -371,8 +373,94 @@ static int16_t bit_query(const or_storage* st, uint64_t h) {
 }
 
 /* bytestorage.cc:60-113 ByteStorage::insert (bigcount disabled, its default) */
+/* The bigcount map: open addressing over keys (OR_EMPTY marks a free slot;
+ * the all-ones key has its own flag), int16 values. */
+static int16_t* big_slot(or_storage* st, uint64_t h, int create) {
+    if (h == OR_EMPTY) {
+        if (!st->big_has_ones) {
+            if (!create) return NULL;
+            st->big_has_ones = 1;
+            st->big_ones_val = 0;
+        }
+        return &st->big_ones_val;
+    }
+    if (!st->big_keys) {
+        if (!create) return NULL;
+        st->big_cap = 64;
+        st->big_keys = (uint64_t*)malloc(st->big_cap * sizeof(uint64_t));
+        st->big_vals = (int16_t*)calloc(st->big_cap, sizeof(int16_t));
+        memset(st->big_keys, 0xFF, st->big_cap * sizeof(uint64_t));
+    }
+    uint64_t s = or_mix(h) & (st->big_cap - 1);
+    while (st->big_keys[s] != OR_EMPTY) {
+        if (st->big_keys[s] == h) return &st->big_vals[s];
+        s = (s + 1) & (st->big_cap - 1);
+    }
+    if (!create) return NULL;
+    if (2 * (st->big_count + 1) > st->big_cap) {
+        uint64_t old_cap = st->big_cap;
+        uint64_t* ok = st->big_keys;
+        int16_t* ov = st->big_vals;
+        st->big_cap = old_cap * 2;
+        st->big_keys = (uint64_t*)malloc(st->big_cap * sizeof(uint64_t));
+        st->big_vals = (int16_t*)calloc(st->big_cap, sizeof(int16_t));
+        memset(st->big_keys, 0xFF, st->big_cap * sizeof(uint64_t));
+        for (uint64_t j = 0; j < old_cap; ++j) {
+            if (ok[j] == OR_EMPTY) continue;
+            uint64_t t = or_mix(ok[j]) & (st->big_cap - 1);
+            while (st->big_keys[t] != OR_EMPTY) t = (t + 1) & (st->big_cap - 1);
+            st->big_keys[t] = ok[j];
+            st->big_vals[t] = ov[j];
+        }
+        free(ok);
+        free(ov);
+        return big_slot(st, h, 1);
+    }
+    st->big_keys[s] = h;
+    st->big_vals[s] = 0;
+    st->big_count++;
+    return &st->big_vals[s];
+}
+
+void or_set_use_bigcount(or_storage* st, int b) { st->use_bigcount = b ? 1 : 0; }
+
+int or_bigcount_get(const or_storage* st, uint64_t h, int16_t* v) {
+    int16_t* p = big_slot((or_storage*)st, h, 0);
+    if (!p) return 0;
+    *v = *p;
+    return 1;
+}
+
+void or_bigcount_set(or_storage* st, uint64_t h, int16_t v) { *big_slot(st, h, 1) = v; }
+
+/* The map's pairs, sorted by key (the order of goetia's unordered_map is its
+ * own); returns the number of pairs. */
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+uint64_t or_bigcount_pairs(const or_storage* st, uint64_t* keys, int16_t* vals, uint64_t cap) {
+    uint64_t n = st->big_count + (st->big_has_ones ? 1 : 0);
+    if (!keys || cap < n) return n;
+    uint64_t k = 0;
+    for (uint64_t j = 0; j < st->big_cap; ++j)
+        if (st->big_keys[j] != OR_EMPTY) keys[k++] = st->big_keys[j];
+    if (st->big_has_ones) keys[k++] = OR_EMPTY;
+    qsort(keys, k, sizeof(uint64_t), cmp_u64);
+    for (uint64_t j = 0; j < k; ++j) or_bigcount_get(st, keys[j], &vals[j]);
+    return n;
+}
+
+/* bytestorage.cc:60-114 ByteStorage::insert.  A table whose count is already
+ * 255 is "full"; when every table is full and bigcount is on, the k-mer's
+ * map entry (0 when absent, as operator[] creates it) becomes 256 if it was
+ * 0, else goes up by one while the int16 value compares below 65535 as an
+ * unsigned int: 1..32766 step up, 32767 wraps to -32768, and a negative
+ * value (a huge unsigned) stays. */
 static int byte_insert(or_storage* st, uint64_t h) {
     int is_new = 0;
+    uint32_t n_full = 0;
     for (uint32_t i = 0; i < st->n_tables; ++i) {
         uint64_t bin = h % st->sizes[i];
         uint8_t cur = st->tables[i][bin];
@@ -383,6 +471,12 @@ static int byte_insert(or_storage* st, uint64_t h) {
             }
         }
         if (cur < 255) st->tables[i][bin] = (uint8_t)(cur + 1);
+        else n_full++;
+    }
+    if (n_full == st->n_tables && st->use_bigcount) {
+        int16_t* v = big_slot(st, h, 1);
+        if (*v == 0) *v = 256;
+        else if ((unsigned)(int)*v < 65535u) *v = (int16_t)(uint16_t)((uint16_t)*v + 1u);
     }
     if (is_new) st->n_unique++;
     return is_new;
@@ -395,6 +489,8 @@ static int16_t byte_query(const or_storage* st, uint64_t h) {
         int16_t c = st->tables[i][h % st->sizes[i]];
         if (c < m) m = c;
     }
+    int16_t big;
+    if (m == 255 && st->use_bigcount && or_bigcount_get(st, h, &big)) m = big;
     return m;
 }
 

@@ -60,6 +60,15 @@ typedef struct or_storage {
     uint64_t  capacity;
     uint64_t  count;
     int       has_ones_key;
+    /* count-min bigcount map (bytestorage.cc:95-106, 130-137): k-mer hash ->
+     * int16 count, open addressing like the exact set's keys. */
+    int       use_bigcount;
+    uint64_t* big_keys;
+    int16_t*  big_vals;
+    uint64_t  big_cap;
+    uint64_t  big_count;
+    int       big_has_ones;
+    int16_t   big_ones_val;
 } or_storage;
 
 uint64_t or_table_value(int c);
@@ -77,6 +86,10 @@ void     or_storage_free(or_storage* st);
 uint64_t or_table_bytes(const or_storage* st, uint32_t i);
 
 int      or_insert(or_storage* st, uint64_t h);
+void     or_set_use_bigcount(or_storage* st, int b);
+int      or_bigcount_get(const or_storage* st, uint64_t h, int16_t* v);
+void     or_bigcount_set(or_storage* st, uint64_t h, int16_t v);
+uint64_t or_bigcount_pairs(const or_storage* st, uint64_t* keys, int16_t* vals, uint64_t cap);
 int16_t  or_query(const or_storage* st, uint64_t h);
 int16_t  or_insert_and_query(or_storage* st, uint64_t h);
 

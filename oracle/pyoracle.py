@@ -23,7 +23,10 @@ class _Storage(C.Structure):
                 ('tables', C.POINTER(C.c_uint8) * MAX_TABLES),
                 ('n_unique', C.c_uint64), ('n_occupied', C.c_uint64),
                 ('keys', C.POINTER(C.c_uint64)), ('capacity', C.c_uint64),
-                ('count', C.c_uint64), ('has_ones_key', C.c_int)]
+                ('count', C.c_uint64), ('has_ones_key', C.c_int),
+                ('use_bigcount', C.c_int), ('big_keys', C.POINTER(C.c_uint64)),
+                ('big_vals', C.POINTER(C.c_int16)), ('big_cap', C.c_uint64), ('big_count', C.c_uint64),
+                ('big_has_ones', C.c_int), ('big_ones_val', C.c_int16)]
 
 
 def build():
@@ -70,6 +73,10 @@ def lib():
             'or_query_reads': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p,
                                             C.c_uint64, i16p]),
             'or_exact_keys': (C.c_uint64, [sp, u64p, C.c_uint64]),
+            'or_set_use_bigcount': (None, [sp, C.c_int]),
+            'or_bigcount_get': (C.c_int, [sp, C.c_uint64, i16p]),
+            'or_bigcount_set': (None, [sp, C.c_uint64, C.c_int16]),
+            'or_bigcount_pairs': (C.c_uint64, [sp, u64p, i16p, C.c_uint64]),
             'or_insert_reads_mt': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p, C.c_uint64,
                                                 C.c_int]),
             'or_query_reads_mt': (C.c_uint64, [sp, C.c_int, C.c_int, C.c_int, C.c_char_p, i64p, C.c_uint64,
@@ -197,6 +204,20 @@ class Storage:
 
     def insert_and_query(self, h):
         return int(lib().or_insert_and_query(self.ref, int(h)))
+
+    def set_use_bigcount(self, b=True):
+        lib().or_set_use_bigcount(self.ref, int(bool(b)))
+
+    def bigcount_pairs(self):
+        """The bigcount map as (keys, values), sorted by key."""
+        n = lib().or_bigcount_pairs(self.ref, None, None, 0)
+        keys = np.zeros(max(n, 1), np.uint64)
+        vals = np.zeros(max(n, 1), np.int16)
+        lib().or_bigcount_pairs(self.ref, _p(keys, C.c_uint64), _p(vals, C.c_int16), n)
+        return keys[:n], vals[:n]
+
+    def bigcount_set(self, h, v):
+        lib().or_bigcount_set(self.ref, int(h), int(v))
 
     def exact_keys(self):
         n = lib().or_exact_keys(self.ref, None, 0)

@@ -38,6 +38,7 @@ struct Handle {
     int kind;       // 1 bit, 2 byte, 3 sparsepp set
     int canonical;
     void* graph;    // Holder<dBG<...>>*
+    std::shared_ptr<ByteStorage> bytes;   // the count-min storage (kind 2)
 };
 
 template <class S, class H>
@@ -260,13 +261,14 @@ uint32_t ref_primes(uint32_t n, uint64_t x, uint64_t* out) {
 }
 
 void* ref_new(int kind, int canonical, int K, uint64_t max_table, uint16_t n_tables) {
-    auto* h = new Handle{kind, canonical, nullptr};
+    auto* h = new Handle{kind, canonical, nullptr, nullptr};
     if (kind == 1) {
         auto s = BitStorage::build(max_table, n_tables);
         h->graph = canonical ? make_graph<BitStorage, CanLemireShifter>(s, K)
                              : make_graph<BitStorage, FwdLemireShifter>(s, K);
     } else if (kind == 2) {
         auto s = ByteStorage::build(max_table, n_tables);
+        h->bytes = s;
         h->graph = canonical ? make_graph<ByteStorage, CanLemireShifter>(s, K)
                              : make_graph<ByteStorage, FwdLemireShifter>(s, K);
     } else {
@@ -542,6 +544,28 @@ uint64_t ref_paired_walk(const char* left, const char* right, int strict, uint32
 
 // dBG::save(filename) (dbg.hh:396-399): the graph's storage writes its OXLI
 // file (BitStorage::save, ByteStorage::save through ByteStorageFile).
+// Storage::set_use_bigcount (storage.hh:129-135) on a count-min handle.
+void ref_set_use_bigcount(void* hv, int b) {
+    Handle* h = (Handle*)hv;
+    if (h->bytes) h->bytes->set_use_bigcount(b != 0);
+}
+
+// dBG::load (dbg.hh:400-403) into a count-min handle's storage; 0, or -1 when
+// the loader threw.
+int ref_load_into(void* hv, const char* path) {
+    Handle* h = (Handle*)hv;
+    if (h->kind != 2) return -1;
+    try {
+        dispatch(h, [&](auto* g) {
+            g->load(std::string(path));
+            return 0;
+        });
+    } catch (std::exception&) {
+        return -1;
+    }
+    return 0;
+}
+
 void ref_save(void* hv, const char* path) {
     Handle* h = (Handle*)hv;
     dispatch(h, [&](auto* g) {

@@ -232,5 +232,8 @@ def test_storage_build_params_and_describe():
     c = lg.ByteStorage.build(10 ** 7, 4)
     assert c.get_tablesizes() == [9999991, 9999973, 9999971, 9999943]
     assert 'Counting:      True' in c.describe()
+    assert not c.get_use_bigcount()
+    c.set_use_bigcount(True)
+    assert c.get_use_bigcount()
     with pytest.raises(lg.GoetiaException):
-        c.set_use_bigcount(True)
+        b.set_use_bigcount(True)

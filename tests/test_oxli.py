@@ -5,7 +5,8 @@ what goetia's loader read back in tests/golden/oxli_ref.json).
 Host (CPU): the library's reader gives goetia's header fields and tables for
 every file, plain and gzip; its writer, given those tables, writes the same
 bytes, compressed files included (goetia picks gzip for count-min files
-named *.gz; Bloom files are never compressed); bigcount files are refused;
+named *.gz; Bloom files are never compressed); the host reader skips a
+count-min file's bigcount pairs;
 where the compiled reference is built, goetia's loader reads files the
 library wrote.
 GPU: a device storage filled as goetia's was saves the same bytes, and loads
@@ -61,19 +62,24 @@ def _patched(tmp_path, name, offset, value):
     return p
 
 
-def test_bigcount_files_refused(tmp_path):
+def test_bigcount_pairs_skipped_by_host_reader(tmp_path):
+    """A count-min file with use_bigcount set and a bigcount pair reads with
+    the same tables; a pair count with no pair after it is a file error."""
     from goetia_amd import _native as N
     from goetia_amd import oxli
     m = META['counts_k21.oxli']
+    base = oxli.read_file(os.path.join(OXLI_DIR, 'counts_k21.oxli'), N.BYTE)
     p = _patched(tmp_path, 'counts_k21.oxli', 6, b'\x01')       # use_bigcount = 1
-    with pytest.raises(N.UnsupportedError):
-        oxli.read_file(p, N.BYTE)
+    got = oxli.read_file(p, N.BYTE)
+    assert got[:3] == base[:3] and all(np.array_equal(a, b) for a, b in zip(got[3], base[3]))
     size = os.path.getsize(os.path.join(OXLI_DIR, 'counts_k21.oxli'))
     p = _patched(tmp_path, 'counts_k21.oxli', size - 8, (1).to_bytes(8, 'little'))   # one bigcount pair
+    with pytest.raises(N.GoetiaFileException):
+        oxli.read_file(p, N.BYTE)
     with open(p, 'ab') as f:
         f.write((12345).to_bytes(8, 'little') + (300).to_bytes(2, 'little'))
-    with pytest.raises(N.UnsupportedError):
-        oxli.read_file(p, N.BYTE)
+    got = oxli.read_file(p, N.BYTE)
+    assert all(np.array_equal(a, b) for a, b in zip(got[3], base[3]))
     assert m['kind'] == N.BYTE
 
 
