@@ -19,6 +19,7 @@
 #include "gt_hash.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace gt {
 
@@ -257,13 +258,17 @@ __device__ __forceinline__ void call_end_tile(Op&, int64_t, long) {}
 // (XOR_i rotl(T[comp(b_i)], i) over them).  Every thread computes its own
 // segment (the first m-1 threads also the segments past the tile), the
 // values meet in LDS, and each thread combines m of them instead of rolling
-// K bytes twice.  kSegExtra bounds m - 1 (K <= 255, PER >= 8).
+// K bytes twice.  kSegExtra bounds m - 1: every K <= 255 at PER >= 8, and
+// K <= 132 at PER = 4 (larger K roll their starts).
 constexpr int kSegExtra = 32;
-struct SegScratch {
-    uint64_t pf[kBlock + kSegExtra], qf[kBlock + kSegExtra];
-    uint64_t pr[kBlock + kSegExtra], qr[kBlock + kSegExtra];
-    uint8_t bf[kBlock + kSegExtra], bq[kBlock + kSegExtra];   // non-ACGT bytes in the segment / its prefix
+template <int N>
+struct SegScratchT {
+    static constexpr int kSegs = N;
+    uint64_t pf[N], qf[N];
+    uint64_t pr[N], qr[N];
+    uint8_t bf[N], bq[N];   // non-ACGT bytes in the segment / its prefix
 };
+using SegScratch = SegScratchT<kBlock + kSegExtra>;
 
 // The op's LDS for stitched starts (free while the windows are visited), or
 // null for ops without it (those roll K bytes per start).
@@ -276,16 +281,18 @@ __device__ __forceinline__ SegScratch* seg_scratch(Op&, long) {
     return nullptr;
 }
 
-template <int PER, class Sh>
-__device__ __forceinline__ void seg_values(const Sh& sh, SegScratch* sg, int u, int L) {
-    static_assert(PER % 8 == 0, "segments are read 8 bytes at a time");
+template <int PER, class Sh, class Sg>
+__device__ __forceinline__ void seg_values(const Sh& sh, Sg* sg, int u, int L) {
+    static_assert(PER % 4 == 0, "segments are read 4 or 8 bytes at a time");
+    constexpr int kW = PER % 8 == 0 ? 8 : 4;
     uint64_t pf = 0, pr = 0, qf = 0, qr = 0;
     uint32_t bf = 0, bq = 0;
 #pragma unroll
-    for (int i0 = 0; i0 < PER; i0 += 8) {
-        const uint64_t w = *reinterpret_cast<const uint64_t*>(sh.bytes + u * PER + i0);
+    for (int i0 = 0; i0 < PER; i0 += kW) {
+        const uint64_t w = kW == 8 ? *reinterpret_cast<const uint64_t*>(sh.bytes + u * PER + i0)
+                                   : (uint64_t)*reinterpret_cast<const uint32_t*>(sh.bytes + u * PER + i0);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < kW; ++k) {
             const int i = i0 + k;
             const uint32_t c = (uint32_t)(w >> (8 * k)) & 0xFFu;
             pf = gt_rotl(pf, 1) ^ sh.tf[c];
@@ -306,9 +313,9 @@ __device__ __forceinline__ void seg_values(const Sh& sh, SegScratch* sg, int u, 
     sg->bq[u] = (uint8_t)bq;
 }
 
-template <int PER, class Op, class Sh>
+template <int PER, int BLK = kBlock, class Op, class Sh>
 __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
-    constexpr int kTile = kBlock * PER;
+    constexpr int kTile = BLK * PER;
     static_assert(sizeof(sh.bytes) >= (size_t)(kTile + kMaxK), "tile LDS too small for PER");
     const int tid = threadIdx.x;
     const int K = R.K;
@@ -333,10 +340,10 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
             const int nvec = nload >> 4;
             const uint4* src = reinterpret_cast<const uint4*>(R.seq + t0);
             uint4* dst = reinterpret_cast<uint4*>(sh.bytes);
-            for (int i = tid; i < nvec; i += kBlock) dst[i] = src[i];
-            for (int i = (nvec << 4) + tid; i < span; i += kBlock) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
+            for (int i = tid; i < nvec; i += BLK) dst[i] = src[i];
+            for (int i = (nvec << 4) + tid; i < span; i += BLK) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
         } else {
-            for (int i = tid; i < span; i += kBlock) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
+            for (int i = tid; i < span; i += BLK) sh.bytes[i] = i < nload ? R.seq[t0 + i] : 0;
         }
         if (tid == 0) {
             const int64_t g_end = (t0 + kTile) / kTileGrain;
@@ -353,17 +360,21 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
         const uint32_t rl = sh.r_lo, rh = sh.r_hi;
         const bool cached = sh.cached;
         if (cached) {
-            for (uint32_t i = tid; i < rh - rl + 2; i += kBlock) sh.roff[i] = R.off[rl + i];
+            for (uint32_t i = tid; i < rh - rl + 2; i += BLK) sh.roff[i] = R.off[rl + i];
         }
         __syncthreads();
 
         op.begin_tile(t0);
-        SegScratch* sg = seg_scratch(op, 0);
+        auto* sg = seg_scratch(op, 0);
         const int n_seg = (K + PER - 1) / PER;
+        if (sg) {
+            static_assert(std::remove_pointer_t<decltype(sg)>::kSegs >= BLK + kSegExtra, "segment scratch too small");
+            if (n_seg - 1 > kSegExtra) sg = nullptr;       // uniform: K too long to stitch at this PER
+        }
         if (sg) {
             const int L = K - (n_seg - 1) * PER;
             seg_values<PER>(sh, sg, tid, L);
-            if (tid < n_seg - 1) seg_values<PER>(sh, sg, kBlock + tid, L);
+            if (tid < n_seg - 1) seg_values<PER>(sh, sg, BLK + tid, L);
             __syncthreads();
         }
         const int64_t s0 = t0 + (int64_t)tid * PER;
@@ -1236,9 +1247,10 @@ __global__ void bin_kbase_kernel(const int64_t* off, const int64_t* kmer_off, ui
 }
 
 // Exclusive scan over the workgroup (one value per thread): a shuffle scan
-// per wave, then the wave totals through `part` (kBlock / 64 entries); the
+// per wave, then the wave totals through `part` (BLK / 64 entries); the
 // workgroup total goes to *total.  Ends with a barrier, so `part` can be
 // reused at once.
+template <int BLK = kBlock>
 __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* total = nullptr) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
@@ -1251,7 +1263,7 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < BLK / 64; ++w) {
         const uint32_t p = part[w];
         off += w < wave ? p : 0u;
         tot += p;
@@ -1261,15 +1273,16 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
     return off + x - v;
 }
 
+template <int BLK>
 struct BinL1Shared {
     uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
     uint32_t ord[kBinTile];
     union {
         uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
-        SegScratch seg;                           // stitched window starts (before end_tile)
+        SegScratchT<BLK + kSegExtra> seg;         // stitched window starts (before end_tile)
     };
     uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
-    uint32_t part[kBlock];
+    uint32_t part[BLK / 64];
 };
 static_assert(sizeof(int64_t) * (kReadCache + 1) >= sizeof(uint64_t) * 4 * kDigits1,
               "the first pass borrows the walker's read-offset cache for its match slots");
@@ -1278,18 +1291,22 @@ __device__ __forceinline__ uint32_t bin_d1(uint32_t g, uint32_t shift2) {
     return g == kNoBin ? (uint32_t)kDigits1 : (g >> kBinBits) >> shift2;
 }
 
-// First pass over one tile.  kU: the events of a digit may leave the tile in
-// any order (plain inserts, whose replay depends only on each bin's event
-// count and first window); otherwise they keep window order (insert_and_query
-// and diginorm replay events in order).
-template <bool kU>
+// First pass over one tile of kBinTile starts, BLK threads of PER starts
+// each.  kU: the events of a digit may leave the tile in any order (plain
+// inserts, whose replay depends only on each bin's event count and first
+// window); otherwise they keep window order (insert_and_query and diginorm
+// replay events in order; that sort ranks per wave, four waves).
+template <bool kU, int BLK, int PER>
 struct BinL1Op {
+    static_assert(BLK * PER == kBinTile, "a first-pass tile is kBinTile starts");
+    static_assert(BLK >= kDigits1, "one thread per first digit");
+    static_assert(kU || BLK == 4 * 64, "the stable sort ranks per wave over four waves");
     Tables T;
     BinPlan P;
     const int64_t* kmer_off;
     int canonical;
     uint64_t kbase;
-    BinL1Shared* sh;
+    BinL1Shared<BLK>* sh;
     uint64_t* slots;          // 4 x kDigits1 match slots (the walker's read-offset cache, idle here)
     int64_t b_lo;             // first start of the sub-batch (tiles may begin before it)
     int64_t t0 = 0;
@@ -1298,10 +1315,10 @@ struct BinL1Op {
 
     __device__ void begin_tile(int64_t tile_base) {
         t0 = tile_base;
-        const int w0 = threadIdx.x * kBinPer;
+        const int w0 = threadIdx.x * PER;
         for (uint32_t t = 0; t < P.n_tables; ++t)
-            for (int j = 0; j < kBinPer; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
-        if (kU) sh->wpos[0][threadIdx.x] = 0;      // digit counters (kDigits1 == kBlock)
+            for (int j = 0; j < PER; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
+        if (kU && threadIdx.x < kDigits1) sh->wpos[0][threadIdx.x] = 0;      // digit counters
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
@@ -1323,20 +1340,20 @@ struct BinL1Op {
 
     __device__ void flush() {}
 
-    __device__ SegScratch* seg_scratch() { return &sh->seg; }
+    __device__ SegScratchT<BLK + kSegExtra>* seg_scratch() { return &sh->seg; }
 
     // Sort of the tile's events by first digit, built in LDS and written out
     // as one contiguous region.  Order-free: an LDS counter per digit gives
     // each event its slot.
     __device__ void end_tile_unordered(int64_t tile) {
         __syncthreads();
-        constexpr int kQ = (int)(kBinMaxTables * kBinTile / kBlock);
+        constexpr int kQ = (int)(kBinMaxTables * kBinTile / BLK);
         const uint32_t ne = P.n_tables * kBinTile;
         uint32_t* cnt = sh->wpos[0];
         uint32_t di[kQ];                                   // digit << 16 | slot in the digit, or ~0
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const uint32_t e = (uint32_t)q * kBlock + threadIdx.x;
+            const uint32_t e = (uint32_t)q * BLK + threadIdx.x;
             di[q] = 0xFFFFFFFFu;
             if (e < ne) {
                 const uint32_t d = bin_d1(sh->g[e], P.shift2);
@@ -1345,17 +1362,19 @@ struct BinL1Op {
         }
         __syncthreads();
         const uint32_t lt1 = (uint32_t)(tile - P.tile0);
-        const int d = threadIdx.x;                         // kDigits1 == kBlock
+        const int d = threadIdx.x;                         // threads below kDigits1 own a digit
         uint32_t n_ev;
-        const uint32_t start = block_exclusive_u32(cnt[d], sh->part, &n_ev);
-        P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
-        if (d == kBlock - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
-        cnt[d] = start;
+        const uint32_t start = block_exclusive_u32<BLK>(d < kDigits1 ? cnt[d] : 0u, sh->part, &n_ev);
+        if (d < kDigits1) {
+            P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
+            cnt[d] = start;
+        }
+        if (d == kDigits1 - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kQ; ++q)
             if (di[q] != 0xFFFFFFFFu)
-                sh->perm[cnt[di[q] >> 16] + (di[q] & 0xFFFFu)] = (uint16_t)((uint32_t)q * kBlock + threadIdx.x);
+                sh->perm[cnt[di[q] >> 16] + (di[q] & 0xFFFFu)] = (uint16_t)((uint32_t)q * BLK + threadIdx.x);
         __syncthreads();
         write_sorted(lt1, n_ev);
     }
@@ -1368,16 +1387,16 @@ struct BinL1Op {
         uint64_t* dst = P.ent1 + r0;
         uint16_t* dst2 = P.dig2 + r0;
         const uint32_t mask2 = (1u << P.shift2) - 1u;
-        for (uint32_t i0 = 0; i0 < n_ev; i0 += 4 * kBlock) {
+        for (uint32_t i0 = 0; i0 < n_ev; i0 += 4 * BLK) {
             uint32_t e[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                const uint32_t i = i0 + q * BLK + threadIdx.x;
                 e[q] = i < n_ev ? sh->perm[i] : 0u;
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t i = i0 + q * kBlock + threadIdx.x;
+                const uint32_t i = i0 + q * BLK + threadIdx.x;
                 if (i < n_ev) {
                     const uint32_t g = sh->g[e[q]];
                     dst[i] = ((uint64_t)g << 32) | sh->ord[e[q] & (kBinTile - 1)];
@@ -1445,12 +1464,19 @@ struct BinL1Op {
     }
 };
 
+// Plain inserts run 512 threads of 4 starts per tile (two workgroups and four
+// waves per SIMD); the stable sort keeps 256 threads of 8.
 template <bool kU>
-__global__ __launch_bounds__(kBlock) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
+constexpr int bin_l1_block() { return kU ? 2 * kBlock : kBlock; }
+
+template <bool kU>
+__global__ __launch_bounds__(bin_l1_block<kU>()) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
+    constexpr int BLK = bin_l1_block<kU>();
+    constexpr int PER = kBinTile / BLK;
     __shared__ TileSharedT<kBinTile> sh;
-    __shared__ BinL1Shared bs;
-    BinL1Op<kU> op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
-    walk_tiles<kBinPer>(R, sh, op);
+    __shared__ BinL1Shared<BLK> bs;
+    BinL1Op<kU, BLK, PER> op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
+    walk_tiles<PER, BLK>(R, sh, op);
 }
 
 // Second pass: work item (d1, group) = the d1 runs of kBinGroup consecutive
@@ -2644,9 +2670,9 @@ void launch_bin_kbase(const Reads& R, int64_t b0, uint64_t* out, hipStream_t s) 
 void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, hipStream_t s) {
     const int g = (int)std::min<int64_t>(std::max<int64_t>(P.tiles1, 1), grid);
     if (P.unordered)
-        bin_l1_kernel<true><<<g, kBlock, 0, s>>>(R, T, P);
+        bin_l1_kernel<true><<<g, bin_l1_block<true>(), 0, s>>>(R, T, P);
     else
-        bin_l1_kernel<false><<<g, kBlock, 0, s>>>(R, T, P);
+        bin_l1_kernel<false><<<g, bin_l1_block<false>(), 0, s>>>(R, T, P);
 }
 
 void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {

@@ -347,6 +347,41 @@ def test_binned_insert_vs_oracle(oracle, kind, sub_batch, monkeypatch):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize('K', [1, 3, 4, 5, 8, 31, 33, 100, 131, 132, 133, 200, 255])
+def test_binned_insert_every_k_vs_oracle(oracle, K, monkeypatch):
+    """The binned plain insert for K from 1 to 255: the first pass stitches
+    window starts from 4-byte segments up to K = 132 and rolls them beyond;
+    tables, per-read n_new, flags and counters equal the sequential oracle's
+    on reads of every length around K, with N bases."""
+    monkeypatch.setenv('GOETIA_AMD_BINNED', '1')
+    rng = np.random.default_rng(1000 + K)
+    reads = []
+    for _ in range(400):
+        L = int(rng.choice([0, K - 1, K, K + 1, K + 7, 150, 600, 3000]))
+        r = bytearray(random_dna(rng, max(L, 0)).encode())
+        for j in np.nonzero(rng.random(len(r)) < 0.002)[0]:
+            r[j] = ord('N')
+        reads.append(bytes(r))
+    reads += [b'ACGT' * 100] * 20
+    for kind in (1, 2):
+        sizes = oracle.primes_near_x(4, 200_000 if kind == 2 else 600_000)
+        store = (lg.ByteStorage if kind == 2 else lg.BitStorage)(sizes)
+        g = lg.dBG[type(store), lg.CanLemireShifter].build(store, K)
+        g.mode = N.MODE_SKIP_N
+        n_new, flags = g.insert_sequences(ReadBatch.from_strings(reads), new_flags=True)
+        o = oracle.Storage(kind, sizes)
+        o_new, o_flags = [], []
+        for r in reads:
+            _, nn, fl = o.insert_sequence(r, K, True, oracle.MODE_SKIP_N, flags=True)
+            o_new.append(nn)
+            o_flags.append(fl)
+        np.testing.assert_array_equal(n_new, o_new)
+        np.testing.assert_array_equal(flags, np.concatenate(o_flags).astype(bool))
+        assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+        for a, b in zip(g.get_raw(), o.tables()):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize('sub_batch', [0, 8192 * 3])
 @pytest.mark.parametrize('n_tables', [1, 3, 4, 5, 16])
 def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatch):

@@ -2,14 +2,15 @@
 gt_kernels.hip) equal CyclicHash rolled over the K bytes
 (cyclichash.h:85-108: fw = XOR_k rotl(T[b_k], K-1-k); the reverse-complement
 hash rc = XOR_k rotl(T[comp(b_k)], k)), for every K the library takes and the
-8-byte segments of the first binned pass, with the rotation amounts reduced
-as the kernel reduces them (an unsigned 32-bit value masked to 6 bits)."""
+segments of the first binned pass (8 bytes for the ordered sort, 4 bytes for
+plain inserts, which stitch K <= 132 and roll longer K), with the rotation
+amounts reduced as the kernel reduces them (an unsigned 32-bit value masked
+to 6 bits)."""
 import random
 
 import pytest
 
 M64 = (1 << 64) - 1
-PER = 8
 
 
 def rotl(x, r):
@@ -17,7 +18,7 @@ def rotl(x, r):
     return ((x << r) | (x >> (64 - r))) & M64 if r else x
 
 
-def seg_values(by, u, L, T, TR):
+def seg_values(by, u, L, T, TR, PER):
     pf = pr = qf = qr = 0
     for i in range(PER):
         c = by[u * PER + i]
@@ -28,8 +29,12 @@ def seg_values(by, u, L, T, TR):
     return pf, pr, qf, qr
 
 
-@pytest.mark.parametrize('K', list(range(1, 34)) + [63, 64, 65, 127, 128, 200, 254, 255])
-def test_stitched_start_equals_rolled(K):
+CASES = ([(8, K) for K in list(range(1, 34)) + [63, 64, 65, 127, 128, 200, 254, 255]]
+         + [(4, K) for K in list(range(1, 34)) + [63, 64, 65, 127, 128, 131, 132]])
+
+
+@pytest.mark.parametrize('PER,K', CASES)
+def test_stitched_start_equals_rolled(PER, K):
     rng = random.Random(K)
     T = [rng.getrandbits(64) for _ in range(256)]
     TR = [rng.getrandbits(64) for _ in range(256)]
@@ -37,7 +42,8 @@ def test_stitched_start_equals_rolled(K):
     L = K - (n_seg - 1) * PER
     n_threads = 40
     by = [rng.randrange(256) for _ in range(PER * (n_threads + n_seg))]
-    S = [seg_values(by, u, L, T, TR) for u in range(n_threads + n_seg - 1)]
+    assert n_seg - 1 <= 32          # the kernel's segment scratch holds 32 extra segments
+    S = [seg_values(by, u, L, T, TR, PER) for u in range(n_threads + n_seg - 1)]
     for t in range(n_threads):
         li = t * PER
         fw = rc = 0
