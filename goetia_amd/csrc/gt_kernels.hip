@@ -1989,10 +1989,11 @@ __global__ __launch_bounds__(kBlock) void bin_apply_kernel(BinPlan P, Tables T, 
 // count (count-min) and the smallest ordinal; the event holding that ordinal
 // then settles the bin and resets both for the next bucket.  The next
 // bucket's slice and first events are loaded while this one is replayed.
-template <bool kBytes>
+template <bool kBytes, int BLK>
 struct ApplyNext {
-    static constexpr int kQ = 16;                                         // events per thread in registers
-    static constexpr int kW = (kBytes ? kBucketBins / 4 : kBucketBins / 32) / kBlock;   // slice words per thread
+    static constexpr int kQ = 4096 / BLK;                                 // events per thread in registers
+    static constexpr int kWords = kBytes ? kBucketBins / 4 : kBucketBins / 32;         // slice words
+    static constexpr int kW = (kWords + BLK - 1) / BLK;                   // slice words per thread
     uint32_t b, lo, hi, t, nw;
     uint32_t* src;
     uint32_t w[kW];
@@ -2006,8 +2007,8 @@ __device__ __forceinline__ uint32_t next_busy_bucket(const BinPlan& P, uint32_t 
     return b < P.n_buckets ? b : P.n_buckets;
 }
 
-template <bool kBytes>
-__device__ __forceinline__ void apply_fetch(const BinPlan& P, const Tables& T, uint32_t b, ApplyNext<kBytes>& x) {
+template <bool kBytes, int BLK>
+__device__ __forceinline__ void apply_fetch(const BinPlan& P, const Tables& T, uint32_t b, ApplyNext<kBytes, BLK>& x) {
     x.b = b;
     if (b >= P.n_buckets) return;
     x.lo = P.cnt2[(size_t)b * P.n_groups];
@@ -2030,39 +2031,44 @@ __device__ __forceinline__ void apply_fetch(const BinPlan& P, const Tables& T, u
     x.nw = kBytes ? (nb + 3u) / 4u : (nb + 31u) / 32u;
     x.src = reinterpret_cast<uint32_t*>(kBytes ? base + bin0 : base + bin0 / 8);
 #pragma unroll
-    for (int k = 0; k < ApplyNext<kBytes>::kW; ++k) {
-        const uint32_t w = (uint32_t)k * kBlock + threadIdx.x;
+    for (int k = 0; k < ApplyNext<kBytes, BLK>::kW; ++k) {
+        const uint32_t w = (uint32_t)k * BLK + threadIdx.x;
         x.w[k] = w < x.nw ? x.src[w] : 0u;
     }
 #pragma unroll
-    for (int q = 0; q < ApplyNext<kBytes>::kQ; ++q) {
-        const uint32_t i = x.lo + (uint32_t)q * kBlock + threadIdx.x;
+    for (int q = 0; q < ApplyNext<kBytes, BLK>::kQ; ++q) {
+        const uint32_t i = x.lo + (uint32_t)q * BLK + threadIdx.x;
         x.ev[q] = i < x.hi ? P.ent2[i] : 0;
     }
 }
 
+// Order-free replay, one bucket per workgroup at a time; 512 threads (four
+// waves per SIMD at two workgroups per CU).
+constexpr int kApplyBlock = 2 * kBlock;
+
 template <bool kBytes>
-__global__ __launch_bounds__(kBlock) void bin_apply_u_kernel(BinPlan P, Tables T, uint64_t* counters) {
-    using Next = ApplyNext<kBytes>;
+__global__ __launch_bounds__(kApplyBlock) void bin_apply_u_kernel(BinPlan P, Tables T, uint64_t* counters) {
+    constexpr int BLK = kApplyBlock;
+    using Next = ApplyNext<kBytes, BLK>;
     constexpr int kQ = Next::kQ;
     __shared__ uint32_t slice[kBytes ? kBucketBins / 4 : kBucketBins / 32];
     __shared__ uint32_t first[kBucketBins];               // smallest ordinal per bin, ~0 = none
     __shared__ uint32_t hits[kBytes ? kBucketBins : 1];   // events per bin
     const uint64_t kb = *P.kbase;
     uint8_t* sb = reinterpret_cast<uint8_t*>(slice);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kBucketBins; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBucketBins; i += BLK) {
         first[i] = 0xFFFFFFFFu;
         if (kBytes) hits[i] = 0;
     }
     uint64_t occ = 0;
     Next cur, nxt;
-    apply_fetch<kBytes>(P, T, next_busy_bucket(P, blockIdx.x, gridDim.x), cur);
+    apply_fetch<kBytes, BLK>(P, T, next_busy_bucket(P, blockIdx.x, gridDim.x), cur);
     while (cur.b < P.n_buckets) {
-        apply_fetch<kBytes>(P, T, next_busy_bucket(P, cur.b + gridDim.x, gridDim.x), nxt);
+        apply_fetch<kBytes, BLK>(P, T, next_busy_bucket(P, cur.b + gridDim.x, gridDim.x), nxt);
         __syncthreads();                                  // the previous bucket is written back
 #pragma unroll
         for (int k = 0; k < Next::kW; ++k) {
-            const uint32_t w = (uint32_t)k * kBlock + threadIdx.x;
+            const uint32_t w = (uint32_t)k * BLK + threadIdx.x;
             if (w < cur.nw) slice[w] = cur.w[k];
         }
         auto hit = [&](uint64_t e) {
@@ -2072,8 +2078,8 @@ __global__ __launch_bounds__(kBlock) void bin_apply_u_kernel(BinPlan P, Tables T
         };
 #pragma unroll
         for (int q = 0; q < kQ; ++q)
-            if (cur.lo + (uint32_t)q * kBlock + threadIdx.x < cur.hi) hit(cur.ev[q]);
-        for (uint32_t i = cur.lo + (uint32_t)kQ * kBlock + threadIdx.x; i < cur.hi; i += kBlock) hit(P.ent2[i]);
+            if (cur.lo + (uint32_t)q * BLK + threadIdx.x < cur.hi) hit(cur.ev[q]);
+        for (uint32_t i = cur.lo + (uint32_t)kQ * BLK + threadIdx.x; i < cur.hi; i += BLK) hit(P.ent2[i]);
         __syncthreads();
         auto settle = [&](uint64_t e) {
             const uint32_t key = (uint32_t)(e >> 32) & (kBucketBins - 1);
@@ -2098,10 +2104,10 @@ __global__ __launch_bounds__(kBlock) void bin_apply_u_kernel(BinPlan P, Tables T
         };
 #pragma unroll
         for (int q = 0; q < kQ; ++q)
-            if (cur.lo + (uint32_t)q * kBlock + threadIdx.x < cur.hi) settle(cur.ev[q]);
-        for (uint32_t i = cur.lo + (uint32_t)kQ * kBlock + threadIdx.x; i < cur.hi; i += kBlock) settle(P.ent2[i]);
+            if (cur.lo + (uint32_t)q * BLK + threadIdx.x < cur.hi) settle(cur.ev[q]);
+        for (uint32_t i = cur.lo + (uint32_t)kQ * BLK + threadIdx.x; i < cur.hi; i += BLK) settle(P.ent2[i]);
         __syncthreads();
-        for (uint32_t w = threadIdx.x; w < cur.nw; w += kBlock) cur.src[w] = slice[w];
+        for (uint32_t w = threadIdx.x; w < cur.nw; w += BLK) cur.src[w] = slice[w];
         cur = nxt;
     }
     add_counter(counters + kNOccupied, occ);
@@ -2693,9 +2699,9 @@ void launch_bin_apply(const BinPlan& P, const Tables& T, uint64_t* counters, int
     if (P.unordered && !P.vq) {
         const int g = (int)std::min<int64_t>(std::max<int64_t>(P.n_buckets, 1), grid);
         if (T.kind == kByte)
-            bin_apply_u_kernel<true><<<g, kBlock, 0, s>>>(P, T, counters);
+            bin_apply_u_kernel<true><<<g, kApplyBlock, 0, s>>>(P, T, counters);
         else
-            bin_apply_u_kernel<false><<<g, kBlock, 0, s>>>(P, T, counters);
+            bin_apply_u_kernel<false><<<g, kApplyBlock, 0, s>>>(P, T, counters);
         return;
     }
     const int g = (int)std::min<int64_t>(std::max<int64_t>((P.n_buckets + 3) / 4, 1), grid);
