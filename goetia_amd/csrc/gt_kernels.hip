@@ -1273,7 +1273,11 @@ __device__ uint32_t block_exclusive_u32(uint32_t v, uint32_t* part, uint32_t* to
     return off + x - v;
 }
 
-template <int BLK>
+// Waves that rank a first-pass tile's events in the stable sort (each a
+// contiguous quarter, with LDS match slots of kDigits1 words each).
+constexpr int kRankWaves = 4;
+
+template <int BLK, bool kU>
 struct BinL1Shared {
     uint32_t g[kBinMaxTables * kBinTile];   // table-major: [table][window]; kNoBin = no event
     uint32_t ord[kBinTile];
@@ -1281,11 +1285,12 @@ struct BinL1Shared {
         uint16_t perm[kBinMaxTables * kBinTile];  // region position -> event index in g
         SegScratchT<BLK + kSegExtra> seg;         // stitched window starts (before end_tile)
     };
-    uint32_t wpos[4][kDigits1 + 1];          // per wave: digit counts, then running positions
+    uint32_t cnt[kU ? kDigits1 : 1];                 // order-free: events per digit, then starts
+    uint32_t wpos[kU ? 1 : kRankWaves][kDigits1 + 1];  // stable: per ranking wave, digit counts then positions
     uint32_t part[BLK / 64];
 };
-static_assert(sizeof(int64_t) * (kReadCache + 1) >= sizeof(uint64_t) * 4 * kDigits1,
-              "the first pass borrows the walker's read-offset cache for its match slots");
+static_assert(sizeof(int64_t) * (kReadCache + 1) >= sizeof(uint64_t) * kRankWaves * kDigits1,
+              "the stable sort borrows the walker's read-offset cache for its match slots");
 
 __device__ __forceinline__ uint32_t bin_d1(uint32_t g, uint32_t shift2) {
     return g == kNoBin ? (uint32_t)kDigits1 : (g >> kBinBits) >> shift2;
@@ -1295,19 +1300,19 @@ __device__ __forceinline__ uint32_t bin_d1(uint32_t g, uint32_t shift2) {
 // each.  kU: the events of a digit may leave the tile in any order (plain
 // inserts, whose replay depends only on each bin's event count and first
 // window); otherwise they keep window order (insert_and_query and diginorm
-// replay events in order; that sort ranks per wave, four waves).
+// replay events in order).
 template <bool kU, int BLK, int PER>
 struct BinL1Op {
     static_assert(BLK * PER == kBinTile, "a first-pass tile is kBinTile starts");
     static_assert(BLK >= kDigits1, "one thread per first digit");
-    static_assert(kU || BLK == 4 * 64, "the stable sort ranks per wave over four waves");
+    static_assert(kU || BLK >= kRankWaves * 64, "the stable sort ranks with kRankWaves waves");
     Tables T;
     BinPlan P;
     const int64_t* kmer_off;
     int canonical;
     uint64_t kbase;
-    BinL1Shared<BLK>* sh;
-    uint64_t* slots;          // 4 x kDigits1 match slots (the walker's read-offset cache, idle here)
+    BinL1Shared<BLK, kU>* sh;
+    uint64_t* slots;          // kRankWaves x kDigits1 match slots (the walker's read-offset cache, idle here)
     int64_t b_lo;             // first start of the sub-batch (tiles may begin before it)
     int64_t t0 = 0;
     uint32_t kr = 0xFFFFFFFFu;  // read whose packed-index offset is cached in kd
@@ -1318,7 +1323,7 @@ struct BinL1Op {
         const int w0 = threadIdx.x * PER;
         for (uint32_t t = 0; t < P.n_tables; ++t)
             for (int j = 0; j < PER; ++j) sh->g[t * kBinTile + w0 + j] = kNoBin;
-        if (kU && threadIdx.x < kDigits1) sh->wpos[0][threadIdx.x] = 0;      // digit counters
+        if (kU && threadIdx.x < kDigits1) sh->cnt[threadIdx.x] = 0;      // digit counters
     }
 
     __device__ void operator()(int64_t s, uint32_t r, int64_t rstart, uint64_t fw, uint64_t rc, bool valid) {
@@ -1349,7 +1354,7 @@ struct BinL1Op {
         __syncthreads();
         constexpr int kQ = (int)(kBinMaxTables * kBinTile / BLK);
         const uint32_t ne = P.n_tables * kBinTile;
-        uint32_t* cnt = sh->wpos[0];
+        uint32_t* cnt = sh->cnt;
         uint32_t di[kQ];                                   // digit << 16 | slot in the digit, or ~0
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
@@ -1413,68 +1418,77 @@ struct BinL1Op {
             end_tile_stable(tile);
     }
 
-    // Stable sort of the tile's events by first digit: each wave ranks a
-    // quarter of the events, the sorted order is built in LDS and the region
-    // is written out contiguously.
+    // Stable sort of the tile's events by first digit: waves 0..kRankWaves-1
+    // each rank a contiguous quarter of the events (the other waves wait at
+    // the barriers), the sorted order is built in LDS and the region is
+    // written out contiguously by every thread.
     __device__ void end_tile_stable(int64_t tile) {
         __syncthreads();
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const bool ranks = wave < kRankWaves;
         const uint64_t lt = lanes_below();
-        const int per_wave = (int)(P.n_tables * kBinTile) / 4;
+        const int per_wave = (int)(P.n_tables * kBinTile) / kRankWaves;   // a multiple of 512
         const int e0 = wave * per_wave;
-        uint32_t* wc = sh->wpos[wave];
-        uint64_t* sl = slots + wave * kDigits1;
-        for (int d = lane; d <= kDigits1; d += 64) wc[d] = 0;
-        __builtin_amdgcn_wave_barrier();
-        for (int c = 0; c < per_wave; c += 4 * 64) {        // per_wave is a multiple of 512
-            uint32_t d[4];
+        uint32_t* wc = sh->wpos[ranks ? wave : 0];
+        uint64_t* sl = slots + (ranks ? wave : 0) * kDigits1;
+        if (ranks) {
+            for (int d = lane; d <= kDigits1; d += 64) wc[d] = 0;
+            __builtin_amdgcn_wave_barrier();
+            for (int c = 0; c < per_wave; c += 4 * 64) {
+                uint32_t d[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[e0 + c + q * 64 + lane], P.shift2);
+                for (int q = 0; q < 4; ++q) d[q] = bin_d1(sh->g[e0 + c + q * 64 + lane], P.shift2);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (d[q] != (uint32_t)kDigits1) atomicAdd(&wc[d[q]], 1u);
+                for (int q = 0; q < 4; ++q)
+                    if (d[q] != (uint32_t)kDigits1) atomicAdd(&wc[d[q]], 1u);
+            }
         }
         __syncthreads();
         const uint32_t lt1 = (uint32_t)(tile - P.tile0);
-        const int d = threadIdx.x;                         // kDigits1 == kBlock
-        const uint32_t tot = (uint32_t)sh->wpos[0][d] + sh->wpos[1][d] + sh->wpos[2][d] + sh->wpos[3][d];
+        const int d = threadIdx.x;                         // threads below kDigits1 own a digit
+        uint32_t tot = 0;
+        if (d < kDigits1)
+            for (int w = 0; w < kRankWaves; ++w) tot += sh->wpos[w][d];
         uint32_t n_ev;
-        const uint32_t start = block_exclusive_u32(tot, sh->part, &n_ev);
-        P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
-        if (d == kBlock - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
-        uint32_t p = start;
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t c = sh->wpos[w][d];
-            sh->wpos[w][d] = p;
-            p += c;
+        const uint32_t start = block_exclusive_u32<BLK>(tot, sh->part, &n_ev);
+        if (d < kDigits1) {
+            P.off1[(size_t)d * P.tiles1 + lt1] = (uint16_t)start;
+            uint32_t p = start;
+            for (int w = 0; w < kRankWaves; ++w) {
+                const uint32_t c = sh->wpos[w][d];
+                sh->wpos[w][d] = p;
+                p += c;
+            }
         }
+        if (d == kDigits1 - 1) P.off1[(size_t)kDigits1 * P.tiles1 + lt1] = (uint16_t)n_ev;
         __syncthreads();
-        for (int c = 0; c < per_wave; c += 64) {
-            const int e = e0 + c + lane;
-            const uint32_t dd = bin_d1(sh->g[e], P.shift2);
-            const bool act = dd != (uint32_t)kDigits1;
-            const uint64_t m = match_lds(sl, dd, act);
-            if (act) sh->perm[wc[dd] + (uint32_t)__popcll(m & lt)] = (uint16_t)e;
-            __builtin_amdgcn_wave_barrier();
-            if (act && !(m & lt)) wc[dd] += (uint32_t)__popcll(m);
-            __builtin_amdgcn_wave_barrier();
+        if (ranks) {
+            for (int c = 0; c < per_wave; c += 64) {
+                const int e = e0 + c + lane;
+                const uint32_t dd = bin_d1(sh->g[e], P.shift2);
+                const bool act = dd != (uint32_t)kDigits1;
+                const uint64_t m = match_lds(sl, dd, act);
+                if (act) sh->perm[wc[dd] + (uint32_t)__popcll(m & lt)] = (uint16_t)e;
+                __builtin_amdgcn_wave_barrier();
+                if (act && !(m & lt)) wc[dd] += (uint32_t)__popcll(m);
+                __builtin_amdgcn_wave_barrier();
+            }
         }
         __syncthreads();
         write_sorted(lt1, n_ev);
     }
 };
 
-// Plain inserts run 512 threads of 4 starts per tile (two workgroups and four
-// waves per SIMD); the stable sort keeps 256 threads of 8.
-template <bool kU>
-constexpr int bin_l1_block() { return kU ? 2 * kBlock : kBlock; }
+// 512 threads of 4 starts per tile: two workgroups (their LDS) and four
+// waves per SIMD.
+constexpr int kBinL1Block = 2 * kBlock;
 
 template <bool kU>
-__global__ __launch_bounds__(bin_l1_block<kU>()) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
-    constexpr int BLK = bin_l1_block<kU>();
+__global__ __launch_bounds__(kBinL1Block) void bin_l1_kernel(Reads R, Tables T, BinPlan P) {
+    constexpr int BLK = kBinL1Block;
     constexpr int PER = kBinTile / BLK;
     __shared__ TileSharedT<kBinTile> sh;
-    __shared__ BinL1Shared<BLK> bs;
+    __shared__ BinL1Shared<BLK, kU> bs;
     BinL1Op<kU, BLK, PER> op{T, P, R.kmer_off, R.canonical, *P.kbase, &bs, reinterpret_cast<uint64_t*>(sh.roff), R.b0};
     walk_tiles<PER, BLK>(R, sh, op);
 }
@@ -2676,9 +2690,9 @@ void launch_bin_kbase(const Reads& R, int64_t b0, uint64_t* out, hipStream_t s) 
 void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, hipStream_t s) {
     const int g = (int)std::min<int64_t>(std::max<int64_t>(P.tiles1, 1), grid);
     if (P.unordered)
-        bin_l1_kernel<true><<<g, bin_l1_block<true>(), 0, s>>>(R, T, P);
+        bin_l1_kernel<true><<<g, kBinL1Block, 0, s>>>(R, T, P);
     else
-        bin_l1_kernel<false><<<g, bin_l1_block<false>(), 0, s>>>(R, T, P);
+        bin_l1_kernel<false><<<g, kBinL1Block, 0, s>>>(R, T, P);
 }
 
 void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
