@@ -336,7 +336,8 @@ int big_query(const BigMapDev& M, const uint64_t* fw, const uint64_t* rc, const 
 int big_insert_query(const BigMapDev& M, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, int canonical,
                      const uint8_t* full, int16_t* counts, uint64_t n, int cus, hipStream_t s) {
     if (n == 0) return GT_OK;
-    if (n >= 0xFFFFFFFFULL) return set_error(GT_ERR_INVALID_ARG, "bigcount insert_and_query: too many windows");
+    if (n > 0x7FFFFFFFULL)   // the selection, sort and scans take int counts
+        return set_error(GT_ERR_INVALID_ARG, "bigcount insert_and_query: more than 2^31 - 1 windows in one call");
     Tmp sel, idx, nsel;
     BIG_TRY(sel.alloc(n));
     BIG_TRY(idx.alloc(n * 4));

@@ -808,6 +808,24 @@ int binned_insert_impl(gt_storage* st, Scratch& sc, const Reads& R, uint64_t* n_
 // before it, so the windows that found every table full are known in window
 // order; insert_and_query takes its bigcount values from them before the map
 // takes the batch (bytestorage.cc:60-150).  Synchronises the stream.
+// The bigcount steps of insert_and_query over n windows whose table counts
+// and full flags are known: answered in window chunks, in order, a chunk's
+// full windows joining the map before the next chunk is answered (goetia's
+// sequential order).  The selection and sort inside take int counts, so a
+// chunk stays below 2^31 windows (GOETIA_AMD_SUB_BATCH caps it further).
+int bigcount_iq_chunks(gt_storage* st, const uint64_t* fw, const uint64_t* rc, const uint8_t* valid, int canonical,
+                       const uint8_t* full, int16_t* counts, uint64_t n, hipStream_t s) {
+    uint64_t chunk = 1ULL << 28;
+    if (const uint64_t cap = sub_batch_cap()) chunk = std::min(chunk, cap);
+    for (uint64_t w0 = 0; w0 < n; w0 += chunk) {
+        const uint64_t m = std::min(chunk, n - w0);
+        GT_TRY(big_insert_query(st->big, fw + w0, rc ? rc + w0 : nullptr, valid ? valid + w0 : nullptr, canonical,
+                                full + w0, counts + w0, m, g_dev.cus, s));
+        GT_TRY(big_insert(st->big, fw + w0, rc ? rc + w0 : nullptr, canonical, full + w0, m, g_dev.cus, s));
+    }
+    return GT_OK;
+}
+
 int bigcount_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_flags, int16_t* iq_counts,
                          hipStream_t s) {
     if (n_new && R.n_reads) launch_fill_u64(n_new, R.n_reads, 0, s);
@@ -833,10 +851,11 @@ int bigcount_insert_impl(gt_storage* st, Reads R, uint64_t* n_new, uint8_t* new_
                                (uint64_t)nk, R.canonical, iq_counts, flags, st->counters.as<uint64_t>(), g_dev.cus, s,
                                full.as<uint8_t>()));
     if (iq_counts)
-        GT_TRY(big_insert_query(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), R.canonical,
-                                full.as<uint8_t>(), iq_counts, (uint64_t)nk, g_dev.cus, s));
-    GT_TRY(big_insert(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), R.canonical, full.as<uint8_t>(), (uint64_t)nk,
-                      g_dev.cus, s));
+        GT_TRY(bigcount_iq_chunks(st, fw.as<uint64_t>(), rc.as<uint64_t>(), valid.as<uint8_t>(), R.canonical,
+                                  full.as<uint8_t>(), iq_counts, (uint64_t)nk, s));
+    else
+        GT_TRY(big_insert(st->big, fw.as<uint64_t>(), rc.as<uint64_t>(), R.canonical, full.as<uint8_t>(),
+                          (uint64_t)nk, g_dev.cus, s));
     if (n_new) {
         // per-read sums of the flags (n_unique was counted by the sort-based pass)
         GT_TRY(junk.ensure(kNCounters * sizeof(uint64_t)));
@@ -2113,11 +2132,9 @@ int gt_insert_and_query_hashes(gt_storage* st, const uint64_t* hashes, uint64_t 
         }
         GT_TRY(sorted_insert_query(st->tables_view(), st->h_out3.as<uint64_t>(), nullptr, nullptr, n, 0,
                                    st->h_out1.as<int16_t>(), nullptr, st->counters.as<uint64_t>(), g_dev.cus, s, fl));
-        if (fl) {
-            GT_TRY(big_insert_query(st->big, st->h_out3.as<uint64_t>(), nullptr, nullptr, 0, fl,
-                                    st->h_out1.as<int16_t>(), n, g_dev.cus, s));
-            GT_TRY(big_insert(st->big, st->h_out3.as<uint64_t>(), nullptr, 0, fl, n, g_dev.cus, s));
-        }
+        if (fl)
+            GT_TRY(bigcount_iq_chunks(st, st->h_out3.as<uint64_t>(), nullptr, nullptr, 0, fl, st->h_out1.as<int16_t>(),
+                                      n, s));
         GT_TRY(stage_out(counts, st->h_out1, n, s));
         GT_HIP(hipStreamSynchronize(s));
     } else {

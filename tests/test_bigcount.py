@@ -133,10 +133,30 @@ def test_device_vs_reference(ci):
 
 
 @pytest.mark.gpu
-def test_device_per_hash_calls_match_oracle(oracle):
-    """Per-k-mer insert / query / insert_and_query on hash values (the
-    storage API) with bigcount on equal the oracle's, across the wrap."""
+@pytest.mark.parametrize('ci', range(len(CASES)))
+def test_device_insert_and_query_in_window_chunks(ci, monkeypatch):
+    """insert_and_query with bigcount answers in window chunks, each chunk's
+    full windows joining the map before the next is answered; chunks of
+    8192 windows (GOETIA_AMD_SUB_BATCH) give goetia's results."""
     from goetia_amd import libgoetia as lg
+    monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', '8192')
+    c = CASES[ci]
+    g, _ = _graph(lg, c)
+    for b, want in zip(c['batches'], c['insert_and_query']):
+        got = g.insert_and_query_sequences(b['reads'])
+        assert got[:1000].tolist() == want['head']
+        assert digest(got) == want['sha256']
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('sub_batch', [None, '8192'])
+def test_device_per_hash_calls_match_oracle(oracle, sub_batch, monkeypatch):
+    """Per-k-mer insert / query / insert_and_query on hash values (the
+    storage API) with bigcount on equal the oracle's, across the wrap, also
+    answered in chunks of 8192 hashes."""
+    from goetia_amd import libgoetia as lg
+    if sub_batch:
+        monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', sub_batch)
     sizes = oracle.primes_near_x(3, 211)
     rng = np.random.default_rng(7)
     hashes = np.concatenate([rng.integers(0, 2 ** 63, 3000, dtype=np.uint64),
