@@ -1495,16 +1495,17 @@ __global__ __launch_bounds__(kBinL1Block) void bin_l1_kernel(Reads R, Tables T, 
 
 // Second pass: work item (d1, group) = the d1 runs of kBinGroup consecutive
 // first-pass tiles, concatenated in tile order.
-constexpr int kRunsPerThread = kBinGroup / kBlock;
-
 struct BinRuns {
     uint32_t rstart[kBinGroup];       // position in ent1 of each tile's run
     uint32_t rpre[kBinGroup + 1];     // exclusive prefix of the run lengths
     uint32_t part[kBlock];
 };
 
-// Loads the runs of item (d1, grp); returns its event count.
+// Loads the runs of item (d1, grp) with BLK threads; returns its event count.
+template <int BLK = kBlock>
 __device__ uint32_t bin_load_runs(const BinPlan& P, uint32_t d1, uint32_t grp, BinRuns& sh) {
+    constexpr int kRunsPerThread = kBinGroup / BLK;
+    static_assert(BLK / 64 <= kBlock, "wave totals fit the runs' part array");
     uint32_t len[kRunsPerThread];
     uint32_t tot = 0;
 #pragma unroll
@@ -1522,13 +1523,13 @@ __device__ uint32_t bin_load_runs(const BinPlan& P, uint32_t d1, uint32_t grp, B
         sh.rstart[j] = st;
         tot += len[k];
     }
-    uint32_t run = block_exclusive_u32(tot, sh.part);
+    uint32_t run = block_exclusive_u32<BLK>(tot, sh.part);
 #pragma unroll
     for (int k = 0; k < kRunsPerThread; ++k) {
         sh.rpre[threadIdx.x * kRunsPerThread + k] = run;
         run += len[k];
     }
-    if (threadIdx.x == kBlock - 1) sh.rpre[kBinGroup] = run;
+    if (threadIdx.x == BLK - 1) sh.rpre[kBinGroup] = run;
     __syncthreads();
     return sh.rpre[kBinGroup];
 }
@@ -1546,9 +1547,9 @@ __device__ __forceinline__ uint32_t bin_run_of(const BinRuns& sh, uint32_t i) {
 // Events [c0, c0 + m) of the loaded item, m <= kBinChunk, are found with one
 // binary search per 64 events (into `first`, by threads 0..63) and a forward
 // step from there per lane; lanes read consecutive events.
-template <int CH = kBinChunk>
+template <int CH = kBinChunk, int BLK = kBlock>
 __device__ __forceinline__ void bin_chunk_first(const BinRuns& sh, uint32_t* first, uint32_t c0, uint32_t m) {
-    static_assert(CH / 64 <= kBlock, "one thread per 64 events of a chunk");
+    static_assert(CH / 64 <= BLK, "one thread per 64 events of a chunk");
     if (threadIdx.x < (unsigned)(CH / 64) && threadIdx.x * 64u < m) first[threadIdx.x] = bin_run_of(sh, c0 + threadIdx.x * 64u);
 }
 
@@ -1578,11 +1579,12 @@ __device__ __forceinline__ uint32_t bin_d2(uint64_t e, uint32_t mask2) {
 // loads are issued together and stay in flight until d is read.
 constexpr uint32_t kNoDigit = 0xFFFFFFFFu;
 
+template <int BLK = kBlock>
 __device__ __forceinline__ void bin_chunk_load_d2(const BinPlan& P, const BinRuns& sh, const uint32_t* first,
                                                   uint32_t c0, uint32_t m, uint32_t* d) {
 #pragma unroll
-    for (int q = 0; q < kBinChunk / kBlock; ++q) {
-        const uint32_t j = threadIdx.x + (uint32_t)q * kBlock;
+    for (int q = 0; q < kBinChunk / BLK; ++q) {
+        const uint32_t j = threadIdx.x + (uint32_t)q * BLK;
         d[q] = kNoDigit;
         if (j < m) {
             const uint32_t i = c0 + j;
@@ -1599,11 +1601,16 @@ __device__ __forceinline__ void bin_chunk_load_d2(const BinPlan& P, const BinRun
 // each second digit's counts for them (adjacent in cnt2) go out as one write.
 constexpr uint32_t kCountGroups = 8;
 
-__global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
+// 512 threads: the count's chains of dependent loads (run offsets, then
+// digits) overlap across eight waves per workgroup.
+constexpr int kCountBlock = 2 * kBlock;
+
+__global__ __launch_bounds__(kCountBlock) void bin_l2_count_kernel(BinPlan P) {
+    constexpr int BLK = kCountBlock;
     __shared__ BinRuns sh;
     __shared__ uint32_t hist[kCountGroups][kDigits2];
     __shared__ uint32_t first[kBinChunk / 64];
-    constexpr int kQ = kBinChunk / kBlock;
+    constexpr int kQ = kBinChunk / BLK;
     const uint32_t g_blocks = (P.n_groups + kCountGroups - 1) / kCountGroups;
     const uint32_t n_blocks = P.n_d1 * g_blocks;
     const uint32_t mask2 = (1u << P.shift2) - 1u;
@@ -1612,17 +1619,17 @@ __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
         const uint32_t d1 = blk / g_blocks, grp0 = (blk % g_blocks) * kCountGroups;
         const uint32_t ng = P.n_groups - grp0 < kCountGroups ? P.n_groups - grp0 : kCountGroups;
         __syncthreads();
-        for (uint32_t x = threadIdx.x; x < kCountGroups * (uint32_t)kDigits2; x += kBlock) (&hist[0][0])[x] = 0;
+        for (uint32_t x = threadIdx.x; x < kCountGroups * (uint32_t)kDigits2; x += BLK) (&hist[0][0])[x] = 0;
         for (uint32_t gi = 0; gi < ng; ++gi) {
             uint32_t* h = hist[gi];
-            const uint32_t n = bin_load_runs(P, d1, grp0 + gi, sh);
+            const uint32_t n = bin_load_runs<BLK>(P, d1, grp0 + gi, sh);
             auto chunk_len = [&](uint32_t c) {
                 return c < n ? (n - c < (uint32_t)kBinChunk ? n - c : (uint32_t)kBinChunk) : 0u;
             };
             auto prefetch = [&](uint32_t c, uint32_t (&d)[kQ]) {
-                bin_chunk_first(sh, first, c, chunk_len(c));
+                bin_chunk_first<kBinChunk, BLK>(sh, first, c, chunk_len(c));
                 __syncthreads();
-                bin_chunk_load_d2(P, sh, first, c, chunk_len(c), d);
+                bin_chunk_load_d2<BLK>(P, sh, first, c, chunk_len(c), d);
                 __syncthreads();                           // `first` is free again
             };
             prefetch(0, da);
@@ -1640,7 +1647,7 @@ __global__ __launch_bounds__(kBlock) void bin_l2_count_kernel(BinPlan P) {
             __syncthreads();                               // bin_load_runs of the next group reuses sh
         }
         // lanes take (digit, group) pairs group-fastest: a digit's groups are adjacent words of cnt2
-        for (uint32_t x = threadIdx.x; x < kCountGroups * (mask2 + 1u); x += kBlock) {
+        for (uint32_t x = threadIdx.x; x < kCountGroups * (mask2 + 1u); x += BLK) {
             const uint32_t d2 = x / kCountGroups, gi = x % kCountGroups;
             if (gi < ng) P.cnt2[(size_t)((d1 << P.shift2) | d2) * P.n_groups + grp0 + gi] = hist[gi][d2];
         }
@@ -2698,7 +2705,7 @@ void launch_bin_l1(const Reads& R, const Tables& T, const BinPlan& P, int grid, 
 void launch_bin_l2_count(const BinPlan& P, int grid, hipStream_t s) {
     const int64_t blocks = (int64_t)P.n_d1 * ((P.n_groups + kCountGroups - 1) / kCountGroups);
     const int g = (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), grid);
-    bin_l2_count_kernel<<<g, kBlock, 0, s>>>(P);
+    bin_l2_count_kernel<<<g, kCountBlock, 0, s>>>(P);
 }
 
 void launch_bin_l2_scatter(const BinPlan& P, int grid, hipStream_t s) {
