@@ -31,6 +31,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -792,7 +793,7 @@ struct gt_inserter {
     hipStream_t stream = nullptr;          // inserts
     hipStream_t copy = nullptr;            // host-to-device copies
     bool pending_work = false;             // inserts queued since the last synchronisation
-    bool holding = false;                  // registered with the storage as holder of taken reads
+    std::atomic<bool> holding{false};      // registered with the storage as holder of taken reads (set under its lock)
 
     ~gt_inserter() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -1037,8 +1038,11 @@ int gt_inserter_free(gt_inserter* ins) {
     if (!ins) return GT_OK;
     int rc = GT_OK;
     if (ins->holding) {
+        // checked again under the storage's lock: another thread's call on
+        // the storage may have flushed the held reads meanwhile (a storage
+        // freed first has flushed them and cleared the flag)
         StorageGuard g(gt::storage_mutex(ins->st));
-        rc = queue_taken(ins, ins->slot[ins->active]);
+        if (ins->holding) rc = queue_taken(ins, ins->slot[ins->active]);
     }
     delete ins;
     return rc;

@@ -382,6 +382,35 @@ def test_binned_insert_every_k_vs_oracle(oracle, K, monkeypatch):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize('n_tables', [1, 2, 3])
+@pytest.mark.parametrize('K', [5, 31])
+def test_binned_insert_fewer_tables_vs_oracle(oracle, n_tables, K, monkeypatch):
+    """The binned plain insert on 1 to 3 tables (a first-pass tile then holds
+    2048 to 6144 events) over several sub-batches, with hot repeats: tables,
+    per-read n_new, flags and counters equal the sequential oracle's."""
+    monkeypatch.setenv('GOETIA_AMD_BINNED', '1')
+    monkeypatch.setenv('GOETIA_AMD_SUB_BATCH', str(8192 * 3))
+    rng = np.random.default_rng(50 * n_tables + K)
+    reads = [random_dna(rng, int(rng.integers(0, 700))).encode() for _ in range(300)]
+    reads += [b'ACGTTGCA' * 40] * 30 + [b'A' * 300] * 20
+    for kind in (1, 2):
+        sizes = oracle.primes_near_x(n_tables, 300_000 if kind == 2 else 900_000)
+        store = (lg.ByteStorage if kind == 2 else lg.BitStorage)(sizes)
+        g = lg.dBG[type(store), lg.CanLemireShifter].build(store, K)
+        n_new, flags = g.insert_sequences(ReadBatch.from_strings(reads), new_flags=True)
+        o = oracle.Storage(kind, sizes)
+        o_new, o_flags = [], []
+        for r in reads:
+            _, nn, fl = o.insert_sequence(r, K, True, flags=True)
+            o_new.append(nn)
+            o_flags.append(fl)
+        np.testing.assert_array_equal(n_new, o_new)
+        np.testing.assert_array_equal(flags, np.concatenate(o_flags).astype(bool))
+        assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+        for a, b in zip(g.get_raw(), o.tables()):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize('sub_batch', [0, 8192 * 3])
 @pytest.mark.parametrize('n_tables', [1, 3, 4, 5, 16])
 def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatch):
