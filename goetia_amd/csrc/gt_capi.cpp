@@ -233,7 +233,7 @@ struct gt_storage {
     DevBuf tables;
     DevBuf counters;                   // kNCounters u64
     // exact set
-    DevBuf ekeys, estamp, eones;
+    DevBuf ekv, eones;                     // exact set: key / stamp word pairs, all-ones flag
     uint64_t ecap = 0;
     uint64_t ecount_ub = 0;            // host upper bound on the set size
     Scratch own;                       // default scratch of calls on this storage
@@ -275,8 +275,7 @@ struct gt_storage {
     }
     ExactSet exact_view() const {
         ExactSet E{};
-        E.keys = ekeys.as<uint64_t>();
-        E.stamp = estamp.as<uint64_t>();
+        E.kv = ekv.as<uint64_t>();
         E.ones = eones.as<uint32_t>();
         E.cap = ecap;
         E.log2cap = log2u(ecap);
@@ -330,27 +329,22 @@ namespace {
 bool valid_storage(const gt_storage* st) { return st && st->kind >= kBit && st->kind <= kExact; }
 
 int alloc_exact(gt_storage* st, uint64_t cap, hipStream_t s) {
-    DevBuf keys, stamp;
-    GT_TRY(keys.ensure((cap + 1) * sizeof(uint64_t)));
-    GT_TRY(stamp.ensure((cap + 1) * sizeof(uint64_t)));
-    launch_fill_u64(keys.as<uint64_t>(), cap + 1, ~0ULL, s);
-    launch_fill_u64(stamp.as<uint64_t>(), cap + 1, 0, s);
+    DevBuf kv;
+    GT_TRY(kv.ensure(2 * (cap + 1) * sizeof(uint64_t)));
+    launch_fill_u64(kv.as<uint64_t>(), 2 * (cap + 1), ~0ULL, s);
     GT_HIP(hipGetLastError());
     if (st->ecap) {
         ExactSet from = st->exact_view();
         ExactSet to = from;
-        to.keys = keys.as<uint64_t>();
-        to.stamp = stamp.as<uint64_t>();
+        to.kv = kv.as<uint64_t>();
         to.cap = cap;
         to.log2cap = log2u(cap);
         launch_exact_rehash(from, to, st->counters.as<uint64_t>(), s);
         GT_HIP(hipGetLastError());
         GT_HIP(hipStreamSynchronize(s));
     }
-    std::swap(st->ekeys.p, keys.p);
-    std::swap(st->ekeys.bytes, keys.bytes);
-    std::swap(st->estamp.p, stamp.p);
-    std::swap(st->estamp.bytes, stamp.bytes);
+    std::swap(st->ekv.p, kv.p);
+    std::swap(st->ekv.bytes, kv.bytes);
     st->ecap = cap;
     st->epoch++;
     return GT_OK;
@@ -358,8 +352,7 @@ int alloc_exact(gt_storage* st, uint64_t cap, hipStream_t s) {
 
 int zero_tables(gt_storage* st, hipStream_t s) {
     if (st->kind == kExact) {
-        launch_fill_u64(st->ekeys.as<uint64_t>(), st->ecap + 1, ~0ULL, s);
-        launch_fill_u64(st->estamp.as<uint64_t>(), st->ecap + 1, 0, s);
+        launch_fill_u64(st->ekv.as<uint64_t>(), 2 * (st->ecap + 1), ~0ULL, s);
         launch_fill_u64(st->eones.as<uint64_t>(), 1, 0, s);
     } else {
         launch_fill_u64(st->tables.as<uint64_t>(), st->tables.bytes / 8, 0, s);

@@ -52,13 +52,19 @@ struct Tables {
 // slots.  keys[cap] is the slot of the all-ones key (which is also the empty
 // marker); flag[0] says whether that key is present.  stamp[] carries the
 // first-touch stamp of keys created in the current batch.
+// Exact set: open addressing with linear probing over cap slots (a power of
+// two); slot i is the word pair kv[2i] (its key, empty = ~0) and kv[2i + 1]
+// (the inverted first-touch stamp of the batch being inserted, ~0 = none), so
+// a key and its stamp share a cache line.  Slot cap holds the all-ones key's
+// stamp.  A new table is all ~0.
 struct ExactSet {
-    uint64_t* keys;    // cap + 1 entries, empty = ~0
-    uint64_t* stamp;   // cap + 1 entries
+    uint64_t* kv;      // 2 (cap + 1) words
     uint32_t* ones;    // presence flag of the all-ones key
     uint64_t cap;
     uint32_t log2cap;
 };
+__host__ __device__ inline uint64_t* exact_key(const ExactSet& es, uint64_t slot) { return es.kv + 2 * slot; }
+__host__ __device__ inline uint64_t* exact_stamp(const ExactSet& es, uint64_t slot) { return es.kv + 2 * slot + 1; }
 
 // First-touch claim map for Bloom / count-min batches: key word =
 // gen:28 | table:4 | bin:32, value word = gen:32 | ~ordinal:32 (atomicMax).

@@ -143,7 +143,7 @@ __device__ uint64_t exact_find(const ExactSet& es, uint64_t key) {
     const uint64_t mask = es.cap - 1;
     uint64_t slot = exact_home(es, key);
     for (uint64_t probes = 0; probes < es.cap; ++probes) {
-        uint64_t cur = es.keys[slot];
+        uint64_t cur = *exact_key(es, slot);
         if (cur == key) return slot;
         if (cur == kEmpty) return kEmpty;
         slot = (slot + 1) & mask;
@@ -160,10 +160,10 @@ __device__ uint64_t exact_insert(const ExactSet& es, uint64_t key, uint64_t* err
     const uint64_t mask = es.cap - 1;
     uint64_t slot = exact_home(es, key);
     for (uint64_t probes = 0; probes < es.cap; ++probes) {
-        uint64_t cur = load_relaxed(&es.keys[slot]);
+        uint64_t cur = load_relaxed(exact_key(es, slot));
         if (cur == key) return slot;
         if (cur == kEmpty) {
-            uint64_t prev = atomicCAS((unsigned long long*)&es.keys[slot],
+            uint64_t prev = atomicCAS((unsigned long long*)exact_key(es, slot),
                                       (unsigned long long)kEmpty, (unsigned long long)key);
             if (prev == kEmpty || prev == key) return slot;
         }
@@ -613,7 +613,10 @@ struct ExactLookupOp {
     __device__ void flush() {}
 };
 
-// Insert pass: fresh windows insert their value and stamp its slot.
+// Insert pass: fresh windows insert their value and stamp its slot.  The
+// slot's stamp word is kept inverted (a new table is all ~0): an atomicMin of
+// ~(gen:32 | ~ordinal:32) keeps the largest stamp, which is the current
+// batch's smallest ordinal.
 struct ExactInsertOp {
     ExactSet E;
     const uint8_t* fresh;
@@ -624,17 +627,17 @@ struct ExactInsertOp {
     __device__ void operator()(int64_t s, uint32_t, int64_t, uint64_t fw, uint64_t rc, bool valid) {
         if (!valid || !fresh[s - b0]) return;
         const uint64_t slot = exact_insert(E, kmer_value(fw, rc, canonical), errors);
-        if (slot != kEmpty) {
-            atomicMax((unsigned long long*)&E.stamp[slot],
-                      (unsigned long long)claim_stamp(gen, (uint32_t)(s - b0)));
-        }
+        if (slot != kEmpty)
+            atomicMin((unsigned long long*)exact_stamp(E, slot),
+                      (unsigned long long)~claim_stamp(gen, (uint32_t)(s - b0)));
     }
     __device__ void begin_tile(int64_t) {}
     __device__ void flush() {}
 };
 
-// Resolve pass: the fresh window holding the smallest stamp of its slot is
-// the one goetia's sequential insert reports as new (sparseppstorage.hh:92-97).
+// Resolve pass: the fresh window whose stamp its slot kept (the batch's
+// smallest ordinal on that key) is the one goetia's sequential insert reports
+// as new (sparseppstorage.hh:92-97).
 struct ExactResolveOp {
     ExactSet E;
     const uint8_t* fresh;
@@ -650,7 +653,7 @@ struct ExactResolveOp {
         bool is_new = false;
         if (valid && fresh[s - b0]) {
             const uint64_t slot = exact_find(E, kmer_value(fw, rc, canonical));
-            is_new = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)(s - b0));
+            is_new = slot != kEmpty && *exact_stamp(E, slot) == ~claim_stamp(gen, (uint32_t)(s - b0));
         }
         if (valid) last_ins = (uint64_t)(s - b0) + 1;
         if (is_new) last_new = (uint64_t)(s - b0) + 1;
@@ -983,9 +986,8 @@ __global__ __launch_bounds__(kBlock) void hash_exact_insert_kernel(const uint64_
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         if (!fresh[i]) continue;
         const uint64_t slot = exact_insert(E, h[i], counters + kErrors);
-        if (slot != kEmpty) {
-            atomicMax((unsigned long long*)&E.stamp[slot], (unsigned long long)claim_stamp(gen, (uint32_t)i));
-        }
+        if (slot != kEmpty)
+            atomicMin((unsigned long long*)exact_stamp(E, slot), (unsigned long long)~claim_stamp(gen, (uint32_t)i));
     }
 }
 
@@ -998,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void hash_exact_resolve_kernel(const uint64
         bool nw = false;
         if (fresh[i]) {
             const uint64_t slot = exact_find(E, h[i]);
-            nw = slot != kEmpty && E.stamp[slot] == claim_stamp(gen, (uint32_t)i);
+            nw = slot != kEmpty && *exact_stamp(E, slot) == ~claim_stamp(gen, (uint32_t)i);
         }
         nn += nw;
         last_ins = i + 1;
@@ -1160,7 +1162,7 @@ __global__ void bit_union_kernel(uint32_t* me, const uint32_t* other, uint64_t n
 __global__ void exact_rehash_kernel(ExactSet from, ExactSet to, uint64_t* counters) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < from.cap;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = from.keys[i];
+        const uint64_t k = *exact_key(from, i);
         if (k != kEmpty) exact_insert(to, k, counters + kErrors);
     }
 }
@@ -1169,7 +1171,7 @@ __global__ void exact_rehash_kernel(ExactSet from, ExactSet to, uint64_t* counte
 __global__ void exact_keys_kernel(ExactSet E, uint64_t* out, uint64_t cap_out, unsigned long long* n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < E.cap;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = E.keys[i];
+        const uint64_t k = *exact_key(E, i);
         if (k != kEmpty) {
             const unsigned long long j = atomicAdd(n, 1ULL);
             if (j < cap_out) out[j] = k;
