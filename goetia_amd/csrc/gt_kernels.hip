@@ -1200,6 +1200,16 @@ __global__ void exact_keys_kernel(ExactSet E, uint64_t* out, uint64_t cap_out, u
 // bitstorage.hh:195-219): the count becomes min(255, count + hits), an event
 // is new iff some table's bin held 0 just before it, and table-0 bins going
 // from 0 to nonzero count as occupied.
+//
+// A plain insert does not need event order inside a bucket: a bin ends at
+// min(255, before + hits) (set, for Bloom) and the k-mer new through it is
+// its event with the smallest window ordinal, when the bin was empty before
+// the batch.  Plain inserts therefore take order-free variants: both passes
+// rank an event within its digit by an LDS atomicAdd (bin_l1_kernel<true>,
+// bin_l2_scatter_u_kernel), and bin_apply_u_kernel replays a bucket per
+// workgroup with an LDS atomicMin of the ordinal and an atomicAdd of one hit
+// per bin, then settles each bin from its smallest-ordinal event.
+// insert_and_query and diginorm keep the ordered kernels.
 
 // Mask of the lanes whose v equals this lane's (all 64 lanes must be active).
 template <int BITS>
