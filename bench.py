@@ -231,6 +231,14 @@ def main():
     log('kernel ms/step: ' + ', '.join(f'{k} {v[1] / args.steps:.1f} ({v[0]} launches)' for k, v in kt.items()))
 
     value = n_kmers * args.steps / elapsed
+    # the insert pass and the query pass separately (N_kmers / T per pass, from
+    # the kernels' HIP-event times; the step's wall clock adds the table reset)
+    ins_ms = sum(v[1] for k, v in kt.items() if k != 'query') / args.steps
+    q_ms = kt['query'][1] / args.steps if 'query' in kt else None
+    passes = {'insert_ms': round(ins_ms, 2), 'query_ms': round(q_ms, 2) if q_ms else None,
+              'insert_kmers_per_s': round(n_kmers / (ins_ms / 1e3), 1) if ins_ms else None,
+              'query_kmers_per_s': round(n_kmers / (q_ms / 1e3), 1) if q_ms else None,
+              'source': 'HIP events around each kernel, per step'}
     dom = max(kt, key=lambda k: kt[k][1])
     dom_ms = kt[dom][1] / args.steps
     achieved = n_kmers * BYTES_PER_KMER[dom] / (dom_ms / 1e3) / 1e9
@@ -331,6 +339,7 @@ def main():
                    'reads': batch.n_reads, 'kmers_per_step': n_kmers, 'K': K,
                    'storage': 'ByteStorage', 'tablesizes': tablesizes},
         'roofline': roofline,
+        'passes': passes,
         'insert_and_query_kmers_per_s': iq_rate,
         'chunk_latency': chunk,
         'vs_plain': served['served_over_plain'] if served else None,
