@@ -91,8 +91,7 @@ constexpr int kDigitBits1 = 8;
 constexpr int kDigits1 = 1 << kDigitBits1;        // first-pass fan-out
 constexpr int kDigitBits2 = 9;
 constexpr int kDigits2 = 1 << kDigitBits2;        // largest second-pass fan-out
-constexpr int kBinPer = 8;                        // windows per thread in the first pass
-constexpr int kBinTile = kBlock * kBinPer;        // starts per first-pass tile
+constexpr int kBinTile = 2048;                    // starts per first-pass tile (512 threads x 4)
 constexpr uint32_t kBinMaxTables = 4;
 constexpr uint32_t kBinGroup = 1024;              // first-pass tiles per second-pass work item
 constexpr int kBinChunk = 4096;                   // events staged in LDS per second-pass step
