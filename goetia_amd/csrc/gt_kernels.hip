@@ -258,9 +258,8 @@ __device__ __forceinline__ void call_end_tile(Op&, int64_t, long) {}
 // (XOR_i rotl(T[comp(b_i)], i) over them).  Every thread computes its own
 // segment (the first m-1 threads also the segments past the tile), the
 // values meet in LDS, and each thread combines m of them instead of rolling
-// K bytes twice.  kSegExtra bounds m - 1: every K <= 255 at PER >= 8, and
-// K <= 132 at PER = 4 (larger K roll their starts).
-constexpr int kSegExtra = 32;
+// K bytes twice.  kSegExtra bounds m - 1 (K <= 255, PER >= 4).
+constexpr int kSegExtra = 64;
 template <int N>
 struct SegScratchT {
     static constexpr int kSegs = N;
@@ -369,7 +368,7 @@ __device__ void walk_tiles(const Reads& R, Sh& sh, Op& op) {
         const int n_seg = (K + PER - 1) / PER;
         if (sg) {
             static_assert(std::remove_pointer_t<decltype(sg)>::kSegs >= BLK + kSegExtra, "segment scratch too small");
-            if (n_seg - 1 > kSegExtra) sg = nullptr;       // uniform: K too long to stitch at this PER
+            if (n_seg - 1 > kSegExtra) sg = nullptr;       // uniform: K too long to stitch at this PER (none here)
         }
         if (sg) {
             const int L = K - (n_seg - 1) * PER;

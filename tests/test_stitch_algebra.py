@@ -2,10 +2,10 @@
 gt_kernels.hip) equal CyclicHash rolled over the K bytes
 (cyclichash.h:85-108: fw = XOR_k rotl(T[b_k], K-1-k); the reverse-complement
 hash rc = XOR_k rotl(T[comp(b_k)], k)), for every K the library takes and the
-segments of the first binned pass (8 bytes for the ordered sort, 4 bytes for
-plain inserts, which stitch K <= 132 and roll longer K), with the rotation
-amounts reduced as the kernel reduces them (an unsigned 32-bit value masked
-to 6 bits)."""
+segments the walker stitches from (4 bytes, as the first binned pass uses
+them, and 8 bytes, which the walker also supports), with the rotation amounts
+reduced as the kernel reduces them (an unsigned 32-bit value masked to 6
+bits)."""
 import random
 
 import pytest
@@ -30,7 +30,7 @@ def seg_values(by, u, L, T, TR, PER):
 
 
 CASES = ([(8, K) for K in list(range(1, 34)) + [63, 64, 65, 127, 128, 200, 254, 255]]
-         + [(4, K) for K in list(range(1, 34)) + [63, 64, 65, 127, 128, 131, 132]])
+         + [(4, K) for K in list(range(1, 34)) + [63, 64, 65, 127, 128, 131, 132, 133, 200, 254, 255]])
 
 
 @pytest.mark.parametrize('PER,K', CASES)
@@ -42,7 +42,7 @@ def test_stitched_start_equals_rolled(PER, K):
     L = K - (n_seg - 1) * PER
     n_threads = 40
     by = [rng.randrange(256) for _ in range(PER * (n_threads + n_seg))]
-    assert n_seg - 1 <= 32          # the kernel's segment scratch holds 32 extra segments
+    assert n_seg - 1 <= 64          # the kernel's segment scratch holds 64 extra segments
     S = [seg_values(by, u, L, T, TR, PER) for u in range(n_threads + n_seg - 1)]
     for t in range(n_threads):
         li = t * PER
