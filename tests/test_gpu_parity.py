@@ -437,6 +437,34 @@ def test_byte_insert_and_query_vs_oracle(oracle, sub_batch, n_tables, monkeypatc
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize('K', [1, 4, 5, 31, 64, 133, 255])
+def test_byte_insert_and_query_every_k_vs_oracle(oracle, K):
+    """Count-min insert_and_query through the ordered binned passes for K from
+    1 to 255 (the ordered first pass stitches its window starts from 4-byte
+    segments for every K), on reads of every length around K with N bases
+    and hot repeats: every window's count, counters and tables equal the
+    sequential oracle's."""
+    rng = np.random.default_rng(2000 + K)
+    reads = []
+    for _ in range(300):
+        L = int(rng.choice([0, K - 1, K, K + 1, K + 9, 150, 600, 2000]))
+        r = bytearray(random_dna(rng, max(L, 0)).encode())
+        for j in np.nonzero(rng.random(len(r)) < 0.003)[0]:
+            r[j] = ord('N')
+        reads.append(bytes(r))
+    reads += [b'A' * (K + 300)] * 3 + [b'ACGT' * 80] * 20
+    sizes = oracle.primes_near_x(4, 150_000)
+    g = lg.dBG[lg.ByteStorage, lg.CanLemireShifter].build(lg.ByteStorage(sizes), K)
+    g.mode = N.MODE_SKIP_N
+    got = g.insert_and_query_sequences(ReadBatch.from_strings(reads))
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp = np.concatenate([o.insert_and_query_sequence(r, K, True, oracle.MODE_SKIP_N) for r in reads])
+    np.testing.assert_array_equal(got, exp)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_byte_insert_and_query_large_table_vs_oracle(oracle):
     """A count-min storage with more than 2^30 bins (one table of 2^30 + 1
     bins beside a small one) takes the sort-based insert_and_query: results,
