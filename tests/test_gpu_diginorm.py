@@ -73,6 +73,25 @@ def test_vs_oracle_high_coverage(oracle, cutoff):
     assert 0.05 < exp.mean() < 0.95
 
 
+@pytest.mark.parametrize('K', [5, 64, 133, 255])
+def test_every_k_vs_oracle(oracle, K):
+    """Diginorm through the ordered binned passes at K from 5 to 255 (window
+    starts stitched from 4-byte segments for every K) on high-coverage
+    reads: pass flags, tables and counters equal the sequential oracle's."""
+    genome = workload.genome(90 + K, 6_000)
+    gen = workload.reads(91 + K, genome, 3000, K + 20, K + 400, 0.005, 0.0)
+    batch = ReadBatch.from_strings([gen.read(r) for r in range(gen.n_reads)])
+    sizes = oracle.primes_near_x(4, 100_003)
+    o = oracle.Storage(oracle.BYTE, sizes)
+    exp = o.diginorm_reads(batch.seq, batch.offsets, K, 5)
+    g = graph(sizes, K)
+    got = lg.DiginormFilter.build(g, 5).filter_sequences(batch)
+    np.testing.assert_array_equal(got, exp)
+    assert (g.n_unique(), g.n_occupied()) == (o.n_unique, o.n_occupied)
+    for a, b in zip(g.get_raw(), o.tables()):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_skip_n_mode_vs_oracle(oracle):
     """Skip-N input policy: windows holding a non-ACGT byte are neither
     counted nor inserted (min_req still counts them)."""
