@@ -208,7 +208,13 @@ def test_serving_results_same_alone_and_in_mix(slots, policy):
     sv.drain()
     st = sv.stats()
     assert st['completed'] == len(chunks) and st['resident_slots'] == 2
-    assert st['evictions'] > 0 and st['loads'] > 0   # the budget forces evictions
+    # the budget (2 of 6 streams resident) forces evictions; in arrival order
+    # (policy 0) the interleaved chunks also force reloads, while the
+    # resident-first policy may finish each stream before evicting it when
+    # every chunk is queued before the worker starts
+    assert st['evictions'] > 0
+    if policy == 0:
+        assert st['loads'] > 0
     assert all(x >= 0 for x in lat)
     # each stream alone, its chunks in order, through the plain dBG path and
     # through the oracle
